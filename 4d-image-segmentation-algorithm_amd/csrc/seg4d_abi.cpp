@@ -1,0 +1,445 @@
+// seg4d_abi.cpp -- the segmentation program's C ABI (include/seg4d.h) on top
+// of the HIP engine.  Globals and entry points keep the reference's names
+// and meaning (segmentation4D.c:13-60, segmentation4D.h:8-62).
+#include "seg4d.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "seg4d_engine.h"
+
+#define API extern "C" __attribute__((visibility("default")))
+
+// ---- reference globals (segmentation4D.c:13-60) ----------------------------
+API int kMax = 0, iMax = 0, jMax = 0, lMax = 0;
+API double *u = nullptr, *prev_u = nullptr, *i0 = nullptr, *Mmask = nullptr;
+API double *e = nullptr, *w = nullptr, *n = nullptr, *s = nullptr;
+API double *t = nullptr, *b = nullptr, *fw = nullptr, *bw = nullptr;
+API double *g_e = nullptr, *g_w = nullptr, *g_n = nullptr, *g_s = nullptr;
+API double *g_t = nullptr, *g_b = nullptr, *g_fw = nullptr, *g_bw = nullptr;
+API int* no_of_centers = nullptr;
+API double **center_x = nullptr, **center_y = nullptr, **center_z = nullptr;
+API double spacing_x = 0, spacing_y = 0, spacing_z = 0;
+API double origin_x = 0, origin_y = 0, origin_z = 0;
+API int maxNoGSIteration = 0;
+API double eps2 = 0, K = 0;
+API int numberOfTimeStep = 0, maxNoOfTimeSteps = 0;
+API double segTol = 0;
+API double tau = 0, h = 0, omega_c = 0, gsTol = 0, omega_sigma = 0;
+API double theta_img = 0, rho_thr = 0, alpha = 0, beta = 0;
+API int mod = 0;
+API double sigma = 0;
+API double maxNoGSIteration_sigma = 0;
+API int timeStepsNum = 0;
+API double ballRadius = 0;
+
+namespace {
+
+s4d::Engine* g_eng = nullptr;
+int g_resident = 0;
+int g_verbose = 1;
+double g_rel = 0.0;
+bool g_g_dirty = true;       // host g_* not yet on the device
+int g_centre_frames = 0;     // frames the centre arrays were allocated for
+s4d::SolveStats g_last;
+
+double** coef_ptrs[8] = {&e, &w, &n, &s, &t, &b, &fw, &bw};
+double** g_ptrs[8] = {&g_e, &g_w, &g_n, &g_s, &g_t, &g_b, &g_fw, &g_bw};
+
+long nelem() { return (long)iMax * jMax * kMax * lMax; }
+
+bool fail(const char* what) {
+    std::string msg = what;
+    if (g_eng && !g_eng->err.empty()) msg += " (" + g_eng->err + ")";
+    s4d::set_error(msg);
+    std::fprintf(stderr, "seg4d: %s\n", msg.c_str());
+    return false;
+}
+
+bool need_engine(const char* fn) {
+    if (g_eng && g_eng->ready()) return true;
+    std::string m = std::string(fn) + ": library not allocated (call _allocateMem_ first)";
+    s4d::set_error(m);
+    std::fprintf(stderr, "seg4d: %s\n", m.c_str());
+    return false;
+}
+
+// Library array id of a host pointer, -1 if foreign, -(10+f) for coef f.
+int resolve(const double* p) {
+    if (!p) return -1;
+    if (p == u) return s4d::A_U;
+    if (p == prev_u) return s4d::A_PREV;
+    if (p == i0) return s4d::A_I0;
+    if (p == Mmask) return s4d::A_MASK;
+    for (int f = 0; f < 8; f++)
+        if (p == *g_ptrs[f]) return s4d::A_G0 + f;
+    for (int f = 0; f < 8; f++)
+        if (p == *coef_ptrs[f]) return -(10 + f);
+    return -1;
+}
+
+s4d::StepParams step_params() {
+    s4d::StepParams P;
+    P.h = h;
+    P.tau = tau;
+    P.eps2 = eps2;
+    P.omega = omega_c;
+    P.gsTol = gsTol;
+    P.rel2 = g_rel * g_rel;
+    P.cap = maxNoGSIteration;
+    return P;
+}
+
+s4d::Balls collect_balls() {
+    s4d::Balls B;
+    B.radius = ballRadius;
+    B.count.assign(lMax, 0);
+    B.offset.assign(lMax, 0);
+    for (int l = 0; l < lMax; l++) {
+        int c = (no_of_centers && l < g_centre_frames) ? no_of_centers[l] : 0;
+        if (c < 0) c = 0;
+        B.offset[l] = (int)B.cx.size();
+        if (l >= 1 && l <= lMax - 2) {
+            B.count[l] = c;
+            for (int q = 0; q < c; q++) {
+                B.cx.push_back(center_x[l][q]);
+                B.cy.push_back(center_y[l][q]);
+                B.cz.push_back(center_z[l][q]);
+            }
+        }
+    }
+    B.total = (int)B.cx.size();
+    return B;
+}
+
+// Device array behind a host pointer.  Library arrays map to their mirror;
+// in mirror mode a foreign host array is staged in scratch `spare`.
+double* device_of(double* hp, s4d::ArrayId spare, bool upload) {
+    const int id = resolve(hp);
+    if (id >= 0) {
+        if (upload && !g_resident && g_eng->upload((s4d::ArrayId)id, hp)) return nullptr;
+        return g_eng->full((s4d::ArrayId)id);
+    }
+    if (g_resident || id <= -10) {
+        s4d::set_error("pointer is not a library array usable here");
+        return nullptr;
+    }
+    double* d = g_eng->scratch(spare);
+    if (!d) return nullptr;
+    if (upload && g_eng->upload(spare, hp)) return nullptr;
+    return d;
+}
+
+bool download_to(double* hp, const double* d) {
+    const int id = resolve(hp);
+    if (id >= 0) return g_eng->download((s4d::ArrayId)id, hp) == 0;
+    // staged foreign array: find which scratch holds it
+    for (s4d::ArrayId a : {s4d::A_TMP, s4d::A_TMP2})
+        if (g_eng->full(a) == d) return g_eng->download(a, hp) == 0;
+    return false;
+}
+
+bool upload_g_if_dirty() {
+    if (g_resident || !g_g_dirty) return true;
+    for (int f = 0; f < 8; f++)
+        if (g_eng->upload((s4d::ArrayId)(s4d::A_G0 + f), *g_ptrs[f])) return false;
+    g_g_dirty = false;
+    return true;
+}
+
+void free_host() {
+    double** all[] = {&u, &prev_u, &i0, &Mmask, &e, &w, &n, &s, &t, &b, &fw, &bw,
+                      &g_e, &g_w, &g_n, &g_s, &g_t, &g_b, &g_fw, &g_bw};
+    for (double** p : all) {
+        std::free(*p);
+        *p = nullptr;
+    }
+    double*** cs[] = {&center_x, &center_y, &center_z};
+    for (double*** c : cs) {
+        if (*c) {
+            for (int l = 0; l < g_centre_frames; l++) std::free((*c)[l]);
+            std::free(*c);
+        }
+        *c = nullptr;
+    }
+    g_centre_frames = 0;
+}
+
+}  // namespace
+
+// ---- segmentation4D.c:61-70 --------------------------------------------------
+API int ijkl(int i, int j, int k, int l) {
+    return l * iMax * jMax * kMax + k * iMax * jMax + j * iMax + i;
+}
+
+API double _gFunction_(double value, double coef) { return 1.0 / (1 + coef * value); }
+
+// ---- segmentation4D.c:71-192 -------------------------------------------------
+API bool _allocateMem_(void) {
+    if (iMax < 3 || jMax < 3 || kMax < 3 || lMax < 3) {
+        s4d::set_error("_allocateMem_: padded dims must be >= 3");
+        return false;
+    }
+    free_host();
+    const long N = nelem();
+    double** all[] = {&u, &prev_u, &i0, &Mmask, &e, &w, &n, &s, &t, &b, &fw, &bw,
+                      &g_e, &g_w, &g_n, &g_s, &g_t, &g_b, &g_fw, &g_bw};
+    for (double** p : all) {
+        // calloc: zero-filled like the reference's explicit loop (:106-133)
+        *p = (double*)std::calloc((size_t)N, sizeof(double));
+        if (!*p) {
+            free_host();
+            return fail("_allocateMem_: host allocation failed");
+        }
+    }
+    center_x = (double**)std::calloc((size_t)lMax, sizeof(double*));
+    center_y = (double**)std::calloc((size_t)lMax, sizeof(double*));
+    center_z = (double**)std::calloc((size_t)lMax, sizeof(double*));
+    if (!center_x || !center_y || !center_z) return fail("_allocateMem_: centre arrays");
+    g_centre_frames = lMax;
+    for (int l = 0; l < lMax; l++) {
+        int c = no_of_centers ? no_of_centers[l] : 0;
+        if (c < 1) c = 1;
+        center_x[l] = (double*)std::calloc((size_t)c, sizeof(double));
+        center_y[l] = (double*)std::calloc((size_t)c, sizeof(double));
+        center_z[l] = (double*)std::calloc((size_t)c, sizeof(double));
+        if (!center_x[l] || !center_y[l] || !center_z[l])
+            return fail("_allocateMem_: centre arrays");
+    }
+    if (!g_eng) g_eng = new s4d::Engine();
+    if (g_eng->init(iMax, jMax, kMax, lMax, true, false))
+        return fail("_allocateMem_: device allocation failed");
+    g_g_dirty = true;
+    return true;
+}
+
+API bool _deallocateMem_(void) {
+    free_host();
+    if (g_eng) {
+        g_eng->release();
+        delete g_eng;
+        g_eng = nullptr;
+    }
+    return true;
+}
+
+// ---- segmentation4D.c:194-213 ------------------------------------------------
+API double _l2norm_(double* a, double* bb) {
+    if (!a || !bb) return false;
+    if (!need_engine("_l2norm_")) return 0.0;
+    double* da = device_of(a, s4d::A_TMP, true);
+    double* db = device_of(bb, s4d::A_TMP2, true);
+    if (!da || !db) {
+        fail("_l2norm_: cannot resolve arguments");
+        return 0.0;
+    }
+    double r = 0.0;
+    if (g_eng->l2norm(da, db, h, &r)) {
+        fail("_l2norm_");
+        return 0.0;
+    }
+    return r;
+}
+
+// ---- segmentation4D.c:446-580 ------------------------------------------------
+API bool _gs_coefficients_(void) {
+    if (!need_engine("_gs_coefficients_")) return false;
+    if (!g_resident && g_eng->upload(s4d::A_U, u)) return fail("_gs_coefficients_: upload");
+    if (!upload_g_if_dirty()) return fail("_gs_coefficients_: upload g");
+    if (g_eng->setup(step_params(), true, false)) return fail("_gs_coefficients_");
+    if (!g_resident)
+        for (int f = 0; f < 8; f++)
+            if (g_eng->download_coef(f, *coef_ptrs[f])) return fail("_gs_coefficients_: download");
+    return g_eng->sync() == 0 ? true : fail("_gs_coefficients_");
+}
+
+// ---- segmentation4D.c:215-315 ------------------------------------------------
+API bool _segTimeStep_(void) {
+    if (!need_engine("_segTimeStep_")) return false;
+    if (!g_resident) {
+        if (g_eng->upload(s4d::A_U, u) || g_eng->upload(s4d::A_MASK, Mmask))
+            return fail("_segTimeStep_: upload");
+    }
+    if (!upload_g_if_dirty()) return fail("_segTimeStep_: upload g");
+    const s4d::Balls balls = collect_balls();
+    s4d::SolveStats st;
+    if (g_eng->seg_step(step_params(), &balls, &st)) return fail("_segTimeStep_");
+    g_last = st;
+    if (!g_resident) {
+        if (g_eng->download(s4d::A_U, u) || g_eng->download(s4d::A_PREV, prev_u))
+            return fail("_segTimeStep_: download");
+    }
+    if (g_verbose) {
+        std::printf("Step is %d\n", numberOfTimeStep);
+        std::printf("Mean square residue is %e and total number of RBSOR is %d\n", st.R, st.z);
+    }
+    if (!std::isfinite(st.R)) return fail("_segTimeStep_: non-finite SOR residual");
+    return true;
+}
+
+// ---- segmentation4D.c:337-409 ------------------------------------------------
+static bool presmooth_device(double* d) {
+    int z = 0;
+    double R = 0.0;
+    if (g_eng->presmooth(d, sigma, h, omega_sigma, gsTol, g_rel * g_rel,
+                         maxNoGSIteration_sigma, &z, &R))
+        return false;
+    g_last.z = z;
+    g_last.R = R;
+    if (g_verbose)
+        std::printf("Presmoothing mean square residue is %e and total number of RBSOR is %d\n",
+                    R, z);
+    return true;
+}
+
+API bool _presmoothStep_(double* a) {
+    if (!a) return false;
+    if (!need_engine("_presmoothStep_")) return false;
+    double* d = device_of(a, s4d::A_TMP, true);
+    if (!d) return fail("_presmoothStep_: cannot resolve argument");
+    if (!presmooth_device(d)) return fail("_presmoothStep_");
+    if (!g_resident) {
+        if (!download_to(a, d) || g_eng->download(s4d::A_PREV, prev_u))
+            return fail("_presmoothStep_: download");
+    }
+    if (!std::isfinite(g_last.R)) return fail("_presmoothStep_: non-finite residual");
+    return true;
+}
+
+// ---- segmentation4D.c:582-823 ------------------------------------------------
+API bool _gf_inputImage_plus_thresholdedImage_(double* a, double* bb) {
+    if (!a || !bb) return false;
+    if (!need_engine("_gf_inputImage_plus_thresholdedImage_")) return false;
+    double* da = device_of(a, s4d::A_TMP, true);
+    double* db = device_of(bb, s4d::A_TMP2, true);
+    if (!da || !db || da == db) return fail("_gf_: cannot resolve arguments");
+    if (!presmooth_device(da) || !presmooth_device(db)) return fail("_gf_: presmoothing");
+    if (g_eng->gfun(da, db, h, K, theta_img, rho_thr)) return fail("_gf_: edge indicator");
+    if (!g_resident) {
+        if (!download_to(a, da) || !download_to(bb, db) ||
+            g_eng->download(s4d::A_PREV, prev_u))
+            return fail("_gf_: download");
+        for (int f = 0; f < 8; f++)
+            if (g_eng->download((s4d::ArrayId)(s4d::A_G0 + f), *g_ptrs[f]))
+                return fail("_gf_: download g");
+    }
+    g_g_dirty = false;
+    return g_eng->sync() == 0 ? true : fail("_gf_");
+}
+
+// ---- segmentation4D.c:826-864 ------------------------------------------------
+API bool _reflection4Ddata_(double* a) {
+    if (!a) return false;
+    if (!need_engine("_reflection4Ddata_")) return false;
+    double* d = device_of(a, s4d::A_TMP, true);
+    if (!d) return fail("_reflection4Ddata_: cannot resolve argument");
+    if (g_eng->reflect(d)) return fail("_reflection4Ddata_");
+    if (!g_resident && !download_to(a, d)) return fail("_reflection4Ddata_: download");
+    return g_eng->sync() == 0 ? true : fail("_reflection4Ddata_");
+}
+
+// ---- segmentation4D.c:866-874 ------------------------------------------------
+API void _copy4DdataFromSrcToDest_(double* src, double* dst) {
+    if (!src || !dst) return;
+    const int ids = resolve(src), idd = resolve(dst);
+    if (!g_resident) {
+        std::memcpy(dst, src, (size_t)nelem() * sizeof(double));
+        if (idd >= s4d::A_G0 && idd <= s4d::A_G7) g_g_dirty = true;
+        return;
+    }
+    if (!need_engine("_copy4DdataFromSrcToDest_")) return;
+    if (ids < 0 || idd < 0 || g_eng->copy((s4d::ArrayId)ids, (s4d::ArrayId)idd) || g_eng->sync())
+        fail("_copy4DdataFromSrcToDest_: arrays must be library arrays in resident mode");
+}
+
+// ---- segmentation4D.c:1020-1104 ----------------------------------------------
+API bool _rescale4DdataToZeroOne_(double* a) {
+    if (!a) return false;
+    if (!need_engine("_rescale4DdataToZeroOne_")) return false;
+    double* d = device_of(a, s4d::A_TMP, true);
+    if (!d) return fail("_rescale4DdataToZeroOne_: cannot resolve argument");
+    if (!g_resident && g_eng->upload(s4d::A_MASK, Mmask)) return fail("_rescale: upload mask");
+    if (g_eng->rescale(d, collect_balls(), true)) return fail("_rescale4DdataToZeroOne_");
+    if (!g_resident && !download_to(a, d)) return fail("_rescale4DdataToZeroOne_: download");
+    return g_eng->sync() == 0 ? true : fail("_rescale4DdataToZeroOne_");
+}
+
+// ---- extensions ------------------------------------------------------------------
+API int seg4dSetResident(int on) {
+    const int old = g_resident;
+    g_resident = on ? 1 : 0;
+    return old;
+}
+
+API void seg4dSetRelativeTolerance(double rel) { g_rel = rel > 0.0 ? rel : 0.0; }
+
+API bool seg4dUpload(double* hp) {
+    if (!need_engine("seg4dUpload")) return false;
+    const int id = resolve(hp);
+    if (id < 0) return fail("seg4dUpload: not an uploadable library array");
+    if (g_eng->upload((s4d::ArrayId)id, hp)) return fail("seg4dUpload");
+    if (id >= s4d::A_G0 && id <= s4d::A_G7) {
+        // the g set is clean once all eight have been uploaded explicitly
+        g_g_dirty = false;
+    }
+    return true;
+}
+
+API bool seg4dDownload(double* hp) {
+    if (!need_engine("seg4dDownload")) return false;
+    const int id = resolve(hp);
+    if (id <= -10) return g_eng->download_coef(-(id + 10), hp) == 0 ? true : fail("seg4dDownload");
+    if (id < 0) return fail("seg4dDownload: not a library array");
+    return g_eng->download((s4d::ArrayId)id, hp) == 0 ? true : fail("seg4dDownload");
+}
+
+API void seg4dHostWrote(double* hp) {
+    const int id = resolve(hp);
+    if (id >= s4d::A_G0 && id <= s4d::A_G7) g_g_dirty = true;
+}
+
+API void seg4dSetVerbose(int on) { g_verbose = on; }
+API const char* seg4dLastError(void) { return s4d::last_error(); }
+API int seg4dLastSweeps(void) { return g_last.z; }
+API double seg4dLastResidual(void) { return g_last.R; }
+API double seg4dLastTolerance(void) { return g_last.tol; }
+
+API void seg4dProfile(int on) {
+    if (g_eng) g_eng->prof.on = on != 0;
+}
+
+API void seg4dProfileReset(void) {
+    if (g_eng) {
+        const bool on = g_eng->prof.on;
+        g_eng->prof = s4d::Profile();
+        g_eng->prof.on = on;
+    }
+}
+
+API int seg4dProfileRead(double* out, int nout) {
+    if (!g_eng || !out) return 0;
+    const s4d::Profile& p = g_eng->prof;
+    const double v[9] = {p.sweep_ms, (double)p.sweep_launches, p.sweep_bytes,
+                         p.setup_ms, (double)p.setup_launches, p.step_ms,
+                         (double)p.steps, (double)p.voxel_updates, (double)p.iterations};
+    int k = nout < 9 ? nout : 9;
+    for (int q = 0; q < k; q++) out[q] = v[q];
+    return k;
+}
+
+API bool seg4dSynchronize(void) {
+    if (!g_eng) return true;
+    return g_eng->sync() == 0;
+}
+
+API int seg4dDeviceInfo(char* buf, int len) {
+    hipDeviceProp_t prop;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return -1;
+    std::snprintf(buf, (size_t)len, "%s|%d CUs|%s|%.1f GB", prop.name, prop.multiProcessorCount,
+                  prop.gcnArchName, prop.totalGlobalMem / 1e9);
+    return 0;
+}

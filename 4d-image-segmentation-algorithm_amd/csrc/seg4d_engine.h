@@ -1,0 +1,147 @@
+// seg4d_engine.h -- device state and operations of one padded 4D volume.
+//
+// The engine owns every device buffer of the time step and launches the
+// kernels of seg4d_kernels.h on one HIP stream.  It knows nothing about the
+// reference's globals; the C-ABI layers (seg4d_abi.cpp, mcf4d_abi.cpp) map
+// those onto an engine.  All entry points return 0 on success and -1 on
+// failure with a message in Engine::err.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+namespace s4d {
+
+struct Geo;
+struct SorCtl;
+
+struct StepParams {
+    double h = 1.0, tau = 0.5, eps2 = 1e-6;
+    double omega = 1.8;     // SOR relaxation
+    double gsTol = 1e-12;   // absolute bound on sum r^2 h^4
+    double rel2 = 0.0;      // > 0: bound = rel2 * sum(prev^2 h^4) instead
+    long cap = 1000;        // sweep cap (maxNoGSIteration)
+};
+
+struct SolveStats {
+    int z = 0;              // SOR iterations performed (reference count)
+    double R = 0.0;         // residual at exit
+    double tol = 0.0;       // bound used
+    double sumsq = 0.0;     // sum(prev^2 h^4) over the interior
+};
+
+struct Balls {              // rescale-epilogue balls, host copies
+    std::vector<int> count, offset;
+    std::vector<double> cx, cy, cz;
+    double radius = 0.0;
+    int total = 0;
+};
+
+struct Profile {            // HIP-event timing of the SOR colour passes
+    bool on = false;
+    double sweep_ms = 0.0;  // summed duration of every sweep launch
+    long sweep_launches = 0;
+    double sweep_bytes = 0; // algorithmic bytes of the working launches
+    double setup_ms = 0.0;  // coefficient / set-up kernel
+    long setup_launches = 0;
+    double step_ms = 0.0;   // whole time steps (setup + solve + epilogue)
+    long steps = 0;
+    long voxel_updates = 0; // interior voxels x reference iterations
+    long iterations = 0;
+};
+
+enum ArrayId {
+    A_U = 0, A_PREV, A_I0, A_MASK, A_EXACT,
+    A_G0, A_G7 = A_G0 + 7,
+    A_TMP, A_TMP2,
+    A_COUNT
+};
+
+class Engine {
+public:
+    Engine() = default;
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    // Allocates device arrays for padded dims I,J,K,L.  `seg` adds i0, Mmask
+    // and the 8 g arrays; `mcf` adds exact_u.  Arrays start zeroed.
+    int init(int I, int J, int K, int L, bool seg, bool mcf);
+    void release();
+    bool ready() const { return ready_; }
+
+    double* full(ArrayId a) const { return full_[a]; }
+    long elems() const { return n_; }
+    int dims(int d) const { return d == 0 ? I_ : d == 1 ? J_ : d == 2 ? K_ : L_; }
+
+    int upload(ArrayId a, const double* host);
+    int download(ArrayId a, double* host);
+    int copy(ArrayId src, ArrayId dst);
+    // Coefficient face f (0..7, order e w n s t b fw bw) in full layout.
+    int download_coef(int f, double* host);
+
+    // --- time step pieces -------------------------------------------------
+    // Coefficients from u (and g if with_g) into the colour layout, u packed
+    // into the colour arrays, and (if write_prev) prev = u.
+    int setup(const StepParams& P, bool with_g, bool write_prev);
+    // Red-black SOR from the packed state to the stopping test.
+    int solve(const StepParams& P, SolveStats* st);
+    // Colour layout -> u (interior), with the Mmask pass fused if mask.
+    int unpack(const double* dmask);
+    int rescale(double* d, const Balls& balls, bool apply_mask);
+    // Lazily allocated full-layout scratch array (A_TMP or A_TMP2).
+    double* scratch(ArrayId a);
+    // Whole segmentation step: setup, solve, unpack, rescale epilogue.
+    int seg_step(const StepParams& P, const Balls* balls, SolveStats* st);
+    // MCF step: setup (g == 1), exact boundary at time t, solve, unpack.
+    int mcf_step(const StepParams& P, double t, SolveStats* st);
+
+    // --- edge indicator -----------------------------------------------------
+    int reflect(double* d);
+    // Implicit heat step in place on device array d (prev receives input).
+    int presmooth(double* d, double sigma, double h, double omega, double gsTol,
+                  double rel2, double cap, int* iters, double* R);
+    int gfun(const double* a, const double* b, double h, double K, double theta,
+             double rho);
+
+    // --- helpers --------------------------------------------------------------
+    int l2norm(const double* a, const double* b, double h, double* out);
+    int mcf_exact(double* d, double h, double t, bool ghosts_only);
+    int sync();
+
+    hipStream_t stream() const { return st_; }
+    Profile prof;
+    std::string err;
+
+private:
+    int check(hipError_t e, const char* what);
+    int launch_check(const char* what);
+    int sumsq_interior(const double* d, double h, double* out);
+    int reduce_to_host(int np, double* out);
+
+    bool ready_ = false;
+    int I_ = 0, J_ = 0, K_ = 0, L_ = 0, H_ = 0;
+    long n_ = 0, np_ = 0;        // full / colour-plane element counts
+    hipStream_t st_ = nullptr;
+    double* full_[A_COUNT] = {};
+    double* cf_[2][8] = {};      // colour-layout coefficients
+    double* pc_[2] = {};         // colour-layout prev
+    double* ur_[2] = {};         // red ping-pong
+    double* ub_ = nullptr;       // black
+    double* part_ = nullptr;     // partial sums (two halves)
+    long npart_ = 0;
+    double* dscal_ = nullptr;    // device scalar scratch
+    double* hscal_ = nullptr;    // pinned host scalar
+    SorCtl* ctl_ = nullptr;      // device control block
+    SorCtl* hctl_ = nullptr;     // pinned host mirror
+    hipEvent_t ev_ = nullptr;
+    std::vector<hipEvent_t> pev_; // profiling events (pairs)
+    int red_final_ = 0;          // which red buffer holds the accepted state
+};
+
+// Last error of the calling library (set by the ABI layers).
+void set_error(const std::string& msg);
+const char* last_error();
+
+}  // namespace s4d

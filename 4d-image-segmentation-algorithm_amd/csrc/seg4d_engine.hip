@@ -1,0 +1,531 @@
+// seg4d_engine.hip -- host driver of the HIP kernels for one 4D volume.
+#include "seg4d_engine.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "seg4d_kernels.h"
+
+namespace s4d {
+
+namespace {
+thread_local std::string g_last_error;
+
+bool is_pow2(double h) {
+    if (!(h > 0.0) || !std::isfinite(h)) return false;
+    int e = 0;
+    return std::frexp(h, &e) == 0.5;
+}
+
+Scale make_scale(double h) {
+    Scale S;
+    S.h = h;
+    S.q = 4.0 * h;
+    S.inv_h = 1.0 / h;
+    S.inv_q = 1.0 / S.q;
+    return S;
+}
+
+inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+}  // namespace
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+const char* last_error() { return g_last_error.c_str(); }
+
+Engine::~Engine() { release(); }
+
+int Engine::check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return 0;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    set_error(err);
+    return -1;
+}
+
+int Engine::launch_check(const char* what) { return check(hipGetLastError(), what); }
+
+int Engine::init(int I, int J, int K, int L, bool seg, bool mcf) {
+    release();
+    if (I < 3 || J < 3 || K < 3 || L < 3) {
+        err = "padded dims must be >= 3 in every direction";
+        set_error(err);
+        return -1;
+    }
+    const long n = (long)I * J * K * L;
+    if (n >= (1L << 31)) {
+        err = "padded volume exceeds 2^31 elements (32-bit index, as the reference)";
+        set_error(err);
+        return -1;
+    }
+    int ndev = 0;
+    if (check(hipGetDeviceCount(&ndev), "hipGetDeviceCount")) return -1;
+    if (ndev < 1) {
+        err = "no HIP device visible";
+        set_error(err);
+        return -1;
+    }
+    I_ = I; J_ = J; K_ = K; L_ = L;
+    H_ = (I + 1) / 2;
+    n_ = n;
+    np_ = (long)H_ * J * K * L;
+    if (check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate")) return -1;
+    if (check(hipEventCreateWithFlags(&ev_, hipEventDisableTiming), "hipEventCreate")) return -1;
+    auto alloc = [&](double** p, long cnt) -> int {
+        if (check(hipMalloc((void**)p, (size_t)cnt * sizeof(double)), "hipMalloc")) return -1;
+        return check(hipMemsetAsync(*p, 0, (size_t)cnt * sizeof(double), st_), "hipMemset");
+    };
+    if (alloc(&full_[A_U], n) || alloc(&full_[A_PREV], n)) return -1;
+    if (seg) {
+        if (alloc(&full_[A_I0], n) || alloc(&full_[A_MASK], n)) return -1;
+        for (int f = 0; f < 8; f++)
+            if (alloc(&full_[A_G0 + f], n)) return -1;
+    }
+    if (mcf && alloc(&full_[A_EXACT], n)) return -1;
+    for (int c = 0; c < 2; c++) {
+        for (int f = 0; f < 8; f++)
+            if (alloc(&cf_[c][f], np_)) return -1;
+        if (alloc(&pc_[c], np_) || alloc(&ur_[c], np_)) return -1;
+    }
+    if (alloc(&ub_, np_)) return -1;
+    // partial-sum capacity: the largest grid of any reducing kernel
+    const long g_full = (long)cdiv((long)I * J, kBlock) * K * L;
+    const long g_sweep = (long)cdiv((long)H_ * (J - 2), kBlock) * (K - 2);
+    long np = g_full > g_sweep ? g_full : g_sweep;
+    if (np < 4096) np = 4096;
+    npart_ = np;
+    if (alloc(&part_, 2 * np)) return -1;
+    if (alloc(&dscal_, 8)) return -1;
+    if (check(hipMalloc((void**)&ctl_, sizeof(SorCtl)), "hipMalloc ctl")) return -1;
+    if (check(hipHostMalloc((void**)&hctl_, sizeof(SorCtl), hipHostMallocDefault),
+              "hipHostMalloc"))
+        return -1;
+    if (check(hipHostMalloc((void**)&hscal_, 8 * sizeof(double), hipHostMallocDefault),
+              "hipHostMalloc"))
+        return -1;
+    std::memset(hctl_, 0, sizeof(SorCtl));
+    if (check(hipStreamSynchronize(st_), "init sync")) return -1;
+    ready_ = true;
+    return 0;
+}
+
+void Engine::release() {
+    if (st_) hipStreamSynchronize(st_);
+    for (auto& p : full_) {
+        if (p) hipFree(p);
+        p = nullptr;
+    }
+    for (int c = 0; c < 2; c++) {
+        for (int f = 0; f < 8; f++) {
+            if (cf_[c][f]) hipFree(cf_[c][f]);
+            cf_[c][f] = nullptr;
+        }
+        if (pc_[c]) hipFree(pc_[c]);
+        if (ur_[c]) hipFree(ur_[c]);
+        pc_[c] = ur_[c] = nullptr;
+    }
+    if (ub_) hipFree(ub_);
+    if (part_) hipFree(part_);
+    if (dscal_) hipFree(dscal_);
+    if (ctl_) hipFree(ctl_);
+    if (hctl_) hipHostFree(hctl_);
+    if (hscal_) hipHostFree(hscal_);
+    ub_ = part_ = dscal_ = hscal_ = nullptr;
+    ctl_ = hctl_ = nullptr;
+    for (auto e : pev_) hipEventDestroy(e);
+    pev_.clear();
+    if (ev_) hipEventDestroy(ev_);
+    ev_ = nullptr;
+    if (st_) hipStreamDestroy(st_);
+    st_ = nullptr;
+    ready_ = false;
+}
+
+static Geo make_geo(int I, int J, int K, int L, int H) {
+    Geo G;
+    G.I = I; G.J = J; G.K = K; G.L = L;
+    G.sy = I; G.sz = I * J; G.sw = I * J * K;
+    G.H = H;
+    G.py = H; G.pz = H * J; G.pw = H * J * K;
+    return G;
+}
+
+int Engine::upload(ArrayId a, const double* host) {
+    if (!full_[a]) return check(hipErrorInvalidValue, "upload: array not allocated");
+    if (check(hipMemcpyAsync(full_[a], host, (size_t)n_ * sizeof(double),
+                             hipMemcpyHostToDevice, st_), "upload"))
+        return -1;
+    return check(hipStreamSynchronize(st_), "upload sync");
+}
+
+int Engine::download(ArrayId a, double* host) {
+    if (!full_[a]) return check(hipErrorInvalidValue, "download: array not allocated");
+    if (check(hipMemcpyAsync(host, full_[a], (size_t)n_ * sizeof(double),
+                             hipMemcpyDeviceToHost, st_), "download"))
+        return -1;
+    return check(hipStreamSynchronize(st_), "download sync");
+}
+
+int Engine::copy(ArrayId src, ArrayId dst) {
+    if (!full_[src] || !full_[dst]) return check(hipErrorInvalidValue, "copy: array not allocated");
+    return check(hipMemcpyAsync(full_[dst], full_[src], (size_t)n_ * sizeof(double),
+                                hipMemcpyDeviceToDevice, st_), "copy");
+}
+
+int Engine::download_coef(int f, double* host) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    if (!scratch(A_TMP)) return -1;
+    dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
+    k_unpack_coef<<<grid, kBlock, 0, st_>>>(G, cf_[0][f], cf_[1][f], full_[A_TMP]);
+    if (launch_check("k_unpack_coef")) return -1;
+    return download(A_TMP, host);
+}
+
+int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    const Scale S = make_scale(P.h);
+    const bool p2 = is_pow2(P.h);
+    const double h4 = P.h * P.h * P.h * P.h;
+    CPtr8 g{};
+    if (with_g)
+        for (int f = 0; f < 8; f++) g.p[f] = full_[A_G0 + f];
+    SplitBufs sb;
+    for (int c = 0; c < 2; c++) {
+        for (int f = 0; f < 8; f++) sb.coef[c][f] = cf_[c][f];
+        sb.prev[c] = pc_[c];
+        sb.ured[c] = ur_[c];
+    }
+    sb.ublack = ub_;
+    double* prevf = write_prev ? full_[A_PREV] : nullptr;
+    if (!write_prev) {
+        prevf = scratch(A_TMP2);
+        if (!prevf) return -1;
+    }
+    dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof.on) {
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventRecord(e0, st_);
+    }
+    if (with_g) {
+        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_);
+        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_);
+    } else {
+        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_);
+        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_);
+    }
+    if (launch_check("k_setup_step")) return -1;
+    if (prof.on) {
+        hipEventRecord(e1, st_);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        prof.setup_ms += ms;
+        prof.setup_launches++;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    }
+    const int np = (int)(grid.x * grid.y * grid.z);
+    k_init_ctl<<<1, 1024, 0, st_>>>(part_, np, ctl_, P.gsTol, P.rel2);
+    return launch_check("k_init_ctl");
+}
+
+int Engine::solve(const StepParams& P, SolveStats* out) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    const double c = P.tau / (P.h * P.h);
+    const double h4 = P.h * P.h * P.h * P.h;
+    const dim3 grid(cdiv((long)H_ * (J_ - 2), kBlock), K_ - 2);
+    const int nb = (int)(grid.x * grid.y);
+    double* part_r = part_;
+    double* part_b = part_ + npart_;
+    CPtr8 cr{}, cb{};
+    for (int f = 0; f < 8; f++) {
+        cr.p[f] = cf_[0][f];
+        cb.p[f] = cf_[1][f];
+    }
+    const int cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
+    // interior red / black point counts for the byte model
+    const long nint = (long)(I_ - 2) * (J_ - 2) * (K_ - 2) * (L_ - 2);
+    const long nred = (nint + 1) / 2, nblk = nint - nred;  // close enough for odd counts
+    int pe = 0;  // profiling event index
+    auto ev_pair = [&](int& idx) -> hipEvent_t* {
+        while ((int)pev_.size() < idx + 2) {
+            hipEvent_t e;
+            hipEventCreate(&e);
+            pev_.push_back(e);
+        }
+        hipEvent_t* r = &pev_[idx];
+        idx += 2;
+        return r;
+    };
+    auto red = [&](int z, bool resid) -> int {
+        hipEvent_t* e = prof.on ? ev_pair(pe) : nullptr;
+        if (e) hipEventRecord(e[0], st_);
+        const double* xin = ur_[(z - 1) & 1];
+        double* xout = ur_[z & 1];
+        if (resid)
+            k_sor_sweep<1><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr,
+                                                     pc_[0], part_r, nullptr);
+        else
+            k_sor_sweep<0><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr,
+                                                     pc_[0], nullptr, nullptr);
+        if (e) hipEventRecord(e[1], st_);
+        return launch_check("k_sor_sweep red");
+    };
+    auto black = [&](int z) -> int {
+        hipEvent_t* e = prof.on ? ev_pair(pe) : nullptr;
+        if (e) hipEventRecord(e[0], st_);
+        k_sor_sweep<2><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb,
+                                                 pc_[1], part_b, ctl_);
+        if (e) hipEventRecord(e[1], st_);
+        return launch_check("k_sor_sweep black");
+    };
+    if (red(1, false) || black(1)) return -1;
+    int z = 1;
+    for (;;) {
+        if (red(z + 1, true)) return -1;
+        k_reduce_resid<<<1, 1024, 0, st_>>>(part_r, nb, part_b, nb, ctl_, nullptr, z, cap);
+        if (launch_check("k_reduce_resid")) return -1;
+        if (check(hipMemcpyAsync(hctl_, ctl_, sizeof(SorCtl), hipMemcpyDeviceToHost, st_),
+                  "ctl copy"))
+            return -1;
+        if (check(hipEventRecord(ev_, st_), "event")) return -1;
+        if (black(z + 1)) return -1;  // becomes a no-op if iteration z stopped
+        hipError_t q;
+        while ((q = hipEventQuery(ev_)) == hipErrorNotReady) {
+        }
+        if (check(q, "solve wait")) return -1;
+        if (hctl_->stop) break;
+        z++;
+    }
+    red_final_ = z & 1;
+    if (out) {
+        out->z = hctl_->zfinal;
+        out->R = hctl_->R;
+        out->tol = hctl_->tol;
+        out->sumsq = hctl_->sumsq;
+    }
+    if (prof.on) {
+        if (check(hipStreamSynchronize(st_), "prof sync")) return -1;
+        for (int q = 0; q < pe; q += 2) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, pev_[q], pev_[q + 1]);
+            prof.sweep_ms += ms;
+            prof.sweep_launches++;
+        }
+        // working launches: red(1..z+1) and black(1..z); 96 B per colour point
+        prof.sweep_bytes += 96.0 * ((double)(z + 1) * nred + (double)z * nblk);
+        prof.iterations += z;
+        prof.voxel_updates += (long)z * nint;
+    }
+    return 0;
+}
+
+int Engine::unpack(const double* dmask) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    dim3 grid(cdiv((long)(I_ - 2) * (J_ - 2), kBlock), K_ - 2, L_ - 2);
+    k_unpack<<<grid, kBlock, 0, st_>>>(G, ur_[red_final_], ub_, full_[A_U], dmask);
+    return launch_check("k_unpack");
+}
+
+double* Engine::scratch(ArrayId a) {
+    if (!full_[a] &&
+        check(hipMalloc((void**)&full_[a], (size_t)n_ * sizeof(double)), "hipMalloc scratch"))
+        return nullptr;
+    return full_[a];
+}
+
+int Engine::rescale(double* d, const Balls& balls, bool apply_mask) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    if (balls.total > 0) {
+        // device copies of the ball lists
+        const size_t ni = (size_t)L_ * sizeof(int), nd = (size_t)balls.total * sizeof(double);
+        char* buf = nullptr;
+        const size_t bytes = 2 * ni + 3 * nd;
+        if (check(hipMalloc((void**)&buf, bytes), "hipMalloc balls")) return -1;
+        int* dcount = (int*)buf;
+        int* doff = (int*)(buf + ni);
+        double* dcx = (double*)(buf + 2 * ni);
+        double* dcy = (double*)(buf + 2 * ni + nd);
+        double* dcz = (double*)(buf + 2 * ni + 2 * nd);
+        int rc = 0;
+        rc |= check(hipMemcpyAsync(dcount, balls.count.data(), ni, hipMemcpyHostToDevice, st_), "balls");
+        rc |= check(hipMemcpyAsync(doff, balls.offset.data(), ni, hipMemcpyHostToDevice, st_), "balls");
+        rc |= check(hipMemcpyAsync(dcx, balls.cx.data(), nd, hipMemcpyHostToDevice, st_), "balls");
+        rc |= check(hipMemcpyAsync(dcy, balls.cy.data(), nd, hipMemcpyHostToDevice, st_), "balls");
+        rc |= check(hipMemcpyAsync(dcz, balls.cz.data(), nd, hipMemcpyHostToDevice, st_), "balls");
+        if (!rc) {
+            k_rescale_balls<<<L_ - 2, kBlock, 0, st_>>>(G, d, dcount, doff, dcx, dcy, dcz,
+                                                      balls.radius);
+            rc |= launch_check("k_rescale_balls");
+        }
+        hipStreamSynchronize(st_);
+        hipFree(buf);
+        if (rc) return -1;
+    }
+    if (apply_mask) {
+        if (!full_[A_MASK]) return check(hipErrorInvalidValue, "rescale: no mask array");
+        dim3 grid(cdiv((long)(I_ - 2) * (J_ - 2), kBlock), K_ - 2, L_ - 2);
+        k_apply_mask<<<grid, kBlock, 0, st_>>>(G, d, full_[A_MASK]);
+        if (launch_check("k_apply_mask")) return -1;
+    }
+    return 0;
+}
+
+int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof.on) {
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventRecord(e0, st_);
+    }
+    if (setup(P, true, true)) return -1;
+    if (solve(P, st)) return -1;
+    const bool has_balls = balls && balls->total > 0;
+    if (has_balls) {
+        if (unpack(nullptr) || rescale(full_[A_U], *balls, true)) return -1;
+    } else {
+        if (unpack(full_[A_MASK])) return -1;
+    }
+    if (prof.on) {
+        hipEventRecord(e1, st_);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        prof.step_ms += ms;
+        prof.steps++;
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+    }
+    return check(hipStreamSynchronize(st_), "seg_step");
+}
+
+int Engine::mcf_step(const StepParams& P, double t, SolveStats* st) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    if (setup(P, false, true)) return -1;
+    dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
+    k_mcf_exact<<<grid, kBlock, 0, st_>>>(G, P.h, t, full_[A_U], 1);
+    if (launch_check("k_mcf_exact")) return -1;
+    k_pack_ghosts<<<grid, kBlock, 0, st_>>>(G, full_[A_U], ur_[0], ur_[1], ub_);
+    if (launch_check("k_pack_ghosts")) return -1;
+    if (solve(P, st)) return -1;
+    if (unpack(nullptr)) return -1;
+    return check(hipStreamSynchronize(st_), "mcf_step");
+}
+
+int Engine::reflect(double* d) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
+    k_reflect<<<grid, kBlock, 0, st_>>>(G, d);
+    return launch_check("k_reflect");
+}
+
+int Engine::reduce_to_host(int np, double* out) {
+    k_reduce_sum<<<1, 1024, 0, st_>>>(part_, np, dscal_);
+    if (launch_check("k_reduce_sum")) return -1;
+    if (check(hipMemcpyAsync(hscal_, dscal_, sizeof(double), hipMemcpyDeviceToHost, st_),
+              "scalar copy"))
+        return -1;
+    if (check(hipStreamSynchronize(st_), "reduce sync")) return -1;
+    *out = hscal_[0];
+    return 0;
+}
+
+namespace {
+__global__ __launch_bounds__(kBlock) void k_sumsq_interior(Geo G, const double* __restrict__ a,
+                                                          double h4,
+                                                          double* __restrict__ partials) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y + 1, l = blockIdx.z + 1;
+    const int w = G.I - 2;
+    double acc = 0.0;
+    if (t < w * (G.J - 2)) {
+        const int j = t / w + 1, i = t - (j - 1) * w + 1;
+        const double v = a[((l * G.K + k) * G.J + j) * G.I + i];
+        acc = v * v * h4;
+    }
+    const double bs = block_sum256(acc);
+    if (threadIdx.x == 0)
+        partials[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = bs;
+}
+}  // namespace
+
+int Engine::sumsq_interior(const double* d, double h, double* out) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    const double h4 = h * h * h * h;
+    dim3 grid(cdiv((long)(I_ - 2) * (J_ - 2), kBlock), K_ - 2, L_ - 2);
+    k_sumsq_interior<<<grid, kBlock, 0, st_>>>(G, d, h4, part_);
+    if (launch_check("k_sumsq_interior")) return -1;
+    return reduce_to_host((int)(grid.x * grid.y * grid.z), out);
+}
+
+int Engine::presmooth(double* d, double sigma, double h, double omega, double gsTol,
+                      double rel2, double cap, int* iters, double* Rout) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    const double c = sigma / (h * h);
+    const double h4 = h * h * h * h;
+    if (check(hipMemcpyAsync(full_[A_PREV], d, (size_t)n_ * sizeof(double),
+                             hipMemcpyDeviceToDevice, st_), "presmooth copy"))
+        return -1;
+    if (reflect(d)) return -1;
+    double tol = gsTol;
+    if (rel2 > 0.0) {
+        double S = 0.0;
+        if (sumsq_interior(full_[A_PREV], h, &S)) return -1;
+        tol = rel2 * S;
+    }
+    const dim3 gp(cdiv((long)H_ * (J_ - 2), kBlock), K_ - 2, L_ - 2);
+    const dim3 gr(cdiv((long)(I_ - 2) * (J_ - 2), kBlock), K_ - 2, L_ - 2);
+    const int np = (int)(gr.x * gr.y * gr.z);
+    CPtr8 none{};
+    int z = 0;
+    double R = 0.0;
+    do {
+        z = z + 1;
+        k_full_pass<true><<<gp, kBlock, 0, st_>>>(G, c, omega, 0, d, full_[A_PREV], none);
+        k_full_pass<true><<<gp, kBlock, 0, st_>>>(G, c, omega, 1, d, full_[A_PREV], none);
+        k_full_resid<true><<<gr, kBlock, 0, st_>>>(G, c, h4, d, full_[A_PREV], none, part_);
+        if (launch_check("presmooth kernels")) return -1;
+        if (reduce_to_host(np, &R)) return -1;
+    } while (R > tol && z < cap);
+    if (iters) *iters = z;
+    if (Rout) *Rout = R;
+    return 0;
+}
+
+int Engine::gfun(const double* a, const double* b, double h, double K, double theta,
+                 double rho) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    const Scale S = make_scale(h);
+    Ptr8 g;
+    for (int f = 0; f < 8; f++) {
+        g.p[f] = full_[A_G0 + f];
+        if (!g.p[f]) return check(hipErrorInvalidValue, "gfun: no g arrays");
+    }
+    dim3 grid(cdiv((long)(I_ - 2) * (J_ - 2), kBlock), K_ - 2, L_ - 2);
+    if (is_pow2(h)) k_gfun<true><<<grid, kBlock, 0, st_>>>(G, S, K, theta, rho, a, b, g);
+    else k_gfun<false><<<grid, kBlock, 0, st_>>>(G, S, K, theta, rho, a, b, g);
+    return launch_check("k_gfun");
+}
+
+int Engine::l2norm(const double* a, const double* b, double h, double* out) {
+    const double h4 = h * h * h * h;
+    const int nblk = 2048;
+    k_sqdiff<<<nblk, kBlock, 0, st_>>>(a, b, n_, h4, part_);
+    if (launch_check("k_sqdiff")) return -1;
+    double S = 0.0;
+    if (reduce_to_host(nblk, &S)) return -1;
+    *out = std::sqrt(S);
+    return 0;
+}
+
+int Engine::mcf_exact(double* d, double h, double t, bool ghosts_only) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
+    k_mcf_exact<<<grid, kBlock, 0, st_>>>(G, h, t, d, ghosts_only ? 1 : 0);
+    return launch_check("k_mcf_exact");
+}
+
+int Engine::sync() { return check(hipStreamSynchronize(st_), "sync"); }
+
+}  // namespace s4d

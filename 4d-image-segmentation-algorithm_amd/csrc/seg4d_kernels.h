@@ -1,0 +1,700 @@
+// seg4d_kernels.h -- HIP kernels (gfx950, wave64) for the semi-implicit
+// subjective-surface / MCF time step.  Included by exactly one translation
+// unit per shared library (seg4d_engine.hip).
+//
+// Arithmetic contract: every FP64 expression below is evaluated in the same
+// grouping as the reference C (segmentation4D.c, mcf_4D.c), the build uses
+// -ffp-contract=off, and FP64 division / sqrt are the correctly rounded IEEE
+// operations, so a colour pass produces bitwise the values of the
+// reference's pass on the same inputs.  Only reductions differ in summation
+// order (they feed the stopping test, never the solution).
+//
+// Device data layout
+//   full  : padded x-fastest, idx = ((l*K + k)*J + j)*I + i  (the reference's)
+//   colour: the red (i+j+k+l even) and black points of every padded row are
+//           stored in separate arrays, row length H = (I+1)/2, packed index
+//           m = i >> 1: pidx = ((l*K + k)*J + j)*H + m.  A colour pass then
+//           streams only its own colour's coefficients/prev/u and reads the
+//           other colour at column m (rows j+-1, k+-1, l+-1) or m-1+2s (row).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace s4d {
+
+constexpr int kBlock = 256;
+
+struct Geo {
+    int I, J, K, L;   // padded dims
+    int sy, sz, sw;   // full-layout strides
+    int H;            // colour row length
+    int py, pz, pw;   // colour-layout strides
+};
+
+struct Ptr8 {
+    double* p[8];
+};
+struct CPtr8 {
+    const double* p[8];
+};
+
+// Per-solve control block in device memory.  `stop` is read by every black
+// pass so that a pass enqueued ahead of the host's decision becomes a no-op.
+struct SorCtl {
+    double R;       // residual of the last reduced iteration
+    double tol;     // absolute bound the residual is compared with
+    double sumsq;   // sum over the interior of prev^2 h^4 (relative mode)
+    int z;          // last reduced iteration
+    int stop;       // 1 once !(R > tol && z < cap)
+    int zfinal;     // iteration at which the solve stopped
+    int pad;
+};
+
+// ----------------------------------------------------------------------------
+// reductions (fixed order => run-to-run deterministic)
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum over a 256-thread block; result valid in thread 0.
+__device__ __forceinline__ double block_sum256(double v) {
+    __shared__ double sm[kBlock / 64];
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sm[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = ((sm[0] + sm[1]) + sm[2]) + sm[3];
+    return r;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X.  Remap the
+// hardware block id so that each XCD receives one contiguous range of
+// logical blocks: neighbouring k-planes then share an XCD (and its L2).
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    const int nx = 8;
+    const int per = nb / nx, rem = nb % nx;
+    const int x = b % nx;
+    return x * per + (x < rem ? x : rem) + b / nx;
+}
+
+// ----------------------------------------------------------------------------
+// face gradients (segmentation4D.c:456-560 for u, :596-820 for the image)
+// ----------------------------------------------------------------------------
+struct Scale {
+    double h, q;          // h and quotient 4h
+    double inv_h, inv_q;  // exact reciprocals when h is a power of two
+};
+
+template <bool POW2>
+__device__ __forceinline__ double scl(double d, double s, double inv) {
+    // When s is a power of two, d * (1/s) is the correctly rounded d / s.
+    if constexpr (POW2) return d * inv;
+    else return d / s;
+}
+
+// Sum of squares of the 4 gradient components of face F at full index p:
+// ((ux^2 + uy^2) + uz^2) + uw^2, components as in the reference.
+template <int F, bool POW2>
+__device__ __forceinline__ double face_sq(const double* __restrict__ u, int p,
+                                          const Geo& G, const Scale& S) {
+    const int st[4] = {1, G.sy, G.sz, G.sw};
+    constexpr int A = F >> 1;
+    const int dA = (F & 1) ? -st[A] : st[A];
+    double acc = 0.0;
+#pragma unroll
+    for (int B = 0; B < 4; B++) {
+        double c;
+        if (B == A) {
+            c = scl<POW2>(u[p + dA] - u[p], S.h, S.inv_h);
+        } else {
+            c = scl<POW2>((u[p + dA + st[B]] + u[p + st[B]]) -
+                          (u[p + dA - st[B]] + u[p - st[B]]), S.q, S.inv_q);
+        }
+        acc = (B == 0) ? c * c : acc + c * c;
+    }
+    return acc;
+}
+
+template <bool POW2>
+__device__ __forceinline__ void all_face_sq(const double* __restrict__ u, int p,
+                                            const Geo& G, const Scale& S,
+                                            double a[8]) {
+    a[0] = face_sq<0, POW2>(u, p, G, S);
+    a[1] = face_sq<1, POW2>(u, p, G, S);
+    a[2] = face_sq<2, POW2>(u, p, G, S);
+    a[3] = face_sq<3, POW2>(u, p, G, S);
+    a[4] = face_sq<4, POW2>(u, p, G, S);
+    a[5] = face_sq<5, POW2>(u, p, G, S);
+    a[6] = face_sq<6, POW2>(u, p, G, S);
+    a[7] = face_sq<7, POW2>(u, p, G, S);
+}
+
+__device__ __forceinline__ bool is_interior(const Geo& G, int i, int j, int k, int l) {
+    return i >= 1 && i <= G.I - 2 && j >= 1 && j <= G.J - 2 && k >= 1 &&
+           k <= G.K - 2 && l >= 1 && l <= G.L - 2;
+}
+
+// ----------------------------------------------------------------------------
+// Step set-up: coefficients + colour packing + prev copy + sum prev^2.
+// segmentation4D.c:224 (copy), :446-580 (coefficients); mcf_4D.c:208-342.
+// Grid: x over the padded (j,i) plane, y = k, z = l (all padded).
+// ----------------------------------------------------------------------------
+struct SplitBufs {
+    double* coef[2][8];  // [colour][face] colour layout
+    double* prev[2];     // colour layout
+    double* ured[2];     // red ping-pong buffers
+    double* ublack;
+};
+
+template <bool WITH_G, bool POW2>
+__global__ __launch_bounds__(kBlock) void k_setup_step(
+    Geo G, Scale S, double eps2, double h4, const double* __restrict__ u,
+    CPtr8 g, SplitBufs sb, double* __restrict__ prev_full,
+    double* __restrict__ partials) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y, l = blockIdx.z;
+    double acc = 0.0;
+    if (t < G.I * G.J) {
+        const int j = t / G.I, i = t - j * G.I;
+        const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+        const int c = (i + j + k + l) & 1;
+        const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
+        const double uc = u[p];
+        prev_full[p] = uc;
+        if (c == 0) {
+            sb.prev[0][pp] = uc;
+            sb.ured[0][pp] = uc;
+            sb.ured[1][pp] = uc;
+        } else {
+            sb.prev[1][pp] = uc;
+            sb.ublack[pp] = uc;
+        }
+        if (is_interior(G, i, j, k, l)) {
+            double a[8];
+            all_face_sq<POW2>(u, p, G, S, a);
+            double N[8];
+#pragma unroll
+            for (int f = 0; f < 8; f++) N[f] = sqrt(a[f] + eps2);
+            const double avg =
+                (((((((N[0] + N[1]) + N[2]) + N[3]) + N[4]) + N[5]) + N[6]) + N[7]) / 8.0;
+            const double vc = sqrt(avg * avg + eps2);
+#pragma unroll
+            for (int f = 0; f < 8; f++) {
+                double cf;
+                if constexpr (WITH_G) cf = vc * g.p[f][p] * (1.0 / N[f]);
+                else cf = vc * (1.0 / N[f]);
+                double* dst = c ? sb.coef[1][f] : sb.coef[0][f];
+                dst[pp] = cf;
+            }
+            acc = uc * uc * h4;
+        }
+    }
+    const double bs = block_sum256(acc);
+    if (threadIdx.x == 0)
+        partials[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = bs;
+}
+
+// ----------------------------------------------------------------------------
+// Red-black SOR colour pass on the colour layout, marching along l.
+// segmentation4D.c:233-285.  One thread owns packed column m of rows (j,k)
+// and walks l = 1..L-2, keeping the other colour's column-m values of rows
+// l-1, l, l+1 in registers (they are the bw / in-row / fw neighbours).
+//
+// RESID = 0: no residual; 1: residual of the point BEFORE its update (the
+// red pass of iteration z+1 yields the red residual terms of iteration z:
+// red values and black neighbours are then exactly those at the end of z);
+// 2: residual AFTER the update (black pass of iteration z: its red
+// neighbours are final for z).  segmentation4D.c:287-308.
+// ----------------------------------------------------------------------------
+template <int RESID>
+__global__ __launch_bounds__(kBlock) void k_sor_sweep(
+    Geo G, double c, double om, double h4, int colour,
+    const double* xin, double* xout, const double* __restrict__ y,
+    CPtr8 cf, const double* __restrict__ prev, double* __restrict__ partials,
+    const SorCtl* __restrict__ ctl) {
+    if (ctl != nullptr && ctl->stop) return;
+    const int nb = gridDim.x * gridDim.y;
+    const int lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, nb);
+    const int bx = lb % gridDim.x, by = lb / gridDim.x;
+    const int k = by + 1;
+    const int t = bx * kBlock + threadIdx.x;
+    const int jr = t / G.H;
+    const int m = t - jr * G.H;
+    const int j = jr + 1;
+    const bool rowok = j <= G.J - 2;
+    double acc = 0.0;
+    const int base = (k * G.J + j) * G.H + m;  // l = 0
+    double y_lm1 = 0.0, y_l = 0.0;
+    if (rowok) {
+        y_lm1 = y[base];
+        y_l = y[base + G.pw];
+    }
+    const double* e_ = cf.p[0];
+    const double* w_ = cf.p[1];
+    const double* n_ = cf.p[2];
+    const double* s_ = cf.p[3];
+    const double* t_ = cf.p[4];
+    const double* b_ = cf.p[5];
+    const double* fw_ = cf.p[6];
+    const double* bw_ = cf.p[7];
+    for (int l = 1; l <= G.L - 2; l++) {
+        const int pp = base + l * G.pw;
+        double y_lp1 = 0.0;
+        if (rowok) y_lp1 = y[pp + G.pw];
+        const int sft = colour ^ ((j + k + l) & 1);
+        const int i = 2 * m + sft;
+        if (rowok && i >= 1 && i <= G.I - 2) {
+            const double uW = sft ? y_l : y[pp - 1];
+            const double uE = sft ? y[pp + 1] : y_l;
+            const double uS = y[pp - G.py], uN = y[pp + G.py];
+            const double uB = y[pp - G.pz], uT = y[pp + G.pz];
+            const double ce = e_[pp], cw = w_[pp], cn = n_[pp], cs = s_[pp];
+            const double ct = t_[pp], cb = b_[pp], cfw = fw_[pp], cbw = bw_[pp];
+            // numerator order e, w, s, n, b, t, bw, fw (segmentation4D.c:242-249)
+            double num = ce * uE;
+            num = num + cw * uW;
+            num = num + cs * uS;
+            num = num + cn * uN;
+            num = num + cb * uB;
+            num = num + ct * uT;
+            num = num + cbw * y_lm1;
+            num = num + cfw * y_lp1;
+            // diagonal order e, w, n, s, t, b, fw, bw (segmentation4D.c:250-252)
+            double dg = ce + cw;
+            dg = dg + cn;
+            dg = dg + cs;
+            dg = dg + ct;
+            dg = dg + cb;
+            dg = dg + cfw;
+            dg = dg + cbw;
+            const double D = 1 + c * dg;
+            const double pv = prev[pp];
+            const double uo = xin[pp];
+            if constexpr (RESID == 1) {
+                const double r = uo * D - c * num - pv;
+                acc += r * r * h4;
+            }
+            const double gs = (pv + c * num) / D;
+            const double un = uo + om * (gs - uo);
+            xout[pp] = un;
+            if constexpr (RESID == 2) {
+                const double r = un * D - c * num - pv;
+                acc += r * r * h4;
+            }
+        }
+        y_lm1 = y_l;
+        y_l = y_lp1;
+    }
+    if constexpr (RESID != 0) {
+        const double bs = block_sum256(acc);
+        if (threadIdx.x == 0) partials[lb] = bs;
+    }
+}
+
+// Final residual reduction of one SOR iteration and the reference's stopping
+// test `while (R > gsTol && z < maxNoGSIteration)` (segmentation4D.c:309).
+__global__ __launch_bounds__(1024) void k_reduce_resid(
+    const double* __restrict__ pa, int na, const double* __restrict__ pb, int nbp,
+    SorCtl* ctl, double* __restrict__ hist, int z, int cap) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < na; q += 1024) s += pa[q];
+    double s2 = 0.0;
+    for (int q = threadIdx.x; q < nbp; q += 1024) s2 += pb[q];
+    s = s + s2;
+    s = wave_sum(s);
+    __shared__ double sm[16];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double R = sm[0];
+        for (int q = 1; q < 16; q++) R += sm[q];
+        ctl->R = R;
+        ctl->z = z;
+        if (hist) hist[z - 1] = R;
+        const bool cont = (R > ctl->tol) && (z < cap);
+        if (!cont) {
+            ctl->stop = 1;
+            ctl->zfinal = z;
+        }
+    }
+}
+
+// Sets the per-solve tolerance: absolute gsTol, or rel^2 * sum(prev^2 h^4).
+__global__ __launch_bounds__(1024) void k_init_ctl(const double* __restrict__ partials,
+                                                  int np, SorCtl* ctl, double gsTol,
+                                                  double rel2) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < np; q += 1024) s += partials[q];
+    s = wave_sum(s);
+    __shared__ double sm[16];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = sm[0];
+        for (int q = 1; q < 16; q++) S += sm[q];
+        ctl->sumsq = S;
+        ctl->tol = rel2 > 0.0 ? rel2 * S : gsTol;
+        ctl->R = 0.0;
+        ctl->z = 0;
+        ctl->stop = 0;
+        ctl->zfinal = 0;
+    }
+}
+
+// Generic fixed-order sum of `np` partials into *out.
+__global__ __launch_bounds__(1024) void k_reduce_sum(const double* __restrict__ partials,
+                                                    int np, double* out) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < np; q += 1024) s += partials[q];
+    s = wave_sum(s);
+    __shared__ double sm[16];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = sm[0];
+        for (int q = 1; q < 16; q++) S += sm[q];
+        *out = S;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Colour layout -> full layout after the solve (interior points only), with
+// the reference's Mmask pass (segmentation4D.c:1093-1101) fused in when
+// there are no balls to rescale first.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_unpack(Geo G, const double* __restrict__ ur,
+                                                  const double* __restrict__ ub,
+                                                  double* __restrict__ u,
+                                                  const double* __restrict__ mask) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y + 1, l = blockIdx.z + 1;
+    const int w = G.I - 2;
+    if (t >= w * (G.J - 2)) return;
+    const int j = t / w + 1, i = t - (j - 1) * w + 1;
+    const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+    const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
+    double v = ((i + j + k + l) & 1) ? ub[pp] : ur[pp];
+    if (mask != nullptr && mask[p] == 0) v = 0;
+    u[p] = v;
+}
+
+// ----------------------------------------------------------------------------
+// Full-layout in-place SOR passes (reference layout).  Used by the implicit
+// heat step (constant coefficients, segmentation4D.c:355-403) and by the
+// reference-layout solver variant (8 coefficient arrays, :233-308).
+// Thread per (packed column m, row j) of one (k,l) row plane.
+// ----------------------------------------------------------------------------
+template <bool CONST_COEF>
+__global__ __launch_bounds__(kBlock) void k_full_pass(Geo G, double c, double om,
+                                                     int colour, double* u,
+                                                     const double* __restrict__ prev,
+                                                     CPtr8 cf) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y + 1, l = blockIdx.z + 1;
+    const int jr = t / G.H;
+    const int m = t - jr * G.H;
+    const int j = jr + 1;
+    if (j > G.J - 2) return;
+    const int sft = colour ^ ((j + k + l) & 1);
+    const int i = 2 * m + sft;
+    if (i < 1 || i > G.I - 2) return;
+    const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+    const double uE = u[p + 1], uW = u[p - 1], uS = u[p - G.sy], uN = u[p + G.sy];
+    const double uB = u[p - G.sz], uT = u[p + G.sz], uBW = u[p - G.sw], uFW = u[p + G.sw];
+    double gs;
+    if constexpr (CONST_COEF) {
+        double nb = uE + uW;
+        nb = nb + uS;
+        nb = nb + uN;
+        nb = nb + uB;
+        nb = nb + uT;
+        nb = nb + uBW;
+        nb = nb + uFW;
+        gs = (prev[p] + c * nb) / (1 + c * 8);
+    } else {
+        const double ce = cf.p[0][p], cw = cf.p[1][p], cn = cf.p[2][p], cs = cf.p[3][p];
+        const double ct = cf.p[4][p], cb = cf.p[5][p], cfw = cf.p[6][p], cbw = cf.p[7][p];
+        double num = ce * uE;
+        num = num + cw * uW;
+        num = num + cs * uS;
+        num = num + cn * uN;
+        num = num + cb * uB;
+        num = num + ct * uT;
+        num = num + cbw * uBW;
+        num = num + cfw * uFW;
+        double dg = ce + cw;
+        dg = dg + cn;
+        dg = dg + cs;
+        dg = dg + ct;
+        dg = dg + cb;
+        dg = dg + cfw;
+        dg = dg + cbw;
+        gs = (prev[p] + c * num) / (1 + c * dg);
+    }
+    const double uo = u[p];
+    u[p] = uo + om * (gs - uo);
+}
+
+// Residual partials of the full-layout system; grid x over the interior
+// (j,i) plane, y = k-1, z = l-1.
+template <bool CONST_COEF>
+__global__ __launch_bounds__(kBlock) void k_full_resid(Geo G, double c, double h4,
+                                                      const double* __restrict__ u,
+                                                      const double* __restrict__ prev,
+                                                      CPtr8 cf,
+                                                      double* __restrict__ partials) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y + 1, l = blockIdx.z + 1;
+    const int w = G.I - 2;
+    double acc = 0.0;
+    if (t < w * (G.J - 2)) {
+        const int j = t / w + 1, i = t - (j - 1) * w + 1;
+        const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+        const double uE = u[p + 1], uW = u[p - 1], uS = u[p - G.sy], uN = u[p + G.sy];
+        const double uB = u[p - G.sz], uT = u[p + G.sz], uBW = u[p - G.sw], uFW = u[p + G.sw];
+        double r;
+        if constexpr (CONST_COEF) {
+            double nb = uE + uW;
+            nb = nb + uS;
+            nb = nb + uN;
+            nb = nb + uB;
+            nb = nb + uT;
+            nb = nb + uBW;
+            nb = nb + uFW;
+            r = u[p] * (1 + c * 8) - c * nb - prev[p];
+        } else {
+            const double ce = cf.p[0][p], cw = cf.p[1][p], cn = cf.p[2][p], cs = cf.p[3][p];
+            const double ct = cf.p[4][p], cb = cf.p[5][p], cfw = cf.p[6][p], cbw = cf.p[7][p];
+            double num = ce * uE;
+            num = num + cw * uW;
+            num = num + cs * uS;
+            num = num + cn * uN;
+            num = num + cb * uB;
+            num = num + ct * uT;
+            num = num + cbw * uBW;
+            num = num + cfw * uFW;
+            double dg = ce + cw;
+            dg = dg + cn;
+            dg = dg + cs;
+            dg = dg + ct;
+            dg = dg + cb;
+            dg = dg + cfw;
+            dg = dg + cbw;
+            r = u[p] * (1 + c * dg) - c * num - prev[p];
+        }
+        acc = r * r * h4;
+    }
+    const double bs = block_sum256(acc);
+    if (threadIdx.x == 0)
+        partials[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = bs;
+}
+
+// ----------------------------------------------------------------------------
+// Ghost shell by clamped-index copy (closed form of the sequential X,Y,Z,W
+// passes of segmentation4D.c:826-864).  Grid x over padded (j,i), y=k, z=l.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+__global__ __launch_bounds__(kBlock) void k_reflect(Geo G, double* u) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y, l = blockIdx.z;
+    if (t >= G.I * G.J) return;
+    const int j = t / G.I, i = t - j * G.I;
+    if (is_interior(G, i, j, k, l)) return;
+    const int ii = clampi(i, 1, G.I - 2), jj = clampi(j, 1, G.J - 2);
+    const int kk = clampi(k, 1, G.K - 2), ll = clampi(l, 1, G.L - 2);
+    u[((l * G.K + k) * G.J + j) * G.I + i] = u[((ll * G.K + kk) * G.J + jj) * G.I + ii];
+}
+
+// ----------------------------------------------------------------------------
+// Edge indicator g on 8 faces (segmentation4D.c:582-823 after presmoothing):
+// g_f = 1/(1 + K (theta |grad I0|_f^2 + rho |grad Ithr|_f^2)).
+// Grid x over interior (j,i), y = k-1, z = l-1.
+// ----------------------------------------------------------------------------
+template <bool POW2>
+__global__ __launch_bounds__(kBlock) void k_gfun(Geo G, Scale S, double K_, double theta,
+                                                double rho, const double* __restrict__ a,
+                                                const double* __restrict__ b, Ptr8 g) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y + 1, l = blockIdx.z + 1;
+    const int w = G.I - 2;
+    if (t >= w * (G.J - 2)) return;
+    const int j = t / w + 1, i = t - (j - 1) * w + 1;
+    const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+    double va[8], vb[8];
+    all_face_sq<POW2>(a, p, G, S, va);
+    all_face_sq<POW2>(b, p, G, S, vb);
+#pragma unroll
+    for (int f = 0; f < 8; f++) g.p[f][p] = 1.0 / (1 + K_ * (theta * va[f] + rho * vb[f]));
+}
+
+// ----------------------------------------------------------------------------
+// Flat helpers over the full padded array
+// ----------------------------------------------------------------------------
+// Partial sums of (a-b)^2 h^4 (segmentation4D.c:194-213); grid-stride with a
+// fixed grid so that the summation order is deterministic.
+__global__ __launch_bounds__(kBlock) void k_sqdiff(const double* __restrict__ a,
+                                                  const double* __restrict__ b, long n,
+                                                  double h4, double* __restrict__ partials) {
+    double acc = 0.0;
+    for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < n;
+         q += (long)gridDim.x * kBlock) {
+        const double d = a[q] - b[q];
+        acc += d * d * h4;
+    }
+    const double bs = block_sum256(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+// MCF exact solution ((x^2+y^2+z^2+w^2-1)/6)+t at cell centres
+// x = ((i-1) h + h/2) - 1.25 (mcf_4D.c:444-462).  With `ghosts_only` the
+// exact-value Dirichlet boundary of mcf_4D.c:344-443 is written instead; its
+// eight sequential face passes leave on every ghost the value of the last
+// pass that touches it (priority w > z > y > x, high side written after low
+// side), and that pass uses the fixed coordinate -(h/2)-1.25 or (h/2)+1.25.
+__global__ __launch_bounds__(kBlock) void k_mcf_exact(Geo G, double h, double tt,
+                                                     double* __restrict__ out,
+                                                     int ghosts_only) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y, l = blockIdx.z;
+    if (t >= G.I * G.J) return;
+    const int j = t / G.I, i = t - j * G.I;
+    double x = ((i - 1) * h + (h / 2.)) - 1.25;
+    double y = ((j - 1) * h + (h / 2.)) - 1.25;
+    double z = ((k - 1) * h + (h / 2.)) - 1.25;
+    double w = ((l - 1) * h + (h / 2.)) - 1.25;
+    if (ghosts_only) {
+        if (is_interior(G, i, j, k, l)) return;
+        const double lo = -(h / 2.) - 1.25, hi = (h / 2.) + 1.25;
+        if (l == 0 || l == G.L - 1) w = (l == 0) ? lo : hi;
+        else if (k == 0 || k == G.K - 1) z = (k == 0) ? lo : hi;
+        else if (j == 0 || j == G.J - 1) y = (j == 0) ? lo : hi;
+        else x = (i == 0) ? lo : hi;
+    }
+    double v = x * x + y * y;
+    v = v + z * z;
+    v = v + w * w;
+    out[((l * G.K + k) * G.J + j) * G.I + i] = ((v - 1.0) / 6.0) + tt;
+}
+
+// Re-pack the ghost shell of the full array into the colour arrays (both red
+// buffers).  Used after the MCF boundary fill.
+__global__ __launch_bounds__(kBlock) void k_pack_ghosts(Geo G, const double* __restrict__ u,
+                                                       double* ur0, double* ur1, double* ub) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y, l = blockIdx.z;
+    if (t >= G.I * G.J) return;
+    const int j = t / G.I, i = t - j * G.I;
+    if (is_interior(G, i, j, k, l)) return;
+    const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+    const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
+    const double v = u[p];
+    if ((i + j + k + l) & 1) {
+        ub[pp] = v;
+    } else {
+        ur0[pp] = v;
+        ur1[pp] = v;
+    }
+}
+
+// Colour layout -> full layout for one coefficient face (interior only),
+// used when the host asks for the coefficient arrays.
+__global__ __launch_bounds__(kBlock) void k_unpack_coef(Geo G, const double* __restrict__ cr,
+                                                       const double* __restrict__ cb,
+                                                       double* __restrict__ out) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y, l = blockIdx.z;
+    if (t >= G.I * G.J) return;
+    const int j = t / G.I, i = t - j * G.I;
+    const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+    double v = 0.0;
+    if (is_interior(G, i, j, k, l)) {
+        const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
+        v = ((i + j + k + l) & 1) ? cb[pp] : cr[pp];
+    }
+    out[p] = v;
+}
+
+// ----------------------------------------------------------------------------
+// Per-ball rescale (segmentation4D.c:1020-1092): one block per frame l,
+// balls processed in order c = 0..count-1 (ball 0 overwrites, later balls
+// keep the larger value; each ball's min/max sees the earlier balls'
+// writes).  Bounding boxes are clamped to the padded array.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_rescale_balls(Geo G, double* u,
+                                                         const int* __restrict__ count,
+                                                         const int* __restrict__ offset,
+                                                         const double* __restrict__ cx,
+                                                         const double* __restrict__ cy,
+                                                         const double* __restrict__ cz,
+                                                         double R) {
+    const int l = blockIdx.x + 1;
+    __shared__ double smin[kBlock], smax[kBlock];
+    for (int c = 0; c < count[l]; c++) {
+        const double x0 = cx[offset[l] + c], y0 = cy[offset[l] + c], z0 = cz[offset[l] + c];
+        const int k0 = clampi((int)(z0 - R - 0.5), 0, G.K - 1);
+        const int k1 = clampi((int)(z0 + R + 0.5), 0, G.K - 1);
+        const int j0 = clampi((int)(y0 - R - 0.5), 0, G.J - 1);
+        const int j1 = clampi((int)(y0 + R + 0.5), 0, G.J - 1);
+        const int i0 = clampi((int)(x0 - R - 0.5), 0, G.I - 1);
+        const int i1 = clampi((int)(x0 + R + 0.5), 0, G.I - 1);
+        const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
+        const int tot = (ni > 0 && nj > 0 && nk > 0) ? ni * nj * nk : 0;
+        double mn = 100000, mx = 0;
+        for (int q = threadIdx.x; q < tot; q += kBlock) {
+            const int i = i0 + q % ni, j = j0 + (q / ni) % nj, k = k0 + q / (ni * nj);
+            const double dx = i - x0, dy = j - y0, dz = k - z0;
+            const double nd = sqrt((dx * dx) + (dy * dy) + (dz * dz));
+            if (nd <= R) {
+                const double v = u[((l * G.K + k) * G.J + j) * G.I + i];
+                if (v < mn) mn = v;
+                if (v > mx) mx = v;
+            }
+        }
+        smin[threadIdx.x] = mn;
+        smax[threadIdx.x] = mx;
+        __syncthreads();
+        for (int s = kBlock / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+                smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+                smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
+            }
+            __syncthreads();
+        }
+        const double quot = 1. / (smax[0] - smin[0]);
+        const double offs = smin[0] * quot;
+        __syncthreads();
+        for (int q = threadIdx.x; q < tot; q += kBlock) {
+            const int i = i0 + q % ni, j = j0 + (q / ni) % nj, k = k0 + q / (ni * nj);
+            const double dx = i - x0, dy = j - y0, dz = k - z0;
+            const double nd = sqrt((dx * dx) + (dy * dy) + (dz * dz));
+            if (nd <= R) {
+                const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+                const double nv = (quot * u[p]) - offs;
+                if (c == 0) u[p] = nv;
+                else if (u[p] < nv) u[p] = nv;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Mmask pass alone (segmentation4D.c:1093-1101).
+__global__ __launch_bounds__(kBlock) void k_apply_mask(Geo G, double* __restrict__ u,
+                                                      const double* __restrict__ mask) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y + 1, l = blockIdx.z + 1;
+    const int w = G.I - 2;
+    if (t >= w * (G.J - 2)) return;
+    const int j = t / w + 1, i = t - (j - 1) * w + 1;
+    const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+    if (mask[p] == 0) u[p] = 0;
+}
+
+}  // namespace s4d

@@ -1,0 +1,355 @@
+"""Python host mirror of the C ABI in include/seg4d.h and include/mcf4d.h.
+
+The product is the pair of C-ABI shared libraries built from csrc/
+(lib/libseg4d_amd.so, lib/libmcf4d_amd.so); this module only binds them with
+ctypes so that tests, the smoke check and the benchmark can drive them the
+way the reference's C drivers do (segmentation4D.c:1326-1424,
+mcf_4D.c:473-524): set the globals, allocate, fill the host arrays through
+the global pointers, call the entry points.
+
+Nothing here computes: there is no CPU fallback.  Loading fails loudly if a
+library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_bool, c_char_p, c_double, c_int, c_uint64
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+SEG_LIB = os.path.join(LIB_DIR, "libseg4d_amd.so")
+MCF_LIB = os.path.join(LIB_DIR, "libmcf4d_amd.so")
+
+# order of the 8 faces everywhere: e(+x) w(-x) n(+y) s(-y) t(+z) b(-z) fw(+t) bw(-t)
+FACES = ("e", "w", "n", "s", "t", "b", "fw", "bw")
+
+GEN_MAX_OBJECTS = 64
+
+
+class GenSpec(ctypes.Structure):
+    """Mirror of seg4d_gen_spec (include/seg4d_gen.h)."""
+
+    _fields_ = [
+        ("nx", c_int), ("ny", c_int), ("nz", c_int), ("nt", c_int),
+        ("n_objects", c_int),
+        ("axes", (c_double * 3) * GEN_MAX_OBJECTS),
+        ("semi_min", c_double), ("semi_max", c_double),
+        ("margin", c_double), ("amp", c_double),
+        ("wedges", c_int), ("wedge_deg", c_double), ("shell", c_double),
+        ("noise_sigma", c_double),
+        ("background", c_double), ("foreground", c_double),
+        ("seed", c_uint64),
+    ]
+
+
+def _load(path: str) -> ctypes.CDLL:
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is not built: run `make -C 4d-image-segmentation-algorithm_amd` "
+            "(or __graft_entry__.build()) first")
+    return ctypes.CDLL(path, mode=os.RTLD_LOCAL)
+
+
+class _Lib:
+    """Common global-variable plumbing of both libraries."""
+
+    _int_globals: tuple = ()
+    _dbl_globals: tuple = ()
+    _arrays: tuple = ()
+
+    def __init__(self, path: str):
+        self.lib = _load(path)
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            if k in self._int_globals:
+                c_int.in_dll(self.lib, k).value = int(v)
+            elif k in self._dbl_globals:
+                c_double.in_dll(self.lib, k).value = float(v)
+            else:
+                raise KeyError(k)
+
+    def get(self, name):
+        if name in self._int_globals:
+            return c_int.in_dll(self.lib, name).value
+        if name in self._dbl_globals:
+            return c_double.in_dll(self.lib, name).value
+        raise KeyError(name)
+
+    @property
+    def shape(self):
+        return (self.get("lMax"), self.get("kMax"), self.get("jMax"), self.get("iMax"))
+
+    def ptr(self, name):
+        """The global pointer `name` (a host array owned by the library)."""
+        return POINTER(c_double).in_dll(self.lib, name)
+
+    def arr(self, name) -> np.ndarray:
+        """Writable numpy view (l, k, j, i) of the host array `name`."""
+        p = self.ptr(name)
+        if not p:
+            raise RuntimeError(f"{name} is not allocated")
+        n = int(np.prod(self.shape))
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(self.shape)
+
+
+class SegLib(_Lib):
+    """libseg4d_amd.so: the segmentation program's ABI."""
+
+    _int_globals = ("iMax", "jMax", "kMax", "lMax", "maxNoGSIteration",
+                    "numberOfTimeStep", "maxNoOfTimeSteps", "mod", "timeStepsNum")
+    _dbl_globals = ("eps2", "K", "segTol", "tau", "h", "omega_c", "gsTol",
+                    "omega_sigma", "theta_img", "rho_thr", "alpha", "beta",
+                    "sigma", "maxNoGSIteration_sigma", "ballRadius",
+                    "spacing_x", "spacing_y", "spacing_z",
+                    "origin_x", "origin_y", "origin_z")
+    _arrays = ("u", "prev_u", "i0", "Mmask") + FACES + tuple("g_" + f for f in FACES)
+
+    def __init__(self, path: str = SEG_LIB):
+        super().__init__(path)
+        L = self.lib
+        for fn in ("_allocateMem_", "_deallocateMem_", "_gs_coefficients_",
+                   "_segTimeStep_", "seg4dSynchronize"):
+            getattr(L, fn).restype = c_bool
+            getattr(L, fn).argtypes = []
+        for fn in ("_presmoothStep_", "_rescale4DdataToZeroOne_", "_reflection4Ddata_",
+                   "seg4dUpload", "seg4dDownload"):
+            getattr(L, fn).restype = c_bool
+            getattr(L, fn).argtypes = [POINTER(c_double)]
+        L._gf_inputImage_plus_thresholdedImage_.restype = c_bool
+        L._gf_inputImage_plus_thresholdedImage_.argtypes = [POINTER(c_double)] * 2
+        L._l2norm_.restype = c_double
+        L._l2norm_.argtypes = [POINTER(c_double)] * 2
+        L._copy4DdataFromSrcToDest_.restype = None
+        L._copy4DdataFromSrcToDest_.argtypes = [POINTER(c_double)] * 2
+        L.ijkl.restype = c_int
+        L.ijkl.argtypes = [c_int] * 4
+        L._gFunction_.restype = c_double
+        L._gFunction_.argtypes = [c_double, c_double]
+        L.seg4dSetResident.restype = c_int
+        L.seg4dSetResident.argtypes = [c_int]
+        L.seg4dSetRelativeTolerance.argtypes = [c_double]
+        L.seg4dHostWrote.argtypes = [POINTER(c_double)]
+        L.seg4dSetVerbose.argtypes = [c_int]
+        L.seg4dLastError.restype = c_char_p
+        L.seg4dLastSweeps.restype = c_int
+        L.seg4dLastResidual.restype = c_double
+        L.seg4dLastTolerance.restype = c_double
+        L.seg4dProfile.argtypes = [c_int]
+        L.seg4dProfileRead.restype = c_int
+        L.seg4dProfileRead.argtypes = [POINTER(c_double), c_int]
+        L.seg4dDeviceInfo.restype = c_int
+        L.seg4dDeviceInfo.argtypes = [c_char_p, c_int]
+        L.seg4d_gen_preset.restype = c_int
+        L.seg4d_gen_preset.argtypes = [c_int, c_uint64, POINTER(GenSpec)]
+        L.seg4d_generate.restype = c_int
+        L.seg4d_generate.argtypes = [POINTER(GenSpec), POINTER(c_double),
+                                     POINTER(c_double), POINTER(c_double)]
+        self._centres = None  # keeps no_of_centers alive
+
+    # -- configuration --------------------------------------------------------
+    def set_centres(self, counts):
+        """no_of_centers[l] for l = 0..lMax-1 (caller-owned, as the reference)."""
+        arr = (c_int * len(counts))(*[int(c) for c in counts])
+        self._centres = arr
+        ctypes.c_void_p.in_dll(self.lib, "no_of_centers").value = ctypes.addressof(arr)
+
+    def centre_arrays(self):
+        """(center_x, center_y, center_z) as ctypes double** globals."""
+        return tuple(POINTER(POINTER(c_double)).in_dll(self.lib, n)
+                     for n in ("center_x", "center_y", "center_z"))
+
+    def allocate(self, nx, ny, nz, nt, centres=None):
+        self.set(iMax=nx + 2, jMax=ny + 2, kMax=nz + 2, lMax=nt + 2)
+        if centres is not None:
+            self.set_centres(centres)
+        else:
+            self._centres = None
+            ctypes.c_void_p.in_dll(self.lib, "no_of_centers").value = None
+        if not self.lib._allocateMem_():
+            raise RuntimeError(f"_allocateMem_ failed: {self.last_error()}")
+
+    def deallocate(self):
+        self.lib._deallocateMem_()
+
+    def last_error(self) -> str:
+        return (self.lib.seg4dLastError() or b"").decode()
+
+    def check(self, ok, what):
+        if not ok:
+            raise RuntimeError(f"{what} failed: {self.last_error()}")
+
+    # -- entry points (by global array name) ----------------------------------
+    def seg_time_step(self):
+        self.check(self.lib._segTimeStep_(), "_segTimeStep_")
+        return self.lib.seg4dLastSweeps(), self.lib.seg4dLastResidual()
+
+    def gs_coefficients(self):
+        self.check(self.lib._gs_coefficients_(), "_gs_coefficients_")
+
+    def presmooth(self, name):
+        self.check(self.lib._presmoothStep_(self.ptr(name)), "_presmoothStep_")
+        return self.lib.seg4dLastSweeps(), self.lib.seg4dLastResidual()
+
+    def gf(self, a="i0", b="u"):
+        self.check(self.lib._gf_inputImage_plus_thresholdedImage_(self.ptr(a), self.ptr(b)),
+                   "_gf_inputImage_plus_thresholdedImage_")
+
+    def reflect(self, name):
+        self.check(self.lib._reflection4Ddata_(self.ptr(name)), "_reflection4Ddata_")
+
+    def rescale(self, name="u"):
+        self.check(self.lib._rescale4DdataToZeroOne_(self.ptr(name)), "_rescale4DdataToZeroOne_")
+
+    def l2norm(self, a, b):
+        return self.lib._l2norm_(self.ptr(a), self.ptr(b))
+
+    def copy(self, src, dst):
+        self.lib._copy4DdataFromSrcToDest_(self.ptr(src), self.ptr(dst))
+
+    def upload(self, name):
+        self.check(self.lib.seg4dUpload(self.ptr(name)), "seg4dUpload")
+
+    def download(self, name):
+        self.check(self.lib.seg4dDownload(self.ptr(name)), "seg4dDownload")
+
+    def host_wrote(self, name):
+        self.lib.seg4dHostWrote(self.ptr(name))
+
+    def set_resident(self, on):
+        return self.lib.seg4dSetResident(1 if on else 0)
+
+    def set_relative_tolerance(self, rel):
+        self.lib.seg4dSetRelativeTolerance(float(rel))
+
+    def set_verbose(self, on):
+        self.lib.seg4dSetVerbose(1 if on else 0)
+
+    def profile(self, on=True):
+        self.lib.seg4dProfile(1 if on else 0)
+
+    def profile_reset(self):
+        self.lib.seg4dProfileReset()
+
+    def profile_read(self):
+        out = (c_double * 9)()
+        self.lib.seg4dProfileRead(out, 9)
+        keys = ("sweep_ms", "sweep_launches", "sweep_bytes", "setup_ms", "setup_launches",
+                "step_ms", "steps", "voxel_updates", "iterations")
+        return dict(zip(keys, list(out)))
+
+    def synchronize(self):
+        self.check(self.lib.seg4dSynchronize(), "seg4dSynchronize")
+
+    def device_info(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        if self.lib.seg4dDeviceInfo(buf, 256) != 0:
+            return "unknown"
+        return buf.value.decode()
+
+    # -- synthetic inputs -------------------------------------------------------
+    def gen_spec(self, cfg: int, seed: int) -> GenSpec:
+        spec = GenSpec()
+        if self.lib.seg4d_gen_preset(int(cfg), int(seed), ctypes.byref(spec)) != 0:
+            raise ValueError(f"unknown preset {cfg}")
+        return spec
+
+    def generate(self, spec: GenSpec, image: np.ndarray, u0: np.ndarray | None):
+        """Fills padded float64 arrays in place; returns the seed centre."""
+        centre = (c_double * 3)()
+        assert image.dtype == np.float64 and image.flags.c_contiguous
+        up = None
+        if u0 is not None:
+            assert u0.dtype == np.float64 and u0.flags.c_contiguous
+            up = u0.ctypes.data_as(POINTER(c_double))
+        rc = self.lib.seg4d_generate(ctypes.byref(spec), image.ctypes.data_as(POINTER(c_double)),
+                                     up, centre)
+        if rc != 0:
+            raise ValueError("seg4d_generate rejected the spec")
+        return tuple(centre)
+
+
+class McfLib(_Lib):
+    """libmcf4d_amd.so: the MCF verification program's ABI."""
+
+    _int_globals = ("iMax", "jMax", "kMax", "lMax", "maxNoGSIteration",
+                    "numberOfTimeStep", "maxNoOfTimeSteps")
+    _dbl_globals = ("eps2", "mcfTol", "tau", "h", "omega_c", "gsTol")
+    _arrays = ("u", "prev_u", "exact_u") + FACES
+
+    def __init__(self, path: str = MCF_LIB):
+        super().__init__(path)
+        L = self.lib
+        for fn in ("_allocateMem_mcf", "_deallocateMem_mcf", "_gs_coefficients_mcf",
+                   "_mcfTimeStep", "mcf4dSynchronize"):
+            getattr(L, fn).restype = c_bool
+            getattr(L, fn).argtypes = []
+        L._set4DBoundaryToExactValues_mcf.restype = c_bool
+        L._set4DBoundaryToExactValues_mcf.argtypes = [c_double]
+        L._exactSolution_mcf.restype = None
+        L._exactSolution_mcf.argtypes = [c_double]
+        L._l2norm_mcf.restype = c_double
+        L._l2norm_mcf.argtypes = [POINTER(c_double)] * 2
+        L._copy4DdataFromSrcToDest_mcf.restype = None
+        L._copy4DdataFromSrcToDest_mcf.argtypes = [POINTER(c_double)] * 2
+        for fn in ("mcf4dUpload", "mcf4dDownload"):
+            getattr(L, fn).restype = c_bool
+            getattr(L, fn).argtypes = [POINTER(c_double)]
+        L.mcf4dSetResident.restype = c_int
+        L.mcf4dSetResident.argtypes = [c_int]
+        L.mcf4dSetRelativeTolerance.argtypes = [c_double]
+        L.mcf4dSetVerbose.argtypes = [c_int]
+        L.mcf4dLastError.restype = c_char_p
+        L.mcf4dLastSweeps.restype = c_int
+        L.mcf4dLastResidual.restype = c_double
+        L.ijkl.restype = c_int
+        L.ijkl.argtypes = [c_int] * 4
+
+    def last_error(self) -> str:
+        return (self.lib.mcf4dLastError() or b"").decode()
+
+    def check(self, ok, what):
+        if not ok:
+            raise RuntimeError(f"{what} failed: {self.last_error()}")
+
+    def allocate(self, n_interior):
+        m = n_interior + 2
+        self.set(iMax=m, jMax=m, kMax=m, lMax=m)
+        self.check(self.lib._allocateMem_mcf(), "_allocateMem_mcf")
+
+    def deallocate(self):
+        self.lib._deallocateMem_mcf()
+
+    def time_step(self):
+        self.check(self.lib._mcfTimeStep(), "_mcfTimeStep")
+        return self.lib.mcf4dLastSweeps(), self.lib.mcf4dLastResidual()
+
+    def gs_coefficients(self):
+        self.check(self.lib._gs_coefficients_mcf(), "_gs_coefficients_mcf")
+
+    def boundary(self, t):
+        self.check(self.lib._set4DBoundaryToExactValues_mcf(float(t)), "_set4DBoundary")
+
+    def exact(self, t):
+        self.lib._exactSolution_mcf(float(t))
+
+    def l2norm(self, a, b):
+        return self.lib._l2norm_mcf(self.ptr(a), self.ptr(b))
+
+    def copy(self, src, dst):
+        self.lib._copy4DdataFromSrcToDest_mcf(self.ptr(src), self.ptr(dst))
+
+    def set_verbose(self, on):
+        self.lib.mcf4dSetVerbose(1 if on else 0)
+
+    def set_resident(self, on):
+        return self.lib.mcf4dSetResident(1 if on else 0)
+
+    def upload(self, name):
+        self.check(self.lib.mcf4dUpload(self.ptr(name)), "mcf4dUpload")
+
+    def download(self, name):
+        self.check(self.lib.mcf4dDownload(self.ptr(name)), "mcf4dDownload")

@@ -1,0 +1,138 @@
+/*
+ * seg4d.h -- C ABI of libseg4d_amd.so: MI355X (gfx950) implementation of the
+ * semi-implicit 4D subjective-surface segmentation time step.
+ *
+ * Drop-in boundary.  The functions and globals below carry the names, types
+ * and meaning of the reference's serial segmentation program
+ * (88MARK08/4D-image-segmentation-algorithm, segmentation4D.h:8-62 for the
+ * prototypes, segmentation4D.c:13-60 for the globals), so a driver written
+ * against that program links against this library unchanged.  The caller
+ * sets the dimensions and parameters in the globals, calls _allocateMem_,
+ * writes the host arrays through the global pointers, and calls the entry
+ * points; the library runs them on the GPU.
+ *
+ * Coherence ("mirror" mode, the default): host arrays are authoritative
+ * between calls.  Each entry point uploads the arrays it reads and
+ * downloads the arrays it writes, with two exceptions that avoid moving
+ * eight full volumes per step:
+ *   - g_e..g_bw are uploaded only when the library did not produce them:
+ *     after _allocateMem_ (first use), or after seg4dHostWrote(g_x);
+ *   - e..bw (coefficients) are downloaded by _gs_coefficients_ and
+ *     seg4dDownload, not by _segTimeStep_ (the reference's drivers never
+ *     read them between steps).
+ * "Resident" mode (seg4dSetResident(1)): device arrays are authoritative;
+ * entry points never copy implicitly; seg4dUpload / seg4dDownload move
+ * arrays explicitly.
+ *
+ * Differences from the reference, all on error paths only:
+ *   - an entry point returns false on a HIP error, on a non-finite SOR
+ *     residual (the reference silently stops, segmentation4D.c:309) or on
+ *     arguments it cannot resolve, with the reason in seg4dLastError();
+ *   - the per-ball loops of the rescale epilogue are clamped to the array
+ *     (the reference indexes them unchecked, segmentation4D.c:1028-1090);
+ *   - no_of_centers == NULL is read as zero balls in every frame.
+ * Not thread-safe (global state, as the reference); one device, one stream.
+ */
+#ifndef SEG4D_H
+#define SEG4D_H
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "seg4d_gen.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference globals (segmentation4D.c:13-60) ------------------------ */
+extern int kMax, iMax, jMax, lMax;          /* padded dims (interior + 2)   */
+extern double *u, *prev_u, *i0, *Mmask;     /* segmentation4D.c:16-19       */
+extern double *e, *w, *n, *s, *t, *b, *fw, *bw;              /* :21-28      */
+extern double *g_e, *g_w, *g_n, *g_s, *g_t, *g_b, *g_fw, *g_bw; /* :30-37   */
+extern int *no_of_centers;                  /* :39, per frame, caller-owned */
+extern double **center_x, **center_y, **center_z;            /* :40         */
+extern double spacing_x, spacing_y, spacing_z;               /* :41         */
+extern double origin_x, origin_y, origin_z;                  /* :42         */
+extern int maxNoGSIteration;                /* :44 sweep cap of a time step */
+extern double eps2, K;                      /* :45-46                       */
+extern int numberOfTimeStep, maxNoOfTimeSteps;               /* :47-48      */
+extern double segTol;                       /* :49                          */
+extern double tau, h, omega_c, gsTol, omega_sigma;           /* :50         */
+extern double theta_img, rho_thr, alpha, beta;               /* :53         */
+extern int mod;                             /* :54                          */
+extern double sigma;                        /* :55                          */
+extern double maxNoGSIteration_sigma;       /* :56                          */
+extern int timeStepsNum;                    /* :57                          */
+extern double ballRadius;                   /* :58                          */
+
+/* ---- reference entry points on the hot path ---------------------------- */
+/* segmentation4D.h:8 / segmentation4D.c:71-155: allocates and zero-fills
+ * the 20 padded arrays (host) and their device mirrors, and the per-frame
+ * centre arrays sized by no_of_centers[l]. */
+bool _allocateMem_(void);
+/* segmentation4D.h:10 / segmentation4D.c:156-192 */
+bool _deallocateMem_(void);
+/* segmentation4D.h:12 / segmentation4D.c:194-213: sqrt(sum (a-b)^2 h^4)
+ * over the padded array; a and b must be library arrays (u, prev_u, i0,
+ * Mmask, g_*) or, in mirror mode, any host arrays of the padded size. */
+double _l2norm_(double *dataArray3DPtr1, double *dataArray3DPtr2);
+/* segmentation4D.h:14 / segmentation4D.c:446-580: the 8 face coefficients
+ * e..bw from u and g_*. */
+bool _gs_coefficients_(void);
+/* segmentation4D.h:16 / segmentation4D.c:215-315: prev_u <- u,
+ * coefficients, red-black SOR to (R <= gsTol or maxNoGSIteration sweeps),
+ * rescale epilogue. */
+bool _segTimeStep_(void);
+/* segmentation4D.h:18 / segmentation4D.c:337-409: implicit heat step in
+ * place (prev_u receives the input; ghosts reflected once). */
+bool _presmoothStep_(double *u);
+/* segmentation4D.h:28 / segmentation4D.c:582-823 */
+bool _gf_inputImage_plus_thresholdedImage_(double *i0, double *i0_thr);
+/* segmentation4D.h:30 / segmentation4D.c:866-874 */
+void _copy4DdataFromSrcToDest_(double *srcDataPtr, double *destDataPtr);
+/* segmentation4D.h:32 / segmentation4D.c:61-65 */
+int ijkl(int i, int j, int k, int l);
+/* segmentation4D.h:34 / segmentation4D.c:67-70 */
+double _gFunction_(double value, double coef);
+/* segmentation4D.h:24 / segmentation4D.c:1020-1104 */
+bool _rescale4DdataToZeroOne_(double *imagePtr);
+/* segmentation4D.h:56 / segmentation4D.c:826-864 */
+bool _reflection4Ddata_(double *u);
+
+/* ---- extensions (no counterpart in the reference) ---------------------- */
+/* 0 = mirror (default), 1 = resident.  Returns the previous mode. */
+int seg4dSetResident(int on);
+/* > 0: before every SOR solve (time step and presmoothing) the library
+ * replaces gsTol by rel^2 * sum(prev^2 h^4) over the interior, i.e. stops at
+ * relative residual ||b - A u|| / ||b|| <= rel.  0 (default): gsTol as set. */
+void seg4dSetRelativeTolerance(double rel);
+/* Explicit transfers of a library array (any global pointer above).  The
+ * coefficient arrays e..bw download in the reference layout. */
+bool seg4dUpload(double *hostArray);
+bool seg4dDownload(double *hostArray);
+/* Tells the library that the caller wrote a library array directly. */
+void seg4dHostWrote(double *hostArray);
+/* 0 silences the reference's per-step printf lines (default 1). */
+void seg4dSetVerbose(int on);
+const char *seg4dLastError(void);
+/* Statistics of the last solve: SOR iterations, residual, bound used. */
+int seg4dLastSweeps(void);
+double seg4dLastResidual(void);
+double seg4dLastTolerance(void);
+/* HIP-event profiling of the SOR passes.  seg4dProfileRead fills
+ * out[0..8] = {sweep_ms, sweep_launches, sweep_bytes, setup_ms,
+ * setup_launches, step_ms, steps, voxel_updates, iterations}
+ * and returns the number of values written. */
+void seg4dProfile(int on);
+int seg4dProfileRead(double *out, int nout);
+void seg4dProfileReset(void);
+/* Blocks until all queued device work is done. */
+bool seg4dSynchronize(void);
+/* Device name / CU count / arch into buf; returns 0 on success. */
+int seg4dDeviceInfo(char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,143 @@
+"""Writes the golden fixtures of this directory by running the reference's own
+C programs (segmentation4D.c, mcf_4D.c), compiled from /root/reference into
+oracle/_ref/ by `make -C oracle ref`, on small seeded inputs.
+
+Each .npz holds inputs and the reference's outputs (float64, bitwise):
+  seg_h1.npz   h = 1 (power of two), theta = rho = 0.5, balls in the rescale
+               epilogue, a partial Mmask, 3 tolerance-driven time steps
+  seg_h001.npz the reference main()'s own parameters (h = tau = 0.01,
+               eps2 = 1, K = 1, sigma = h^2/8, 5 sweeps cap, 10 presmoothing
+               sweeps cap), 2 time steps
+  mcf.npz      the MCF program's parameters (h = 2.5/n, tau = eps2 = h^2),
+               3 time steps, L2 error per step
+
+Run:  make -C oracle ref && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+from ctypes import c_double, c_int
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+from harness import DP, RefMcf, RefSeg  # noqa: E402
+
+FACES = ("e", "w", "n", "s", "t", "b", "fw", "bw")
+
+
+def zero_ghosts(a):
+    a = a.copy()
+    a[0] = a[-1] = 0
+    a[:, 0] = a[:, -1] = 0
+    a[:, :, 0] = a[:, :, -1] = 0
+    a[..., 0] = a[..., -1] = 0
+    return a
+
+
+def seg_case(name, shape, seed, dparams, iparams, steps, counts=None, centres=None,
+             radius=0.0, partial_mask=False):
+    rng = np.random.default_rng(seed)
+    i0 = zero_ghosts(np.round(rng.uniform(0, 255, size=shape)) / 255.0)
+    thr = zero_ghosts(np.round(rng.uniform(0, 255, size=shape)) / 255.0)
+    u0 = zero_ghosts(rng.uniform(0.05, 1.0, size=shape))
+    mask = np.ones(shape)
+    if partial_mask:
+        mask = (rng.uniform(size=shape) > 0.1).astype(np.float64)
+    L_ = shape[0]
+    counts = counts or [0] * L_
+    ref = RefSeg()
+    ref.allocate(shape, counts)
+    if centres:
+        ref.set_centres(centres)
+    ref.setd(ballRadius=radius, **dparams)
+    ref.seti(**iparams)
+    ref.arr("i0")[...] = i0
+    ref.arr("u")[...] = thr
+    ref.arr("Mmask")[...] = mask
+    gf = ref.fn("_gf_inputImage_plus_thresholdedImage_", argtypes=[DP, DP])
+    assert gf(ref.ptr("i0"), ref.ptr("u"))
+    out = dict(i0=i0, thr=thr, u0=u0, mask=mask, counts=np.array(counts, np.int32),
+               centres=np.array([c for lst in (centres or []) for c in lst] or
+                                np.zeros((0, 3))).reshape(-1, 3),
+               radius=np.float64(radius),
+               i0_s=ref.arr("i0").copy(), thr_s=ref.arr("u").copy(),
+               prev_gf=ref.arr("prev_u").copy())
+    for f in FACES:
+        out["g_" + f] = ref.arr("g_" + f).copy()
+    ref.arr("u")[...] = u0
+    step = ref.fn("_segTimeStep_")
+    l2 = ref.fn("_l2norm_", restype=c_double, argtypes=[DP, DP])
+    norms = []
+    for st in range(steps):
+        c_int.in_dll(ref.L, "numberOfTimeStep").value = st + 1
+        assert step()
+        out[f"u{st + 1}"] = ref.arr("u").copy()
+        norms.append(l2(ref.ptr("prev_u"), ref.ptr("u")))
+        if st == 0:
+            for f in FACES:
+                out["coef1_" + f] = ref.arr(f).copy()
+    out["prev_last"] = ref.arr("prev_u").copy()
+    out["l2"] = np.array(norms)
+    out["steps"] = np.int32(steps)
+    for k, v in dparams.items():
+        out["p_" + k] = np.float64(v)
+    for k, v in iparams.items():
+        out["p_" + k] = np.int64(v)
+    ref.deallocate()
+    np.savez_compressed(os.path.join(HERE, name), **out)
+    print("wrote", name)
+
+
+def mcf_case(name, m, steps):
+    ref = RefMcf()
+    ref.seti(iMax=m, jMax=m, kMax=m, lMax=m, maxNoGSIteration=1000)
+    hh = 2.5 / (m - 2)
+    ref.setd(h=hh, tau=hh * hh, eps2=hh * hh, gsTol=1e-12, omega_c=1.8, mcfTol=1e-12)
+    assert ref.fn("_allocateMem_mcf")()
+    exact = ref.fn("_exactSolution_mcf", restype=None, argtypes=[c_double])
+    copy = ref.fn("_copy4DdataFromSrcToDest_mcf", restype=None, argtypes=[DP, DP])
+    step = ref.fn("_mcfTimeStep")
+    l2 = ref.fn("_l2norm_mcf", restype=c_double, argtypes=[DP, DP])
+    exact(0.0)
+    copy(ref.ptr("exact_u"), ref.ptr("u"))
+    out = dict(u0=ref.arr("u").copy(), h=np.float64(hh), m=np.int32(m))
+    errs = []
+    for st in range(1, steps + 1):
+        c_int.in_dll(ref.L, "numberOfTimeStep").value = st
+        assert step()
+        out[f"u{st}"] = ref.arr("u").copy()
+        exact(st * hh * hh)
+        errs.append(l2(ref.ptr("exact_u"), ref.ptr("u")))
+    out["exact_last"] = ref.arr("exact_u").copy()
+    out["err"] = np.array(errs)
+    out["steps"] = np.int32(steps)
+    ref.fn("_deallocateMem_mcf")()
+    np.savez_compressed(os.path.join(HERE, name), **out)
+    print("wrote", name)
+
+
+def main():
+    shape = (5, 6, 7, 8)  # padded (lMax, kMax, jMax, iMax)
+    counts = [0, 1, 2, 1, 0]
+    centres = [[], [(3.0, 3.0, 2.0)], [(2.0, 4.0, 3.0), (5.0, 3.0, 3.0)], [(4.0, 3.0, 2.5)], []]
+    seg_case("seg_h1.npz", shape, 11,
+             dict(h=1.0, tau=0.5, eps2=1e-6, K=1000.0, omega_c=1.8, omega_sigma=1.3,
+                  sigma=0.5, gsTol=1e-14, theta_img=0.5, rho_thr=0.5,
+                  maxNoGSIteration_sigma=50.0),
+             dict(maxNoGSIteration=500), steps=3, counts=counts, centres=centres,
+             radius=1.6, partial_mask=True)
+    h = 0.01
+    seg_case("seg_h001.npz", (5, 5, 6, 7), 12,
+             dict(h=h, tau=0.01, eps2=1.0, K=1.0, omega_c=1.8, omega_sigma=1.3,
+                  sigma=h * h / 8, gsTol=1e-12, theta_img=0.5, rho_thr=0.5,
+                  maxNoGSIteration_sigma=10.0),
+             dict(maxNoGSIteration=5), steps=2)
+    mcf_case("mcf.npz", 7, 3)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,277 @@
+"""ctypes wrappers of the CPU oracle (oracle/liboracle.so) and of the
+reference's own C programs compiled as libraries (oracle/_ref/*.so).
+
+Test infrastructure only: the oracle is the checker, never the thing
+measured or shipped.  The reference libraries exist only where
+`make -C oracle ref` could compile the reference sources (this container);
+tests that need them skip elsewhere.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import sys
+from ctypes import POINTER, c_bool, c_double, c_int, c_void_p
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "4d-image-segmentation-algorithm_amd")
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
+
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+DP = POINTER(c_double)
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(DP)
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("iMax", c_int), ("jMax", c_int), ("kMax", c_int), ("lMax", c_int),
+        ("h", c_double), ("tau", c_double), ("eps2", c_double), ("K", c_double),
+        ("omega_c", c_double), ("omega_sigma", c_double), ("sigma", c_double),
+        ("gsTol", c_double), ("theta_img", c_double), ("rho_thr", c_double),
+        ("maxNoGSIteration", c_int), ("maxNoGSIteration_sigma", c_double),
+        ("nthreads", c_int), ("par_reduce", c_int),
+    ]
+
+
+class OrcBalls(ctypes.Structure):
+    _fields_ = [("count", POINTER(c_int)), ("offset", POINTER(c_int)),
+                ("cx", DP), ("cy", DP), ("cz", DP), ("radius", c_double)]
+
+
+def params(shape, **kw) -> OrcParams:
+    """shape = padded (lMax, kMax, jMax, iMax)."""
+    p = OrcParams()
+    p.lMax, p.kMax, p.jMax, p.iMax = shape
+    defaults = dict(h=1.0, tau=0.5, eps2=1e-6, K=1000.0, omega_c=1.8, omega_sigma=1.3,
+                    sigma=0.5, gsTol=1e-12, theta_img=1.0, rho_thr=0.0,
+                    maxNoGSIteration=100000, maxNoGSIteration_sigma=100000.0, nthreads=1,
+                    par_reduce=0)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(p, k, v)
+    return p
+
+
+class Oracle:
+    """The clean-room FP64 CPU restatement (oracle/seg4d_oracle.c)."""
+
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: make -C oracle")
+        L = ctypes.CDLL(path, mode=os.RTLD_LOCAL)
+        self.L = L
+        P = POINTER(OrcParams)
+        L.orc_gfunction.restype = c_double
+        L.orc_gfunction.argtypes = [c_double, c_double]
+        L.orc_l2norm.restype = c_double
+        L.orc_l2norm.argtypes = [P, DP, DP]
+        L.orc_sumsq_interior.restype = c_double
+        L.orc_sumsq_interior.argtypes = [P, DP]
+        L.orc_reflection.argtypes = [P, DP]
+        L.orc_presmooth.restype = c_int
+        L.orc_presmooth.argtypes = [P, DP, DP, DP]
+        L.orc_gf.argtypes = [P, DP, DP, DP, POINTER(DP), POINTER(c_int), DP]
+        L.orc_coefficients.argtypes = [P, DP, POINTER(DP), POINTER(DP)]
+        L.orc_face_norm.restype = c_double
+        L.orc_face_norm.argtypes = [P, DP, c_int, c_int, c_int, c_int, c_int]
+        L.orc_sor.restype = c_int
+        L.orc_sor.argtypes = [P, DP, DP, POINTER(DP), DP, DP]
+        L.orc_sor_pass.argtypes = [P, DP, DP, POINTER(DP), c_int]
+        L.orc_residual.restype = c_double
+        L.orc_residual.argtypes = [P, DP, DP, POINTER(DP)]
+        L.orc_rescale.argtypes = [P, DP, DP, POINTER(OrcBalls)]
+        L.orc_seg_time_step.restype = c_int
+        L.orc_seg_time_step.argtypes = [P, DP, DP, POINTER(DP), POINTER(DP), DP,
+                                        POINTER(OrcBalls), DP]
+        L.orc_mcf_boundary.argtypes = [P, DP, c_double]
+        L.orc_mcf_exact.argtypes = [P, DP, c_double]
+        L.orc_mcf_time_step.restype = c_int
+        L.orc_mcf_time_step.argtypes = [P, DP, DP, POINTER(DP), c_int, DP]
+        L.orc_mask.argtypes = [P, DP, POINTER(ctypes.c_ubyte)]
+        L.orc_iou.restype = c_double
+        L.orc_iou.argtypes = [P, POINTER(ctypes.c_ubyte), POINTER(ctypes.c_ubyte)]
+        L.orc_wtime.restype = c_double
+        L.orc_max_threads.restype = c_int
+
+    @staticmethod
+    def _arr8(arrs):
+        return (DP * 8)(*[dptr(a) for a in arrs])
+
+    def gfunction(self, v, c):
+        return self.L.orc_gfunction(v, c)
+
+    def l2norm(self, p, a, b):
+        return self.L.orc_l2norm(ctypes.byref(p), dptr(a), dptr(b))
+
+    def sumsq_interior(self, p, a):
+        return self.L.orc_sumsq_interior(ctypes.byref(p), dptr(a))
+
+    def reflection(self, p, a):
+        self.L.orc_reflection(ctypes.byref(p), dptr(a))
+
+    def presmooth(self, p, a, prev):
+        R = c_double()
+        z = self.L.orc_presmooth(ctypes.byref(p), dptr(a), dptr(prev), ctypes.byref(R))
+        return z, R.value
+
+    def gf(self, p, i0, thr, prev, g):
+        it = (c_int * 2)()
+        res = (c_double * 2)()
+        self.L.orc_gf(ctypes.byref(p), dptr(i0), dptr(thr), dptr(prev), self._arr8(g), it, res)
+        return list(it), list(res)
+
+    def coefficients(self, p, u, g, coef):
+        self.L.orc_coefficients(ctypes.byref(p), dptr(u),
+                                self._arr8(g) if g is not None else None, self._arr8(coef))
+
+    def face_norm(self, p, u, i, j, k, l, f):
+        return self.L.orc_face_norm(ctypes.byref(p), dptr(u), i, j, k, l, f)
+
+    def sor(self, p, u, prev, coef, hist=None):
+        R = c_double()
+        z = self.L.orc_sor(ctypes.byref(p), dptr(u), dptr(prev), self._arr8(coef),
+                           ctypes.byref(R), dptr(hist) if hist is not None else None)
+        return z, R.value
+
+    def sor_pass(self, p, u, prev, coef, colour):
+        self.L.orc_sor_pass(ctypes.byref(p), dptr(u), dptr(prev), self._arr8(coef), colour)
+
+    def residual(self, p, u, prev, coef):
+        return self.L.orc_residual(ctypes.byref(p), dptr(u), dptr(prev), self._arr8(coef))
+
+    def make_balls(self, counts, centres, radius):
+        """counts[l]; centres[l] = list of (cx, cy, cz)."""
+        cnt = np.asarray(counts, dtype=np.int32)
+        off = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int32)
+        flat = [c for l in range(len(counts)) for c in centres[l][: counts[l]]]
+        cx = np.array([c[0] for c in flat] or [0.0])
+        cy = np.array([c[1] for c in flat] or [0.0])
+        cz = np.array([c[2] for c in flat] or [0.0])
+        keep = (cnt, off, cx, cy, cz)
+        b = OrcBalls(cnt.ctypes.data_as(POINTER(c_int)), off.ctypes.data_as(POINTER(c_int)),
+                     dptr(cx), dptr(cy), dptr(cz), float(radius))
+        b._keep = keep
+        return b
+
+    def rescale(self, p, u, mask, balls=None):
+        self.L.orc_rescale(ctypes.byref(p), dptr(u), dptr(mask),
+                           ctypes.byref(balls) if balls is not None else None)
+
+    def seg_time_step(self, p, u, prev, g, coef, mask, balls=None):
+        R = c_double()
+        z = self.L.orc_seg_time_step(ctypes.byref(p), dptr(u), dptr(prev), self._arr8(g),
+                                     self._arr8(coef), dptr(mask),
+                                     ctypes.byref(balls) if balls is not None else None,
+                                     ctypes.byref(R))
+        return z, R.value
+
+    def mcf_boundary(self, p, u, t):
+        self.L.orc_mcf_boundary(ctypes.byref(p), dptr(u), t)
+
+    def mcf_exact(self, p, ex, t):
+        self.L.orc_mcf_exact(ctypes.byref(p), dptr(ex), t)
+
+    def mcf_time_step(self, p, u, prev, coef, step):
+        R = c_double()
+        z = self.L.orc_mcf_time_step(ctypes.byref(p), dptr(u), dptr(prev), self._arr8(coef),
+                                     step, ctypes.byref(R))
+        return z, R.value
+
+    def mask(self, p, u):
+        m = np.zeros(u.shape, dtype=np.uint8)
+        self.L.orc_mask(ctypes.byref(p), dptr(u), m.ctypes.data_as(POINTER(ctypes.c_ubyte)))
+        return m
+
+    def iou(self, p, a, b):
+        return self.L.orc_iou(ctypes.byref(p), a.ctypes.data_as(POINTER(ctypes.c_ubyte)),
+                              b.ctypes.data_as(POINTER(ctypes.c_ubyte)))
+
+
+def zeros(shape, n=1):
+    return [np.zeros(shape, dtype=np.float64) for _ in range(n)]
+
+
+# ----------------------------------------------------------------------------
+# The reference's own programs, compiled from /root/reference into oracle/_ref
+# by `make -C oracle ref` (main renamed).  Driven through their globals.
+# ----------------------------------------------------------------------------
+def ref_available(name="libref_seg.so"):
+    return os.path.exists(os.path.join(REF_DIR, name))
+
+
+class RefProgram:
+    def __init__(self, name):
+        path = os.path.join(REF_DIR, name)
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built (make -C oracle ref)")
+        self.L = ctypes.CDLL(path, mode=os.RTLD_LOCAL)
+
+    def seti(self, **kw):
+        for k, v in kw.items():
+            c_int.in_dll(self.L, k).value = int(v)
+
+    def setd(self, **kw):
+        for k, v in kw.items():
+            c_double.in_dll(self.L, k).value = float(v)
+
+    def shape(self):
+        g = lambda n: c_int.in_dll(self.L, n).value  # noqa: E731
+        return (g("lMax"), g("kMax"), g("jMax"), g("iMax"))
+
+    def ptr(self, name):
+        return DP.in_dll(self.L, name)
+
+    def arr(self, name):
+        shp = self.shape()
+        return np.ctypeslib.as_array(self.ptr(name), shape=(int(np.prod(shp)),)).reshape(shp)
+
+    def fn(self, name, restype=c_bool, argtypes=()):
+        f = getattr(self.L, name)
+        f.restype = restype
+        f.argtypes = list(argtypes)
+        return f
+
+
+class RefSeg(RefProgram):
+    """segmentation4D.c as a library (globals of segmentation4D.c:13-60)."""
+
+    def __init__(self):
+        super().__init__("libref_seg.so")
+        self._counts = None
+
+    def allocate(self, shape, counts=None):
+        L_, K_, J_, I_ = shape
+        self.seti(iMax=I_, jMax=J_, kMax=K_, lMax=L_)
+        if counts is None:
+            counts = [0] * L_
+        self._counts = (c_int * L_)(*counts)
+        c_void_p.in_dll(self.L, "no_of_centers").value = ctypes.addressof(self._counts)
+        assert self.fn("_allocateMem_")()
+
+    def set_centres(self, centres):
+        cx = POINTER(DP).in_dll(self.L, "center_x")
+        cy = POINTER(DP).in_dll(self.L, "center_y")
+        cz = POINTER(DP).in_dll(self.L, "center_z")
+        for l, lst in enumerate(centres):
+            for c, (x, y, z) in enumerate(lst):
+                cx[l][c] = x
+                cy[l][c] = y
+                cz[l][c] = z
+
+    def deallocate(self):
+        self.fn("_deallocateMem_")()
+
+
+class RefMcf(RefProgram):
+    def __init__(self, name="libref_mcf.so"):
+        super().__init__(name)

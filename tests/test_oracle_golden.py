@@ -1,0 +1,77 @@
+"""The oracle reproduces, bitwise, the golden fixtures that the reference's own
+compiled programs wrote (tests/golden/make_golden.py)."""
+import os
+
+import numpy as np
+import pytest
+
+from harness import GOLDEN, params, zeros
+
+FACES = ("e", "w", "n", "s", "t", "b", "fw", "bw")
+
+
+def load(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def seg_params(fx):
+    shape = fx["i0"].shape
+    kw = {k[2:]: fx[k].item() for k in fx.files if k.startswith("p_")}
+    return params(shape, **kw)
+
+
+def balls_of(oracle, fx):
+    counts = [int(c) for c in fx["counts"]]
+    flat = [tuple(r) for r in fx["centres"]]
+    centres, q = [], 0
+    for c in counts:
+        centres.append(flat[q:q + c])
+        q += c
+    return oracle.make_balls(counts, centres, float(fx["radius"]))
+
+
+@pytest.mark.parametrize("name", ["seg_h1.npz", "seg_h001.npz"])
+def test_seg_golden(oracle, name):
+    fx = load(name)
+    p = seg_params(fx)
+    shape = fx["i0"].shape
+    a, b, u0, mask = fx["i0"].copy(), fx["thr"].copy(), fx["u0"].copy(), fx["mask"].copy()
+    prev = np.zeros(shape)
+    g = zeros(shape, 8)
+    oracle.gf(p, a, b, prev, g)
+    assert np.array_equal(a, fx["i0_s"])
+    assert np.array_equal(b, fx["thr_s"])
+    assert np.array_equal(prev, fx["prev_gf"])
+    for f, gf in zip(FACES, g):
+        assert np.array_equal(gf, fx["g_" + f]), f
+    balls = balls_of(oracle, fx)
+    u = u0.copy()
+    coef = zeros(shape, 8)
+    for st in range(int(fx["steps"])):
+        oracle.seg_time_step(p, u, prev, g, coef, mask, balls)
+        if st == 0:
+            for f, c in zip(FACES, coef):
+                assert np.array_equal(c, fx["coef1_" + f]), f
+        assert np.array_equal(u, fx[f"u{st + 1}"]), f"step {st + 1}"
+        assert oracle.l2norm(p, prev, u) == fx["l2"][st]
+    assert np.array_equal(prev, fx["prev_last"])
+
+
+def test_mcf_golden(oracle):
+    fx = load("mcf.npz")
+    m, hh = int(fx["m"]), float(fx["h"])
+    shape = (m, m, m, m)
+    p = params(shape, h=hh, tau=hh * hh, eps2=hh * hh, gsTol=1e-12, omega_c=1.8,
+               maxNoGSIteration=1000)
+    u = np.zeros(shape)
+    oracle.mcf_exact(p, u, 0.0)
+    assert np.array_equal(u, fx["u0"])
+    prev = np.zeros(shape)
+    coef = zeros(shape, 8)
+    ex = np.zeros(shape)
+    for st in range(1, int(fx["steps"]) + 1):
+        oracle.mcf_time_step(p, u, prev, coef, st)
+        assert np.array_equal(u, fx[f"u{st}"]), st
+        oracle.mcf_exact(p, ex, st * hh * hh)
+        assert oracle.l2norm(p, ex, u) == fx["err"][st - 1]
+    assert np.array_equal(ex, fx["exact_last"])
