@@ -421,10 +421,13 @@ API void seg4dProfileReset(void) {
 API int seg4dProfileRead(double* out, int nout) {
     if (!g_eng || !out) return 0;
     const s4d::Profile& p = g_eng->prof;
-    const double v[9] = {p.sweep_ms, (double)p.sweep_launches, p.sweep_bytes,
-                         p.setup_ms, (double)p.setup_launches, p.step_ms,
-                         (double)p.steps, (double)p.voxel_updates, (double)p.iterations};
-    int k = nout < 9 ? nout : 9;
+    const double v[14] = {p.sweep_ms[0], (double)p.sweep_launches[0],
+                          p.sweep_ms[1], (double)p.sweep_launches[1],
+                          p.sweep_ms[2], (double)p.sweep_launches[2],
+                          p.setup_ms, (double)p.setup_launches, p.step_ms, (double)p.steps,
+                          (double)p.voxel_updates, (double)p.iterations,
+                          (double)p.nred, (double)p.nblack};
+    int k = nout < 14 ? nout : 14;
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;
 }

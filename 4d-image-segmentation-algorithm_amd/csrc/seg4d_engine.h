@@ -38,17 +38,20 @@ struct Balls {              // rescale-epilogue balls, host copies
     int total = 0;
 };
 
-struct Profile {            // HIP-event timing of the SOR colour passes
+struct Profile {            // HIP-event timing (events on the engine's stream)
     bool on = false;
-    double sweep_ms = 0.0;  // summed duration of every sweep launch
-    long sweep_launches = 0;
-    double sweep_bytes = 0; // algorithmic bytes of the working launches
+    // SOR colour passes by kernel variant: 0 = red without residual (first
+    // sweep of a solve), 1 = red with the fused residual, 2 = black with the
+    // fused residual (the last black launch of a solve is a no-op).
+    double sweep_ms[3] = {0, 0, 0};
+    long sweep_launches[3] = {0, 0, 0};
     double setup_ms = 0.0;  // coefficient / set-up kernel
     long setup_launches = 0;
     double step_ms = 0.0;   // whole time steps (setup + solve + epilogue)
     long steps = 0;
     long voxel_updates = 0; // interior voxels x reference iterations
     long iterations = 0;
+    long nred = 0, nblack = 0;  // interior points per colour
 };
 
 enum ArrayId {

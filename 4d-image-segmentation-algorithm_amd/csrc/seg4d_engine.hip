@@ -244,23 +244,26 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
         cb.p[f] = cf_[1][f];
     }
     const int cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
-    // interior red / black point counts for the byte model
+    // interior red / black point counts for the byte model: the corner
+    // (1,1,1,1) is red, so red has the extra point when the count is odd
     const long nint = (long)(I_ - 2) * (J_ - 2) * (K_ - 2) * (L_ - 2);
-    const long nred = (nint + 1) / 2, nblk = nint - nred;  // close enough for odd counts
+    const long nred = (nint + 1) / 2, nblk = nint - nred;
     int pe = 0;  // profiling event index
-    auto ev_pair = [&](int& idx) -> hipEvent_t* {
-        while ((int)pev_.size() < idx + 2) {
+    std::vector<int> kinds;
+    auto ev_pair = [&](int kind) -> hipEvent_t* {
+        while ((int)pev_.size() < pe + 2) {
             hipEvent_t e;
-            hipEventCreate(&e);
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
             pev_.push_back(e);
         }
-        hipEvent_t* r = &pev_[idx];
-        idx += 2;
+        hipEvent_t* r = &pev_[pe];
+        pe += 2;
+        kinds.push_back(kind);
         return r;
     };
     auto red = [&](int z, bool resid) -> int {
-        hipEvent_t* e = prof.on ? ev_pair(pe) : nullptr;
-        if (e) hipEventRecord(e[0], st_);
+        hipEvent_t* e = prof.on ? ev_pair(resid ? 1 : 0) : nullptr;
+        if (e) (void)hipEventRecord(e[0], st_);
         const double* xin = ur_[(z - 1) & 1];
         double* xout = ur_[z & 1];
         if (resid)
@@ -269,15 +272,15 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
         else
             k_sor_sweep<0><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr,
                                                      pc_[0], nullptr, nullptr);
-        if (e) hipEventRecord(e[1], st_);
+        if (e) (void)hipEventRecord(e[1], st_);
         return launch_check("k_sor_sweep red");
     };
     auto black = [&](int z) -> int {
-        hipEvent_t* e = prof.on ? ev_pair(pe) : nullptr;
-        if (e) hipEventRecord(e[0], st_);
+        hipEvent_t* e = prof.on ? ev_pair(2) : nullptr;
+        if (e) (void)hipEventRecord(e[0], st_);
         k_sor_sweep<2><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb,
                                                  pc_[1], part_b, ctl_);
-        if (e) hipEventRecord(e[1], st_);
+        if (e) (void)hipEventRecord(e[1], st_);
         return launch_check("k_sor_sweep black");
     };
     if (red(1, false) || black(1)) return -1;
@@ -309,14 +312,14 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
         if (check(hipStreamSynchronize(st_), "prof sync")) return -1;
         for (int q = 0; q < pe; q += 2) {
             float ms = 0.f;
-            hipEventElapsedTime(&ms, pev_[q], pev_[q + 1]);
-            prof.sweep_ms += ms;
-            prof.sweep_launches++;
+            (void)hipEventElapsedTime(&ms, pev_[q], pev_[q + 1]);
+            prof.sweep_ms[kinds[q / 2]] += ms;
+            prof.sweep_launches[kinds[q / 2]]++;
         }
-        // working launches: red(1..z+1) and black(1..z); 96 B per colour point
-        prof.sweep_bytes += 96.0 * ((double)(z + 1) * nred + (double)z * nblk);
         prof.iterations += z;
         prof.voxel_updates += (long)z * nint;
+        prof.nred = nred;
+        prof.nblack = nblk;
     }
     return 0;
 }

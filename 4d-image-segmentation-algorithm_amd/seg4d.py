@@ -235,10 +235,11 @@ class SegLib(_Lib):
         self.lib.seg4dProfileReset()
 
     def profile_read(self):
-        out = (c_double * 9)()
-        self.lib.seg4dProfileRead(out, 9)
-        keys = ("sweep_ms", "sweep_launches", "sweep_bytes", "setup_ms", "setup_launches",
-                "step_ms", "steps", "voxel_updates", "iterations")
+        out = (c_double * 14)()
+        self.lib.seg4dProfileRead(out, 14)
+        keys = ("red0_ms", "red0_launches", "red1_ms", "red1_launches",
+                "black_ms", "black_launches", "setup_ms", "setup_launches",
+                "step_ms", "steps", "voxel_updates", "iterations", "nred", "nblack")
         return dict(zip(keys, list(out)))
 
     def synchronize(self):

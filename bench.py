@@ -1,0 +1,237 @@
+"""Benchmark of the plain FP64 time step on one MI355X.
+
+Workload (default): one synthetic 4D volume 256x256x128x16 (x,y,z,t), six
+ellipsoids with time-periodic radii and wedge gaps, noise sigma 25; edge
+indicator with K = 1000 after Gaussian presmoothing (sigma = 1 voxel); then
+30 semi-implicit time steps (eps = 1e-3, tau = 0.5, h = 1), each solved by
+red-black SOR (omega = 1.8) to relative residual 1e-6, in FP64.
+
+One bench step = one time step (coefficient assembly + SOR to tolerance +
+epilogue) with every array resident in HBM.  The timed region covers
+exactly --steps time steps from the initial function, after --warmup
+untimed steps (the state is reset to u0 after the warm-up).
+
+Reported:
+  value          SOR voxel-updates/s = interior voxels x SOR iterations / time
+  roofline       the red SOR pass with the fused residual (k_sor_sweep<1>):
+                 96 algorithmic bytes per red point per launch, divided by its
+                 average launch duration from HIP events on the library's
+                 stream over the timed region; peak 8 TB/s (MI355X HBM3E)
+  cpu_baseline   the oracle's OpenMP restatement of the reference path
+                 (OpenMP over the t axis, as the reference's OpenMP program),
+                 timed on a bounded sample on this host: one coefficient pass
+                 plus a fixed number of SOR iterations of step 1, extrapolated
+                 with the GPU run's mean sweeps per step.
+
+Usage: python bench.py [--steps 30] [--warmup 1] [--no-cpu] [--cpu-iters 10]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "4d-image-segmentation-algorithm_amd")
+sys.path.insert(0, PKG)
+
+METRIC = "SOR voxel-updates/s + HBM GB/s; time-to-tolerance plain FP64 vs mixed+learned"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = 96.0  # 8 coefficients + prev + u in/out + other colour, FP64
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def load_traffic():
+    """HBM bytes per launch of k_sor_sweep<1> measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), as committed in
+    profiles/sor_sweep_pmc.md; None if that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "sor_sweep_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def params(lib):
+    lib.set(h=1.0, tau=0.5, eps2=1e-6, K=1000.0, omega_c=1.8, omega_sigma=1.3, sigma=0.5,
+            theta_img=1.0, rho_thr=0.0, gsTol=0.0, maxNoGSIteration=100000,
+            maxNoGSIteration_sigma=100000.0)
+    lib.set_relative_tolerance(1e-6)
+
+
+def cpu_baseline(lib, shape, u0, iters, mean_sweeps, nint):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from harness import Oracle, params as oparams, zeros
+
+    O = Oracle()
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    threads = max(1, min(avail, 16, shape[0] - 2))
+    p = oparams(shape, h=1.0, tau=0.5, eps2=1e-6, omega_c=1.8, gsTol=-1.0,
+                maxNoGSIteration=iters, nthreads=threads, par_reduce=1)
+    g = [np.ascontiguousarray(lib.arr("g_" + f)) for f in ("e", "w", "n", "s", "t", "b", "fw", "bw")]
+    u = u0.copy()
+    prev = u0.copy()
+    coef = zeros(shape, 8)
+    t0 = O.L.orc_wtime()
+    O.coefficients(p, u, g, coef)
+    t1 = O.L.orc_wtime()
+    O.sor(p, u, prev, coef)
+    t2 = O.L.orc_wtime()
+    t_coef, t_iter = t1 - t0, (t2 - t1) / iters
+    step_s = t_coef + mean_sweeps * t_iter
+    return {
+        "value": nint * mean_sweeps / step_s,
+        "unit": "voxel-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle OpenMP restatement of the reference path: 1 coefficient pass "
+                   f"({t_coef:.2f} s) + {iters} SOR iterations ({t_iter:.3f} s each) of "
+                   f"time step 1 on the same volume; per-step time extrapolated with "
+                   f"{mean_sweeps:.1f} sweeps/step"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--preset", type=int, default=1, help="synthetic volume preset (0 or 1)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
+              f"(asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+
+    import seg4d
+
+    lib = seg4d.SegLib()
+    lib.set_verbose(False)
+    spec = lib.gen_spec(args.preset, args.preset)
+    nx, ny, nz, nt = spec.nx, spec.ny, spec.nz, spec.nt
+    shape = (nt + 2, nz + 2, ny + 2, nx + 2)
+    nint = nx * ny * nz * nt
+    lib.allocate(nx, ny, nz, nt)
+    params(lib)
+    log(f"[bench] volume {nx}x{ny}x{nz}x{nt} on {lib.device_info()}")
+    t = time.perf_counter()
+    u0 = np.zeros(shape)
+    lib.generate(spec, lib.arr("i0"), u0)
+    lib.arr("u")[...] = 0.0
+    lib.arr("Mmask")[...] = 1.0
+    log(f"[bench] generated in {time.perf_counter() - t:.2f} s")
+
+    # edge indicator (mirror mode: includes the host<->device copies)
+    t = time.perf_counter()
+    lib.gf("i0", "u")
+    edge_s = time.perf_counter() - t
+    log(f"[bench] edge indicator {edge_s:.2f} s (PCIe-inclusive)")
+
+    lib.set_resident(True)
+    lib.arr("u")[...] = u0
+    lib.upload("u")
+    lib.upload("Mmask")
+    for w in range(args.warmup):
+        lib.set(numberOfTimeStep=w + 1)
+        lib.seg_time_step()
+    lib.arr("u")[...] = u0
+    lib.upload("u")
+    lib.synchronize()
+    log("[bench] warm-up done")
+
+    lib.profile(True)
+    lib.profile_reset()
+    sweeps = []
+    lib.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        lib.set(numberOfTimeStep=s + 1)
+        z, _ = lib.seg_time_step()
+        sweeps.append(z)
+    lib.synchronize()
+    t1 = time.perf_counter()
+    prof = lib.profile_read()
+    lib.profile(False)
+    total = t1 - t0
+    log(f"[bench] {args.steps} steps in {total:.3f} s, sweeps/step {sweeps}")
+
+    lib.download("u")
+    u = lib.arr("u")
+    inner = u[1:-1, 1:-1, 1:-1, 1:-1]
+    thr = 0.5 * (inner.max() + inner.min())
+    mask_frac = float((inner >= thr).mean())
+
+    nred, nblack = prof["nred"], prof["nblack"]
+    red1_avg_s = prof["red1_ms"] / max(prof["red1_launches"], 1) / 1e3
+    achieved = nred * BYTES_PER_UPDATE / red1_avg_s / 1e9
+    sweep_ms = prof["red0_ms"] + prof["red1_ms"] + prof["black_ms"]
+    it = prof["iterations"]
+    sweep_bytes = BYTES_PER_UPDATE * ((it + args.steps) * nred + it * nblack)
+    traffic = load_traffic()
+    value = prof["voxel_updates"] / total
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline sample ...")
+        cpu = cpu_baseline(lib, shape, u0, args.cpu_iters, float(np.mean(sweeps)), nint)
+        log(f"[bench] CPU baseline {cpu['value']:.3e} voxel-updates/s on {cpu['cores']} threads")
+    lib.deallocate()
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"plain FP64 subjective-surface time steps, {nx}x{ny}x{nz}x{nt} "
+                         f"volume, red-black SOR omega 1.8 to relative residual 1e-6"),
+            "volume": [nx, ny, nz, nt],
+            "time_steps": args.steps,
+            "solver": "fused colour-layout red-black SOR, residual fused into the passes",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_sor_sweep<1> (red pass + fused residual)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_launch": nred * BYTES_PER_UPDATE,
+            "avg_launch_ms": red1_avg_s * 1e3,
+            "launches": int(prof["red1_launches"]),
+        },
+        "hbm_gbps_all_sweeps": sweep_bytes / (sweep_ms / 1e3) / 1e9 if sweep_ms > 0 else None,
+        "time_to_tolerance_s": total,
+        "sweeps_per_step": sweeps,
+        "setup_ms_per_step": prof["setup_ms"] / max(prof["setup_launches"], 1),
+        "edge_indicator_s_pcie_inclusive": edge_s,
+        "mask_fraction": mask_frac,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

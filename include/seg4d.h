@@ -120,10 +120,12 @@ const char *seg4dLastError(void);
 int seg4dLastSweeps(void);
 double seg4dLastResidual(void);
 double seg4dLastTolerance(void);
-/* HIP-event profiling of the SOR passes.  seg4dProfileRead fills
- * out[0..8] = {sweep_ms, sweep_launches, sweep_bytes, setup_ms,
- * setup_launches, step_ms, steps, voxel_updates, iterations}
- * and returns the number of values written. */
+/* HIP-event profiling (events on the library's stream).  seg4dProfileRead
+ * fills out[0..13] = {ms, launches} for the three SOR pass kernels (red
+ * without residual, red with fused residual, black with fused residual),
+ * then setup_ms, setup_launches, step_ms, steps, voxel_updates (interior
+ * voxels x SOR iterations), iterations, red points, black points; returns
+ * the number of values written. */
 void seg4dProfile(int on);
 int seg4dProfileRead(double *out, int nout);
 void seg4dProfileReset(void);
