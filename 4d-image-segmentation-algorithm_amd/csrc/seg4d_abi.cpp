@@ -432,6 +432,13 @@ API int seg4dProfileRead(double* out, int nout) {
     return k;
 }
 
+API int seg4dSetSweepVariant(int v) {
+    if (!g_eng) return -1;
+    const int old = g_eng->sweep_variant_;
+    g_eng->sweep_variant_ = (v == 1) ? 1 : 0;
+    return old;
+}
+
 API bool seg4dSynchronize(void) {
     if (!g_eng) return true;
     return g_eng->sync() == 0;

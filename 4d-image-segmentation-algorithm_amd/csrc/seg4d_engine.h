@@ -114,6 +114,9 @@ public:
     int sync();
 
     hipStream_t stream() const { return st_; }
+    // SOR pass kernel variant (tuning knob): 0 = cached loads, 1 = streamed
+    // operands loaded/stored nontemporally.
+    int sweep_variant_ = 0;
     Profile prof;
     std::string err;
 

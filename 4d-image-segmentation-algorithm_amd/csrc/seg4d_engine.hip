@@ -261,25 +261,27 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
         kinds.push_back(kind);
         return r;
     };
+    const bool nt = sweep_variant_ == 1;
     auto red = [&](int z, bool resid) -> int {
         hipEvent_t* e = prof.on ? ev_pair(resid ? 1 : 0) : nullptr;
         if (e) (void)hipEventRecord(e[0], st_);
         const double* xin = ur_[(z - 1) & 1];
         double* xout = ur_[z & 1];
-        if (resid)
-            k_sor_sweep<1><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr,
-                                                     pc_[0], part_r, nullptr);
-        else
-            k_sor_sweep<0><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr,
-                                                     pc_[0], nullptr, nullptr);
+        if (resid) {
+            if (nt) k_sor_sweep<1, true><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
+            else k_sor_sweep<1, false><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
+        } else {
+            if (nt) k_sor_sweep<0, true><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
+            else k_sor_sweep<0, false><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
+        }
         if (e) (void)hipEventRecord(e[1], st_);
         return launch_check("k_sor_sweep red");
     };
     auto black = [&](int z) -> int {
         hipEvent_t* e = prof.on ? ev_pair(2) : nullptr;
         if (e) (void)hipEventRecord(e[0], st_);
-        k_sor_sweep<2><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb,
-                                                 pc_[1], part_b, ctl_);
+        if (nt) k_sor_sweep<2, true><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
+        else k_sor_sweep<2, false><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
         if (e) (void)hipEventRecord(e[1], st_);
         return launch_check("k_sor_sweep black");
     };

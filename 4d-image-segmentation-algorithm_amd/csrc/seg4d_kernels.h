@@ -210,7 +210,21 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
 // 2: residual AFTER the update (black pass of iteration z: its red
 // neighbours are final for z).  segmentation4D.c:287-308.
 // ----------------------------------------------------------------------------
-template <int RESID>
+// Streamed operands (coefficients, prev, own colour) are touched once per
+// pass; with NT they bypass L2 retention (nontemporal) so that the other
+// colour's planes, which 6 neighbours re-read, stay cached.
+template <bool NT>
+__device__ __forceinline__ double ld_s(const double* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st_s(double* p, double v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <int RESID, bool NT>
 __global__ __launch_bounds__(kBlock) void k_sor_sweep(
     Geo G, double c, double om, double h4, int colour,
     const double* xin, double* xout, const double* __restrict__ y,
@@ -252,8 +266,10 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(
             const double uE = sft ? y[pp + 1] : y_l;
             const double uS = y[pp - G.py], uN = y[pp + G.py];
             const double uB = y[pp - G.pz], uT = y[pp + G.pz];
-            const double ce = e_[pp], cw = w_[pp], cn = n_[pp], cs = s_[pp];
-            const double ct = t_[pp], cb = b_[pp], cfw = fw_[pp], cbw = bw_[pp];
+            const double ce = ld_s<NT>(e_ + pp), cw = ld_s<NT>(w_ + pp);
+            const double cn = ld_s<NT>(n_ + pp), cs = ld_s<NT>(s_ + pp);
+            const double ct = ld_s<NT>(t_ + pp), cb = ld_s<NT>(b_ + pp);
+            const double cfw = ld_s<NT>(fw_ + pp), cbw = ld_s<NT>(bw_ + pp);
             // numerator order e, w, s, n, b, t, bw, fw (segmentation4D.c:242-249)
             double num = ce * uE;
             num = num + cw * uW;
@@ -272,15 +288,15 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(
             dg = dg + cfw;
             dg = dg + cbw;
             const double D = 1 + c * dg;
-            const double pv = prev[pp];
-            const double uo = xin[pp];
+            const double pv = ld_s<NT>(prev + pp);
+            const double uo = ld_s<NT>(xin + pp);
             if constexpr (RESID == 1) {
                 const double r = uo * D - c * num - pv;
                 acc += r * r * h4;
             }
             const double gs = (pv + c * num) / D;
             const double un = uo + om * (gs - uo);
-            xout[pp] = un;
+            st_s<NT>(xout + pp, un);
             if constexpr (RESID == 2) {
                 const double r = un * D - c * num - pv;
                 acc += r * r * h4;

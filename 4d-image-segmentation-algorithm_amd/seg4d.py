@@ -141,6 +141,8 @@ class SegLib(_Lib):
         L.seg4dProfile.argtypes = [c_int]
         L.seg4dProfileRead.restype = c_int
         L.seg4dProfileRead.argtypes = [POINTER(c_double), c_int]
+        L.seg4dSetSweepVariant.restype = c_int
+        L.seg4dSetSweepVariant.argtypes = [c_int]
         L.seg4dDeviceInfo.restype = c_int
         L.seg4dDeviceInfo.argtypes = [c_char_p, c_int]
         L.seg4d_gen_preset.restype = c_int
@@ -241,6 +243,9 @@ class SegLib(_Lib):
                 "black_ms", "black_launches", "setup_ms", "setup_launches",
                 "step_ms", "steps", "voxel_updates", "iterations", "nred", "nblack")
         return dict(zip(keys, list(out)))
+
+    def set_sweep_variant(self, v):
+        return self.lib.seg4dSetSweepVariant(int(v))
 
     def synchronize(self):
         self.check(self.lib.seg4dSynchronize(), "seg4dSynchronize")

@@ -110,6 +110,10 @@ def main():
     ap.add_argument("--preset", type=int, default=1, help="synthetic volume preset (0 or 1)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--variant", type=int, default=None, help="SOR pass kernel variant")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the timed region, one device copy of u (padded volume, "
+                         "FP64) as the known-size reference for rocprofv3 --pmc byte counts")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -126,6 +130,8 @@ def main():
     nint = nx * ny * nz * nt
     lib.allocate(nx, ny, nz, nt)
     params(lib)
+    if args.variant is not None:
+        lib.set_sweep_variant(args.variant)
     log(f"[bench] volume {nx}x{ny}x{nz}x{nt} on {lib.device_info()}")
     t = time.perf_counter()
     u0 = np.zeros(shape)
@@ -167,6 +173,10 @@ def main():
     lib.profile(False)
     total = t1 - t0
     log(f"[bench] {args.steps} steps in {total:.3f} s, sweeps/step {sweeps}")
+    if args.calib_copy:
+        lib.copy("u", "prev_u")  # resident mode: one device-to-device copy
+        lib.synchronize()
+        log(f"[bench] calibration copy of {8 * np.prod(shape)} bytes done")
 
     lib.download("u")
     u = lib.arr("u")

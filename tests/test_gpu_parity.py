@@ -162,9 +162,10 @@ def test_seg_random_vs_oracle(seglib, oracle, dims):
         lib.deallocate()
 
 
-@pytest.mark.parametrize("sweeps", [1, 2, 7])
-def test_fixed_sweep_count(seglib, oracle, sweeps):
-    """gsTol < 0 forces exactly maxNoGSIteration sweeps (sweep cap path)."""
+@pytest.mark.parametrize("sweeps,variant", [(1, 0), (2, 0), (7, 0), (7, 1)])
+def test_fixed_sweep_count(seglib, oracle, sweeps, variant):
+    """gsTol < 0 forces exactly maxNoGSIteration sweeps (sweep cap path);
+    every SOR pass kernel variant gives the same bits."""
     nx, ny, nz, nt = 12, 10, 9, 4
     shape = (nt + 2, nz + 2, ny + 2, nx + 2)
     rng = np.random.default_rng(sweeps)
@@ -174,6 +175,7 @@ def test_fixed_sweep_count(seglib, oracle, sweeps):
     p = params(shape, gsTol=-1.0, **kw)
     lib = seglib
     lib.allocate(nx, ny, nz, nt)
+    lib.set_sweep_variant(variant)
     try:
         lib.set(gsTol=-1.0, **kw)
         for f, gg in zip(FACES, gs):
