@@ -165,10 +165,24 @@ int Engine::download(ArrayId a, double* host) {
     return check(hipStreamSynchronize(st_), "download sync");
 }
 
+namespace {
+// Element copy over the padded array (segmentation4D.c:866-874): one FP64
+// load and store per lane, the same access width as the SOR passes, so it
+// doubles as the known-size reference for hardware byte counters.
+__global__ __launch_bounds__(kBlock) void k_copy_f64(const double* __restrict__ src,
+                                                    double* __restrict__ dst, long n) {
+    for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < n;
+         q += (long)gridDim.x * kBlock)
+        dst[q] = src[q];
+}
+}  // namespace
+
 int Engine::copy(ArrayId src, ArrayId dst) {
     if (!full_[src] || !full_[dst]) return check(hipErrorInvalidValue, "copy: array not allocated");
-    return check(hipMemcpyAsync(full_[dst], full_[src], (size_t)n_ * sizeof(double),
-                                hipMemcpyDeviceToDevice, st_), "copy");
+    if (src == dst) return 0;
+    const long nb = (n_ + kBlock - 1) / kBlock;
+    k_copy_f64<<<(unsigned)(nb < 65536 ? nb : 65536), kBlock, 0, st_>>>(full_[src], full_[dst], n_);
+    return launch_check("k_copy_f64");
 }
 
 int Engine::download_coef(int f, double* host) {
