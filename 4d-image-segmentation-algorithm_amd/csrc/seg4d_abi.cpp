@@ -435,7 +435,7 @@ API int seg4dProfileRead(double* out, int nout) {
 API int seg4dSetSweepVariant(int v) {
     if (!g_eng) return -1;
     const int old = g_eng->sweep_variant_;
-    g_eng->sweep_variant_ = (v == 1) ? 1 : 0;
+    g_eng->sweep_variant_ = (v >= 0 && v <= 3) ? v : 0;
     return old;
 }
 

@@ -248,8 +248,16 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     const double c = P.tau / (P.h * P.h);
     const double h4 = P.h * P.h * P.h * P.h;
-    const dim3 grid(cdiv((long)H_ * (J_ - 2), kBlock), K_ - 2);
-    const int nb = (int)(grid.x * grid.y);
+    SweepMap M;
+    M.gx = (int)cdiv((long)H_ * (J_ - 2), kBlock);
+    M.gk = K_ - 2;
+    M.tiled = (sweep_variant_ & 2) ? 1 : 0;
+    M.tb = 8;
+    M.tk = 32;
+    M.ntb = (M.gx + M.tb - 1) / M.tb;
+    M.ntiles = M.ntb * ((M.gk + M.tk - 1) / M.tk);
+    const int nb = M.gx * M.gk;  // partial sums: one per (run, plane)
+    const dim3 grid(M.tiled ? (unsigned)(((M.ntiles + 7) / 8) * 8 * M.tb * M.tk) : (unsigned)nb);
     double* part_r = part_;
     double* part_b = part_ + npart_;
     CPtr8 cr{}, cb{};
@@ -275,18 +283,18 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
         kinds.push_back(kind);
         return r;
     };
-    const bool nt = sweep_variant_ == 1;
+    const bool nt = (sweep_variant_ & 1) != 0;
     auto red = [&](int z, bool resid) -> int {
         hipEvent_t* e = prof.on ? ev_pair(resid ? 1 : 0) : nullptr;
         if (e) (void)hipEventRecord(e[0], st_);
         const double* xin = ur_[(z - 1) & 1];
         double* xout = ur_[z & 1];
         if (resid) {
-            if (nt) k_sor_sweep<1, true><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
-            else k_sor_sweep<1, false><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
+            if (nt) k_sor_sweep<1, true><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
+            else k_sor_sweep<1, false><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
         } else {
-            if (nt) k_sor_sweep<0, true><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
-            else k_sor_sweep<0, false><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
+            if (nt) k_sor_sweep<0, true><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
+            else k_sor_sweep<0, false><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
         }
         if (e) (void)hipEventRecord(e[1], st_);
         return launch_check("k_sor_sweep red");
@@ -294,8 +302,8 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
     auto black = [&](int z) -> int {
         hipEvent_t* e = prof.on ? ev_pair(2) : nullptr;
         if (e) (void)hipEventRecord(e[0], st_);
-        if (nt) k_sor_sweep<2, true><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
-        else k_sor_sweep<2, false><<<grid, kBlock, 0, st_>>>(G, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
+        if (nt) k_sor_sweep<2, true><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
+        else k_sor_sweep<2, false><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
         if (e) (void)hipEventRecord(e[1], st_);
         return launch_check("k_sor_sweep black");
     };

@@ -224,17 +224,46 @@ __device__ __forceinline__ void st_s(double* p, double v) {
     else *p = v;
 }
 
+// Block -> (run bx, plane k) map of the SOR passes (1-D grid).
+//  linear: XCD-contiguous ranges of the run-major order (xcd_remap);
+//  tiled : every XCD processes whole tiles of tb runs x tk planes, one tile
+//          per residency round, so that the blocks sharing the other colour's
+//          k+-1 planes are co-resident on one XCD (one L2) and march l in step.
+struct SweepMap {
+    int gx, gk;      // runs of kBlock points per row plane, interior k planes
+    int tiled;
+    int tb, tk;      // tile shape; tb * tk blocks per tile
+    int ntb, ntiles;
+};
+
+__device__ __forceinline__ bool map_block(const SweepMap& M, int b, int nb, int& bx, int& k) {
+    if (!M.tiled) {
+        const int lb = xcd_remap(b, nb);
+        bx = lb % M.gx;
+        k = lb / M.gx + 1;
+        return true;
+    }
+    const int x = b % 8, s = b / 8;
+    const int per = M.tb * M.tk;
+    const int r = s / per, w = s - r * per;
+    const int T = r * 8 + x;
+    if (T >= M.ntiles) return false;
+    const int tbi = T % M.ntb, tki = T / M.ntb;
+    bx = tbi * M.tb + w % M.tb;
+    k = tki * M.tk + w / M.tb + 1;
+    return bx < M.gx && k <= M.gk;
+}
+
 template <int RESID, bool NT>
 __global__ __launch_bounds__(kBlock) void k_sor_sweep(
-    Geo G, double c, double om, double h4, int colour,
+    Geo G, SweepMap M, double c, double om, double h4, int colour,
     const double* xin, double* xout, const double* __restrict__ y,
     CPtr8 cf, const double* __restrict__ prev, double* __restrict__ partials,
     const SorCtl* __restrict__ ctl) {
     if (ctl != nullptr && ctl->stop) return;
-    const int nb = gridDim.x * gridDim.y;
-    const int lb = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, nb);
-    const int bx = lb % gridDim.x, by = lb / gridDim.x;
-    const int k = by + 1;
+    int bx, k;
+    if (!map_block(M, blockIdx.x, gridDim.x, bx, k)) return;
+    const int lb = (k - 1) * M.gx + bx;
     const int t = bx * kBlock + threadIdx.x;
     const int jr = t / G.H;
     const int m = t - jr * G.H;

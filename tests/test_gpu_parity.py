@@ -162,7 +162,7 @@ def test_seg_random_vs_oracle(seglib, oracle, dims):
         lib.deallocate()
 
 
-@pytest.mark.parametrize("sweeps,variant", [(1, 0), (2, 0), (7, 0), (7, 1)])
+@pytest.mark.parametrize("sweeps,variant", [(1, 0), (2, 0), (7, 0), (7, 1), (7, 2), (7, 3)])
 def test_fixed_sweep_count(seglib, oracle, sweeps, variant):
     """gsTol < 0 forces exactly maxNoGSIteration sweeps (sweep cap path);
     every SOR pass kernel variant gives the same bits."""
