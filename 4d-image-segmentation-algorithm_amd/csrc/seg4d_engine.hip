@@ -283,18 +283,18 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
         kinds.push_back(kind);
         return r;
     };
-    const bool nt = (sweep_variant_ & 1) != 0;
+    const bool u2 = (sweep_variant_ & 1) != 0;
     auto red = [&](int z, bool resid) -> int {
         hipEvent_t* e = prof.on ? ev_pair(resid ? 1 : 0) : nullptr;
         if (e) (void)hipEventRecord(e[0], st_);
         const double* xin = ur_[(z - 1) & 1];
         double* xout = ur_[z & 1];
         if (resid) {
-            if (nt) k_sor_sweep<1, true><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
-            else k_sor_sweep<1, false><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
+            if (u2) k_sor_sweep<1, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
+            else k_sor_sweep<1, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
         } else {
-            if (nt) k_sor_sweep<0, true><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
-            else k_sor_sweep<0, false><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
+            if (u2) k_sor_sweep<0, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
+            else k_sor_sweep<0, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
         }
         if (e) (void)hipEventRecord(e[1], st_);
         return launch_check("k_sor_sweep red");
@@ -302,8 +302,8 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
     auto black = [&](int z) -> int {
         hipEvent_t* e = prof.on ? ev_pair(2) : nullptr;
         if (e) (void)hipEventRecord(e[0], st_);
-        if (nt) k_sor_sweep<2, true><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
-        else k_sor_sweep<2, false><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
+        if (u2) k_sor_sweep<2, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
+        else k_sor_sweep<2, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
         if (e) (void)hipEventRecord(e[1], st_);
         return launch_check("k_sor_sweep black");
     };

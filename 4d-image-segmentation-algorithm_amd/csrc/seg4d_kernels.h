@@ -210,20 +210,6 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
 // 2: residual AFTER the update (black pass of iteration z: its red
 // neighbours are final for z).  segmentation4D.c:287-308.
 // ----------------------------------------------------------------------------
-// Streamed operands (coefficients, prev, own colour) are touched once per
-// pass; with NT they bypass L2 retention (nontemporal) so that the other
-// colour's planes, which 6 neighbours re-read, stay cached.
-template <bool NT>
-__device__ __forceinline__ double ld_s(const double* p) {
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-template <bool NT>
-__device__ __forceinline__ void st_s(double* p, double v) {
-    if constexpr (NT) __builtin_nontemporal_store(v, p);
-    else *p = v;
-}
-
 // Block -> (run bx, plane k) map of the SOR passes (1-D grid).
 //  linear: XCD-contiguous ranges of the run-major order (xcd_remap);
 //  tiled : every XCD processes whole tiles of tb runs x tk planes, one tile
@@ -254,7 +240,93 @@ __device__ __forceinline__ bool map_block(const SweepMap& M, int b, int nb, int&
     return bx < M.gx && k <= M.gk;
 }
 
-template <int RESID, bool NT>
+// Operands of one colour point, gathered before any arithmetic so that the
+// loads of U consecutive l-steps are all in flight together.
+struct PointIn {
+    double c[8];      // e w n s t b fw bw
+    double pv, uo;    // prev, own value
+    double yo;        // in-row neighbour that is not the column partner
+    double yS, yN, yB, yT;
+    int pp;
+    int sft;          // own i = 2m + sft
+    bool act;
+};
+
+__device__ __forceinline__ void load_point(const Geo& G, const CPtr8& cf,
+                                           const double* __restrict__ y,
+                                           const double* __restrict__ prev,
+                                           const double* xin, int pp, int sft, bool act,
+                                           PointIn& P) {
+    P.pp = pp;
+    P.sft = sft;
+    P.act = act;
+    if (act) {
+        P.yo = y[pp + (sft ? 1 : -1)];
+        P.yS = y[pp - G.py];
+        P.yN = y[pp + G.py];
+        P.yB = y[pp - G.pz];
+        P.yT = y[pp + G.pz];
+#pragma unroll
+        for (int f = 0; f < 8; f++) P.c[f] = cf.p[f][pp];
+        P.pv = prev[pp];
+        P.uo = xin[pp];
+    }
+}
+
+// One point update, reference operand order (segmentation4D.c:242-256); the
+// residual term (segmentation4D.c:290-307) before (RESID 1) or after (2).
+template <int RESID>
+__device__ __forceinline__ void update_point(const PointIn& P, double ybw, double ypart,
+                                             double yfw, double c, double om, double h4,
+                                             double* xout, double& acc) {
+    if (!P.act) return;
+    const double uW = P.sft ? ypart : P.yo;
+    const double uE = P.sft ? P.yo : ypart;
+    // numerator order e, w, s, n, b, t, bw, fw
+    double num = P.c[0] * uE;
+    num = num + P.c[1] * uW;
+    num = num + P.c[3] * P.yS;
+    num = num + P.c[2] * P.yN;
+    num = num + P.c[5] * P.yB;
+    num = num + P.c[4] * P.yT;
+    num = num + P.c[7] * ybw;
+    num = num + P.c[6] * yfw;
+    // diagonal order e, w, n, s, t, b, fw, bw
+    double dg = P.c[0] + P.c[1];
+    dg = dg + P.c[2];
+    dg = dg + P.c[3];
+    dg = dg + P.c[4];
+    dg = dg + P.c[5];
+    dg = dg + P.c[6];
+    dg = dg + P.c[7];
+    const double D = 1 + c * dg;
+    if constexpr (RESID == 1) {
+        const double r = P.uo * D - c * num - P.pv;
+        acc += r * r * h4;
+    }
+    const double gs = (P.pv + c * num) / D;
+    const double un = P.uo + om * (gs - P.uo);
+    xout[P.pp] = un;
+    if constexpr (RESID == 2) {
+        const double r = un * D - c * num - P.pv;
+        acc += r * r * h4;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Red-black SOR colour pass on the colour layout, marching along l, U steps
+// per group (all loads of a group are issued before its arithmetic).
+// segmentation4D.c:233-285.  One thread owns packed column m of rows (j,k)
+// and walks l = 1..L-2, keeping the other colour's column-m values of rows
+// l-1 .. l+U in registers (they are the bw / in-row / fw neighbours).
+//
+// RESID = 0: no residual; 1: residual of the point BEFORE its update (the
+// red pass of iteration z+1 yields the red residual terms of iteration z:
+// red values and black neighbours are then exactly those at the end of z);
+// 2: residual AFTER the update (black pass of iteration z: its red
+// neighbours are final for z).  segmentation4D.c:287-308.
+// ----------------------------------------------------------------------------
+template <int RESID, int U>
 __global__ __launch_bounds__(kBlock) void k_sor_sweep(
     Geo G, SweepMap M, double c, double om, double h4, int colour,
     const double* xin, double* xout, const double* __restrict__ y,
@@ -271,68 +343,38 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(
     const bool rowok = j <= G.J - 2;
     double acc = 0.0;
     const int base = (k * G.J + j) * G.H + m;  // l = 0
-    double y_lm1 = 0.0, y_l = 0.0;
-    if (rowok) {
-        y_lm1 = y[base];
-        y_l = y[base + G.pw];
+    // window w[q] = other colour at column m, row l-1+q
+    double w[U + 2];
+    w[0] = rowok ? y[base] : 0.0;
+    w[1] = rowok ? y[base + G.pw] : 0.0;
+    const int lmax = G.L - 2;
+    int l = 1;
+    for (; l + U - 1 <= lmax; l += U) {
+        PointIn P[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const int lq = l + q;
+            const int pp = base + lq * G.pw;
+            w[q + 2] = rowok ? y[pp + G.pw] : 0.0;
+            const int sft = colour ^ ((j + k + lq) & 1);
+            const int i = 2 * m + sft;
+            load_point(G, cf, y, prev, xin, pp, sft, rowok && i >= 1 && i <= G.I - 2, P[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) update_point<RESID>(P[q], w[q], w[q + 1], w[q + 2], c, om, h4, xout, acc);
+        w[0] = w[U];
+        w[1] = w[U + 1];
     }
-    const double* e_ = cf.p[0];
-    const double* w_ = cf.p[1];
-    const double* n_ = cf.p[2];
-    const double* s_ = cf.p[3];
-    const double* t_ = cf.p[4];
-    const double* b_ = cf.p[5];
-    const double* fw_ = cf.p[6];
-    const double* bw_ = cf.p[7];
-    for (int l = 1; l <= G.L - 2; l++) {
+    for (; l <= lmax; l++) {  // remainder, one step at a time
+        PointIn P;
         const int pp = base + l * G.pw;
-        double y_lp1 = 0.0;
-        if (rowok) y_lp1 = y[pp + G.pw];
+        w[2] = rowok ? y[pp + G.pw] : 0.0;
         const int sft = colour ^ ((j + k + l) & 1);
         const int i = 2 * m + sft;
-        if (rowok && i >= 1 && i <= G.I - 2) {
-            const double uW = sft ? y_l : y[pp - 1];
-            const double uE = sft ? y[pp + 1] : y_l;
-            const double uS = y[pp - G.py], uN = y[pp + G.py];
-            const double uB = y[pp - G.pz], uT = y[pp + G.pz];
-            const double ce = ld_s<NT>(e_ + pp), cw = ld_s<NT>(w_ + pp);
-            const double cn = ld_s<NT>(n_ + pp), cs = ld_s<NT>(s_ + pp);
-            const double ct = ld_s<NT>(t_ + pp), cb = ld_s<NT>(b_ + pp);
-            const double cfw = ld_s<NT>(fw_ + pp), cbw = ld_s<NT>(bw_ + pp);
-            // numerator order e, w, s, n, b, t, bw, fw (segmentation4D.c:242-249)
-            double num = ce * uE;
-            num = num + cw * uW;
-            num = num + cs * uS;
-            num = num + cn * uN;
-            num = num + cb * uB;
-            num = num + ct * uT;
-            num = num + cbw * y_lm1;
-            num = num + cfw * y_lp1;
-            // diagonal order e, w, n, s, t, b, fw, bw (segmentation4D.c:250-252)
-            double dg = ce + cw;
-            dg = dg + cn;
-            dg = dg + cs;
-            dg = dg + ct;
-            dg = dg + cb;
-            dg = dg + cfw;
-            dg = dg + cbw;
-            const double D = 1 + c * dg;
-            const double pv = ld_s<NT>(prev + pp);
-            const double uo = ld_s<NT>(xin + pp);
-            if constexpr (RESID == 1) {
-                const double r = uo * D - c * num - pv;
-                acc += r * r * h4;
-            }
-            const double gs = (pv + c * num) / D;
-            const double un = uo + om * (gs - uo);
-            st_s<NT>(xout + pp, un);
-            if constexpr (RESID == 2) {
-                const double r = un * D - c * num - pv;
-                acc += r * r * h4;
-            }
-        }
-        y_lm1 = y_l;
-        y_l = y_lp1;
+        load_point(G, cf, y, prev, xin, pp, sft, rowok && i >= 1 && i <= G.I - 2, P);
+        update_point<RESID>(P, w[0], w[1], w[2], c, om, h4, xout, acc);
+        w[0] = w[1];
+        w[1] = w[2];
     }
     if constexpr (RESID != 0) {
         const double bs = block_sum256(acc);
