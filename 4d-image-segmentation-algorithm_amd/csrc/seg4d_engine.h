@@ -131,8 +131,8 @@ private:
     long n_ = 0, np_ = 0;        // full / colour-plane element counts
     hipStream_t st_ = nullptr;
     double* full_[A_COUNT] = {};
-    double* cf_[2][8] = {};      // colour-layout coefficients
-    double* pc_[2] = {};         // colour-layout prev
+    double* op_[2] = {};         // operand blocks (8 coefficients + prev) per colour
+    long nop_ = 0;               // elements of one operand-block array
     double* ur_[2] = {};         // red ping-pong
     double* ub_ = nullptr;       // black
     double* part_ = nullptr;     // partial sums (two halves)

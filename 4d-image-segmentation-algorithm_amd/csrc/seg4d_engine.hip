@@ -81,10 +81,12 @@ int Engine::init(int I, int J, int K, int L, bool seg, bool mcf) {
             if (alloc(&full_[A_G0 + f], n)) return -1;
     }
     if (mcf && alloc(&full_[A_EXACT], n)) return -1;
+    {
+        const long rp = (((long)(J - 2) * H_ + 63) / 64) * 64;
+        nop_ = (long)(K - 2) * (L - 2) * rp * kOpnd;
+    }
     for (int c = 0; c < 2; c++) {
-        for (int f = 0; f < 8; f++)
-            if (alloc(&cf_[c][f], np_)) return -1;
-        if (alloc(&pc_[c], np_) || alloc(&ur_[c], np_)) return -1;
+        if (alloc(&op_[c], nop_) || alloc(&ur_[c], np_)) return -1;
     }
     if (alloc(&ub_, np_)) return -1;
     // partial-sum capacity: the largest grid of any reducing kernel
@@ -115,13 +117,9 @@ void Engine::release() {
         p = nullptr;
     }
     for (int c = 0; c < 2; c++) {
-        for (int f = 0; f < 8; f++) {
-            if (cf_[c][f]) hipFree(cf_[c][f]);
-            cf_[c][f] = nullptr;
-        }
-        if (pc_[c]) hipFree(pc_[c]);
+        if (op_[c]) hipFree(op_[c]);
         if (ur_[c]) hipFree(ur_[c]);
-        pc_[c] = ur_[c] = nullptr;
+        op_[c] = ur_[c] = nullptr;
     }
     if (ub_) hipFree(ub_);
     if (part_) hipFree(part_);
@@ -146,6 +144,7 @@ static Geo make_geo(int I, int J, int K, int L, int H) {
     G.sy = I; G.sz = I * J; G.sw = I * J * K;
     G.H = H;
     G.py = H; G.pz = H * J; G.pw = H * J * K;
+    G.RP = (((J - 2) * H + 63) / 64) * 64;
     return G;
 }
 
@@ -189,7 +188,7 @@ int Engine::download_coef(int f, double* host) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     if (!scratch(A_TMP)) return -1;
     dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
-    k_unpack_coef<<<grid, kBlock, 0, st_>>>(G, cf_[0][f], cf_[1][f], full_[A_TMP]);
+    k_unpack_coef<<<grid, kBlock, 0, st_>>>(G, op_[0], op_[1], f, full_[A_TMP]);
     if (launch_check("k_unpack_coef")) return -1;
     return download(A_TMP, host);
 }
@@ -204,8 +203,7 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
         for (int f = 0; f < 8; f++) g.p[f] = full_[A_G0 + f];
     SplitBufs sb;
     for (int c = 0; c < 2; c++) {
-        for (int f = 0; f < 8; f++) sb.coef[c][f] = cf_[c][f];
-        sb.prev[c] = pc_[c];
+        sb.opnd[c] = op_[c];
         sb.ured[c] = ur_[c];
     }
     sb.ublack = ub_;
@@ -260,11 +258,8 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
     const dim3 grid(M.tiled ? (unsigned)(((M.ntiles + 7) / 8) * 8 * M.tb * M.tk) : (unsigned)nb);
     double* part_r = part_;
     double* part_b = part_ + npart_;
-    CPtr8 cr{}, cb{};
-    for (int f = 0; f < 8; f++) {
-        cr.p[f] = cf_[0][f];
-        cb.p[f] = cf_[1][f];
-    }
+    const double* cr = op_[0];
+    const double* cb = op_[1];
     const int cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
     // interior red / black point counts for the byte model: the corner
     // (1,1,1,1) is red, so red has the extra point when the count is odd
@@ -290,11 +285,11 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
         const double* xin = ur_[(z - 1) & 1];
         double* xout = ur_[z & 1];
         if (resid) {
-            if (u2) k_sor_sweep<1, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
-            else k_sor_sweep<1, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], part_r, nullptr);
+            if (u2) k_sor_sweep<1, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, part_r, nullptr);
+            else k_sor_sweep<1, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, part_r, nullptr);
         } else {
-            if (u2) k_sor_sweep<0, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
-            else k_sor_sweep<0, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, pc_[0], nullptr, nullptr);
+            if (u2) k_sor_sweep<0, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, nullptr, nullptr);
+            else k_sor_sweep<0, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, nullptr, nullptr);
         }
         if (e) (void)hipEventRecord(e[1], st_);
         return launch_check("k_sor_sweep red");
@@ -302,8 +297,8 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
     auto black = [&](int z) -> int {
         hipEvent_t* e = prof.on ? ev_pair(2) : nullptr;
         if (e) (void)hipEventRecord(e[0], st_);
-        if (u2) k_sor_sweep<2, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
-        else k_sor_sweep<2, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, pc_[1], part_b, ctl_);
+        if (u2) k_sor_sweep<2, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, part_b, ctl_);
+        else k_sor_sweep<2, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, part_b, ctl_);
         if (e) (void)hipEventRecord(e[1], st_);
         return launch_check("k_sor_sweep black");
     };
