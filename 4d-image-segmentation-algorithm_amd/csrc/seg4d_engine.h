@@ -114,9 +114,10 @@ public:
     int sync();
 
     hipStream_t stream() const { return st_; }
-    // SOR pass kernel variant (tuning knob): 0 = cached loads, 1 = streamed
-    // operands loaded/stored nontemporally.
-    int sweep_variant_ = 0;
+    // SOR pass kernel variant (tuning knob, identical results), bit flags:
+    // 1 = two l-steps per load phase, 2 = tiled XCD-resident launch order.
+    // 3 measured fastest on MI355X at 256x256x128x16.
+    int sweep_variant_ = 3;
     Profile prof;
     std::string err;
 

@@ -218,7 +218,8 @@ def main():
                          f"volume, red-black SOR omega 1.8 to relative residual 1e-6"),
             "volume": [nx, ny, nz, nt],
             "time_steps": args.steps,
-            "solver": "fused colour-layout red-black SOR, residual fused into the passes",
+            "solver": "colour-layout red-black SOR, operand blocks, residual fused into the passes",
+            "sweep_variant": args.variant if args.variant is not None else 3,
         },
         "roofline": {
             "bound": "hbm",

@@ -130,9 +130,9 @@ void seg4dProfile(int on);
 int seg4dProfileRead(double *out, int nout);
 void seg4dProfileReset(void);
 /* SOR pass kernel variant (tuning knob, identical results), bit flags:
- * 1 = coefficients/prev/own colour streamed nontemporally; 2 = tiled launch
- * order (the blocks that share neighbour planes are co-resident on one XCD).
- * Returns the previous value, -1 before _allocateMem_. */
+ * 1 = two l-steps per load phase; 2 = tiled launch order (the blocks that
+ * share neighbour planes are co-resident on one XCD).  Default 3.  Returns
+ * the previous value, -1 before _allocateMem_. */
 int seg4dSetSweepVariant(int v);
 /* Blocks until all queued device work is done. */
 bool seg4dSynchronize(void);
