@@ -421,16 +421,35 @@ API void seg4dProfileReset(void) {
 API int seg4dProfileRead(double* out, int nout) {
     if (!g_eng || !out) return 0;
     const s4d::Profile& p = g_eng->prof;
-    const double v[14] = {p.sweep_ms[0], (double)p.sweep_launches[0],
+    const double v[23] = {p.sweep_ms[0], (double)p.sweep_launches[0],
                           p.sweep_ms[1], (double)p.sweep_launches[1],
                           p.sweep_ms[2], (double)p.sweep_launches[2],
                           p.setup_ms, (double)p.setup_launches, p.step_ms, (double)p.steps,
                           (double)p.voxel_updates, (double)p.iterations,
-                          (double)p.nred, (double)p.nblack};
-    int k = nout < 14 ? nout : 14;
+                          (double)p.nred, (double)p.nblack,
+                          p.sweep_ms[3], (double)p.sweep_launches[3],
+                          p.sweep_ms[4], (double)p.sweep_launches[4],
+                          p.sweep_ms[5], (double)p.sweep_launches[5],
+                          p.defect_ms, (double)p.defect_launches, (double)p.rounds};
+    int k = nout < 23 ? nout : 23;
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;
 }
+
+API int seg4dSetPrecision(int mode) {
+    if (!g_eng) return -1;
+    const int old = g_eng->precision_;
+    g_eng->precision_ = (mode == 1) ? 1 : 0;
+    return old;
+}
+
+API void seg4dSetRefinement(int max_rounds, double inner_floor) {
+    if (!g_eng) return;
+    if (max_rounds > 0) g_eng->max_rounds_ = max_rounds;
+    if (inner_floor > 0) g_eng->inner_floor2_ = inner_floor * inner_floor;
+}
+
+API int seg4dLastRounds(void) { return g_last.rounds; }
 
 API int seg4dSetSweepVariant(int v) {
     if (!g_eng) return -1;

@@ -15,6 +15,7 @@ namespace s4d {
 
 struct Geo;
 struct SorCtl;
+struct SweepMap;
 
 struct StepParams {
     double h = 1.0, tau = 0.5, eps2 = 1e-6;
@@ -25,10 +26,11 @@ struct StepParams {
 };
 
 struct SolveStats {
-    int z = 0;              // SOR iterations performed (reference count)
-    double R = 0.0;         // residual at exit
+    int z = 0;              // SOR iterations performed (FP32-refined: FP32 sweeps)
+    double R = 0.0;         // residual at exit (FP64 system)
     double tol = 0.0;       // bound used
     double sumsq = 0.0;     // sum(prev^2 h^4) over the interior
+    int rounds = 0;         // FP32-refined: defect-correction rounds
 };
 
 struct Balls {              // rescale-epilogue balls, host copies
@@ -42,9 +44,13 @@ struct Profile {            // HIP-event timing (events on the engine's stream)
     bool on = false;
     // SOR colour passes by kernel variant: 0 = red without residual (first
     // sweep of a solve), 1 = red with the fused residual, 2 = black with the
-    // fused residual (the last black launch of a solve is a no-op).
-    double sweep_ms[3] = {0, 0, 0};
-    long sweep_launches[3] = {0, 0, 0};
+    // fused residual (the last black launch of a solve is a no-op); 3..5 the
+    // same for the FP32 correction solves.
+    double sweep_ms[6] = {0, 0, 0, 0, 0, 0};
+    long sweep_launches[6] = {0, 0, 0, 0, 0, 0};
+    double defect_ms = 0.0; // FP64 defect passes of the FP32-refined form
+    long defect_launches = 0;
+    long rounds = 0;
     double setup_ms = 0.0;  // coefficient / set-up kernel
     long setup_launches = 0;
     double step_ms = 0.0;   // whole time steps (setup + solve + epilogue)
@@ -118,14 +124,24 @@ public:
     // 1 = two l-steps per load phase, 2 = tiled XCD-resident launch order.
     // 3 measured fastest on MI355X at 256x256x128x16.
     int sweep_variant_ = 3;
+    // 0 = FP64 (the reference's arithmetic); 1 = FP32-refined: FP32 SOR on
+    // the correction equation inside FP64 defect correction.
+    int precision_ = 0;
+    int max_rounds_ = 8;           // FP32-refined: correction rounds per solve
+    double inner_frac_ = 0.25;     // inner stop: R32 <= max(frac*tol, floor2*R)
+    double inner_floor2_ = 1e-10;  // (1e-5)^2: what an FP32 solve can resolve
     Profile prof;
     std::string err;
 
-private:
+    // --- internals (used by the solver templates in seg4d_engine.hip) --------
     int check(hipError_t e, const char* what);
     int launch_check(const char* what);
     int sumsq_interior(const double* d, double h, double* out);
     int reduce_to_host(int np, double* out);
+    struct SweepMap sweep_map() const;
+    int ensure_mixed();
+    int defect64(const StepParams& P);
+    int solve_mixed(const StepParams& P, SolveStats* st);
 
     bool ready_ = false;
     int I_ = 0, J_ = 0, K_ = 0, L_ = 0, H_ = 0;
@@ -145,6 +161,12 @@ private:
     hipEvent_t ev_ = nullptr;
     std::vector<hipEvent_t> pev_; // profiling events (pairs)
     int red_final_ = 0;          // which red buffer holds the accepted state
+    // FP32-refined form (allocated on first use)
+    float* op32_[2] = {};        // FP32 operand blocks: 8 coefficients + defect
+    float* dr_[2] = {};          // correction, red ping-pong
+    float* db_ = nullptr;        // correction, black
+    SorCtl* ctl2_ = nullptr;     // control block of the correction solves
+    SorCtl* hctl2_ = nullptr;
 };
 
 // Last error of the calling library (set by the ABI layers).

@@ -128,6 +128,16 @@ void Engine::release() {
     if (hctl_) hipHostFree(hctl_);
     if (hscal_) hipHostFree(hscal_);
     ub_ = part_ = dscal_ = hscal_ = nullptr;
+    for (int c = 0; c < 2; c++) {
+        if (op32_[c]) hipFree(op32_[c]);
+        if (dr_[c]) hipFree(dr_[c]);
+        op32_[c] = dr_[c] = nullptr;
+    }
+    if (db_) hipFree(db_);
+    if (ctl2_) hipFree(ctl2_);
+    if (hctl2_) hipHostFree(hctl2_);
+    db_ = nullptr;
+    ctl2_ = hctl2_ = nullptr;
     ctl_ = hctl_ = nullptr;
     for (auto e : pev_) hipEventDestroy(e);
     pev_.clear();
@@ -201,9 +211,11 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
     CPtr8 g{};
     if (with_g)
         for (int f = 0; f < 8; f++) g.p[f] = full_[A_G0 + f];
+    if (precision_ == 1 && ensure_mixed()) return -1;
     SplitBufs sb;
     for (int c = 0; c < 2; c++) {
         sb.opnd[c] = op_[c];
+        sb.opnd32[c] = precision_ == 1 ? op32_[c] : nullptr;
         sb.ured[c] = ur_[c];
     }
     sb.ublack = ub_;
@@ -242,10 +254,7 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
     return launch_check("k_init_ctl");
 }
 
-int Engine::solve(const StepParams& P, SolveStats* out) {
-    const Geo G = make_geo(I_, J_, K_, L_, H_);
-    const double c = P.tau / (P.h * P.h);
-    const double h4 = P.h * P.h * P.h * P.h;
+SweepMap Engine::sweep_map() const {
     SweepMap M;
     M.gx = (int)cdiv((long)H_ * (J_ - 2), kBlock);
     M.gk = K_ - 2;
@@ -254,91 +263,249 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
     M.tk = 32;
     M.ntb = (M.gx + M.tb - 1) / M.tb;
     M.ntiles = M.ntb * ((M.gk + M.tk - 1) / M.tk);
-    const int nb = M.gx * M.gk;  // partial sums: one per (run, plane)
+    return M;
+}
+
+// One red-black SOR solve of precision T on the colour layout, pipelined:
+// red(z+1) yields the red residual terms of iteration z, reduce(z) applies
+// the stopping test on the device, black(z+1) is enqueued before the host has
+// read the decision and becomes a no-op if iteration z stopped.
+template <typename T>
+struct SorJob {
+    T c, om;
+    double h4;
+    const T* opnd[2];   // operand blocks per colour
+    T* red[2];          // red ping-pong
+    T* black;
+    SorCtl* ctl;        // device control block (tol already set)
+    SorCtl* hctl;       // pinned host mirror
+    int cap;
+    int kind0;          // profiling kinds kind0 .. kind0+2
+};
+
+template <typename T>
+int sor_run(Engine& E, const SorJob<T>& J, int* zout) {
+    const Geo G = make_geo(E.I_, E.J_, E.K_, E.L_, E.H_);
+    const SweepMap M = E.sweep_map();
+    const int nb = M.gx * M.gk;
     const dim3 grid(M.tiled ? (unsigned)(((M.ntiles + 7) / 8) * 8 * M.tb * M.tk) : (unsigned)nb);
-    double* part_r = part_;
-    double* part_b = part_ + npart_;
-    const double* cr = op_[0];
-    const double* cb = op_[1];
-    const int cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
-    // interior red / black point counts for the byte model: the corner
-    // (1,1,1,1) is red, so red has the extra point when the count is odd
-    const long nint = (long)(I_ - 2) * (J_ - 2) * (K_ - 2) * (L_ - 2);
-    const long nred = (nint + 1) / 2, nblk = nint - nred;
-    int pe = 0;  // profiling event index
+    double* part_r = E.part_;
+    double* part_b = E.part_ + E.npart_;
+    hipStream_t st = E.st_;
+    int pe = 0;
     std::vector<int> kinds;
     auto ev_pair = [&](int kind) -> hipEvent_t* {
-        while ((int)pev_.size() < pe + 2) {
+        while ((int)E.pev_.size() < pe + 2) {
             hipEvent_t e;
             if (hipEventCreate(&e) != hipSuccess) return nullptr;
-            pev_.push_back(e);
+            E.pev_.push_back(e);
         }
-        hipEvent_t* r = &pev_[pe];
+        hipEvent_t* r = &E.pev_[pe];
         pe += 2;
         kinds.push_back(kind);
         return r;
     };
-    const bool u2 = (sweep_variant_ & 1) != 0;
+    const bool u2 = (E.sweep_variant_ & 1) != 0;
     auto red = [&](int z, bool resid) -> int {
-        hipEvent_t* e = prof.on ? ev_pair(resid ? 1 : 0) : nullptr;
-        if (e) (void)hipEventRecord(e[0], st_);
-        const double* xin = ur_[(z - 1) & 1];
-        double* xout = ur_[z & 1];
+        hipEvent_t* e = E.prof.on ? ev_pair(J.kind0 + (resid ? 1 : 0)) : nullptr;
+        if (e) (void)hipEventRecord(e[0], st);
+        const T* xin = J.red[(z - 1) & 1];
+        T* xout = J.red[z & 1];
         if (resid) {
-            if (u2) k_sor_sweep<1, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, part_r, nullptr);
-            else k_sor_sweep<1, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, part_r, nullptr);
+            if (u2) k_sor_sweep<T, 1, 2><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 0, xin, xout, J.black, J.opnd[0], part_r, nullptr, nullptr);
+            else k_sor_sweep<T, 1, 1><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 0, xin, xout, J.black, J.opnd[0], part_r, nullptr, nullptr);
         } else {
-            if (u2) k_sor_sweep<0, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, nullptr, nullptr);
-            else k_sor_sweep<0, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, xin, xout, ub_, cr, nullptr, nullptr);
+            if (u2) k_sor_sweep<T, 0, 2><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 0, xin, xout, J.black, J.opnd[0], nullptr, nullptr, nullptr);
+            else k_sor_sweep<T, 0, 1><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 0, xin, xout, J.black, J.opnd[0], nullptr, nullptr, nullptr);
         }
-        if (e) (void)hipEventRecord(e[1], st_);
-        return launch_check("k_sor_sweep red");
+        if (e) (void)hipEventRecord(e[1], st);
+        return E.launch_check("k_sor_sweep red");
     };
     auto black = [&](int z) -> int {
-        hipEvent_t* e = prof.on ? ev_pair(2) : nullptr;
-        if (e) (void)hipEventRecord(e[0], st_);
-        if (u2) k_sor_sweep<2, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, part_b, ctl_);
-        else k_sor_sweep<2, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[z & 1], cb, part_b, ctl_);
-        if (e) (void)hipEventRecord(e[1], st_);
-        return launch_check("k_sor_sweep black");
+        hipEvent_t* e = E.prof.on ? ev_pair(J.kind0 + 2) : nullptr;
+        if (e) (void)hipEventRecord(e[0], st);
+        if (u2) k_sor_sweep<T, 2, 2><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 1, J.black, J.black, J.red[z & 1], J.opnd[1], part_b, J.ctl, nullptr);
+        else k_sor_sweep<T, 2, 1><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 1, J.black, J.black, J.red[z & 1], J.opnd[1], part_b, J.ctl, nullptr);
+        if (e) (void)hipEventRecord(e[1], st);
+        return E.launch_check("k_sor_sweep black");
     };
     if (red(1, false) || black(1)) return -1;
     int z = 1;
     for (;;) {
         if (red(z + 1, true)) return -1;
-        k_reduce_resid<<<1, 1024, 0, st_>>>(part_r, nb, part_b, nb, ctl_, nullptr, z, cap);
-        if (launch_check("k_reduce_resid")) return -1;
-        if (check(hipMemcpyAsync(hctl_, ctl_, sizeof(SorCtl), hipMemcpyDeviceToHost, st_),
-                  "ctl copy"))
+        k_reduce_resid<<<1, 1024, 0, st>>>(part_r, nb, part_b, nb, J.ctl, nullptr, z, J.cap);
+        if (E.launch_check("k_reduce_resid")) return -1;
+        if (E.check(hipMemcpyAsync(J.hctl, J.ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, st),
+                    "ctl copy"))
             return -1;
-        if (check(hipEventRecord(ev_, st_), "event")) return -1;
+        if (E.check(hipEventRecord(E.ev_, st), "event")) return -1;
         if (black(z + 1)) return -1;  // becomes a no-op if iteration z stopped
         hipError_t q;
-        while ((q = hipEventQuery(ev_)) == hipErrorNotReady) {
+        while ((q = hipEventQuery(E.ev_)) == hipErrorNotReady) {
         }
-        if (check(q, "solve wait")) return -1;
-        if (hctl_->stop) break;
+        if (E.check(q, "solve wait")) return -1;
+        if (J.hctl->stop) break;
         z++;
     }
+    if (E.prof.on) {
+        if (E.check(hipStreamSynchronize(st), "prof sync")) return -1;
+        for (int q = 0; q < pe; q += 2) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, E.pev_[q], E.pev_[q + 1]);
+            E.prof.sweep_ms[kinds[q / 2]] += ms;
+            E.prof.sweep_launches[kinds[q / 2]]++;
+        }
+    }
+    *zout = z;
+    return 0;
+}
+
+int Engine::solve(const StepParams& P, SolveStats* out) {
+    const long nint = (long)(I_ - 2) * (J_ - 2) * (K_ - 2) * (L_ - 2);
+    // interior red / black point counts for the byte model: the corner
+    // (1,1,1,1) is red, so red has the extra point when the count is odd
+    prof.nred = (nint + 1) / 2;
+    prof.nblack = nint - prof.nred;
+    if (precision_ == 1) return solve_mixed(P, out);
+    SorJob<double> J;
+    J.c = P.tau / (P.h * P.h);
+    J.om = P.omega;
+    J.h4 = P.h * P.h * P.h * P.h;
+    J.opnd[0] = op_[0];
+    J.opnd[1] = op_[1];
+    J.red[0] = ur_[0];
+    J.red[1] = ur_[1];
+    J.black = ub_;
+    J.ctl = ctl_;
+    J.hctl = hctl_;
+    J.cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
+    J.kind0 = 0;
+    int z = 0;
+    if (sor_run<double>(*this, J, &z)) return -1;
     red_final_ = z & 1;
     if (out) {
         out->z = hctl_->zfinal;
         out->R = hctl_->R;
         out->tol = hctl_->tol;
         out->sumsq = hctl_->sumsq;
+        out->rounds = 0;
     }
     if (prof.on) {
-        if (check(hipStreamSynchronize(st_), "prof sync")) return -1;
-        for (int q = 0; q < pe; q += 2) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, pev_[q], pev_[q + 1]);
-            prof.sweep_ms[kinds[q / 2]] += ms;
-            prof.sweep_launches[kinds[q / 2]]++;
-        }
         prof.iterations += z;
         prof.voxel_updates += (long)z * nint;
-        prof.nred = nred;
-        prof.nblack = nblk;
+    }
+    return 0;
+}
+
+int Engine::ensure_mixed() {
+    if (op32_[0]) return 0;
+    auto allocf = [&](float** p, long cnt) -> int {
+        if (check(hipMalloc((void**)p, (size_t)cnt * sizeof(float)), "hipMalloc")) return -1;
+        return check(hipMemsetAsync(*p, 0, (size_t)cnt * sizeof(float), st_), "hipMemset");
+    };
+    for (int c = 0; c < 2; c++)
+        if (allocf(&op32_[c], nop_) || allocf(&dr_[c], np_)) return -1;
+    if (allocf(&db_, np_)) return -1;
+    if (check(hipMalloc((void**)&ctl2_, sizeof(SorCtl)), "hipMalloc ctl")) return -1;
+    if (check(hipHostMalloc((void**)&hctl2_, sizeof(SorCtl), hipHostMallocDefault), "hipHostMalloc"))
+        return -1;
+    std::memset(hctl2_, 0, sizeof(SorCtl));
+    return check(hipStreamSynchronize(st_), "mixed alloc");
+}
+
+// FP64 residual of the reference system for the current colour-layout state
+// (red in ur_[0], black in ub_); writes the FP32 defect into the FP32 operand
+// blocks and the residual into ctl_->R (host copy in hctl_).
+int Engine::defect64(const StepParams& P) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    const SweepMap M = sweep_map();
+    const int nb = M.gx * M.gk;
+    const dim3 grid(M.tiled ? (unsigned)(((M.ntiles + 7) / 8) * 8 * M.tb * M.tk) : (unsigned)nb);
+    const double c = P.tau / (P.h * P.h);
+    const double h4 = P.h * P.h * P.h * P.h;
+    double* part_r = part_;
+    double* part_b = part_ + npart_;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof.on) {
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, st_);
+    }
+    k_sor_sweep<double, 3, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
+    k_sor_sweep<double, 3, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);
+    if (launch_check("k_sor_sweep defect")) return -1;
+    if (prof.on) (void)hipEventRecord(e1, st_);
+    // z = 1, cap = 1: the reduction always records R and raises stop; the
+    // outer decision is taken on the host from R and tol
+    k_reduce_resid<<<1, 1024, 0, st_>>>(part_r, nb, part_b, nb, ctl_, nullptr, 1, 1);
+    if (launch_check("k_reduce_resid")) return -1;
+    if (check(hipMemcpyAsync(hctl_, ctl_, sizeof(SorCtl), hipMemcpyDeviceToHost, st_), "ctl copy"))
+        return -1;
+    if (check(hipStreamSynchronize(st_), "defect sync")) return -1;
+    if (prof.on) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        prof.defect_ms += ms;
+        prof.defect_launches += 2;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return 0;
+}
+
+// FP32-refined solve: FP64 defect correction around FP32 red-black SOR on the
+// correction equation A d = b - A u (same matrix, coefficients rounded to
+// FP32).  Stops on the FP64 residual: R <= tol (the reference's test), or
+// after max_rounds_ corrections.
+int Engine::solve_mixed(const StepParams& P, SolveStats* out) {
+    if (ensure_mixed()) return -1;
+    const long nint = (long)(I_ - 2) * (J_ - 2) * (K_ - 2) * (L_ - 2);
+    SorJob<float> J;
+    J.c = (float)(P.tau / (P.h * P.h));
+    J.om = (float)P.omega;
+    J.h4 = P.h * P.h * P.h * P.h;
+    J.opnd[0] = op32_[0];
+    J.opnd[1] = op32_[1];
+    J.red[0] = dr_[0];
+    J.red[1] = dr_[1];
+    J.black = db_;
+    J.ctl = ctl2_;
+    J.hctl = hctl2_;
+    J.cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
+    J.kind0 = 3;
+    long sweeps = 0;
+    int rounds = 0;
+    if (defect64(P)) return -1;
+    while (hctl_->R > hctl_->tol && rounds < max_rounds_) {
+        // d = 0 (ghosts stay 0: homogeneous boundary of the correction)
+        for (float* d : {dr_[0], dr_[1], db_})
+            if (check(hipMemsetAsync(d, 0, (size_t)np_ * sizeof(float), st_), "memset d")) return -1;
+        k_inner_ctl<<<1, 64, 0, st_>>>(ctl_, ctl2_, inner_frac_, inner_floor2_);
+        if (launch_check("k_inner_ctl")) return -1;
+        int z = 0;
+        if (sor_run<float>(*this, J, &z)) return -1;
+        sweeps += z;
+        rounds++;
+        const long nblk = (np_ + kBlock - 1) / kBlock;
+        const unsigned g = (unsigned)(nblk < 8192 ? nblk : 8192);
+        // the accepted red correction is the buffer written by red(z)
+        k_correct<<<g, kBlock, 0, st_>>>(ur_[0], dr_[z & 1], np_);
+        k_correct<<<g, kBlock, 0, st_>>>(ub_, db_, np_);
+        if (launch_check("k_correct")) return -1;
+        if (defect64(P)) return -1;
+    }
+    red_final_ = 0;
+    if (out) {
+        out->z = (int)sweeps;
+        out->R = hctl_->R;
+        out->tol = hctl_->tol;
+        out->sumsq = hctl_->sumsq;
+        out->rounds = rounds;
+    }
+    if (prof.on) {
+        prof.iterations += sweeps;
+        prof.voxel_updates += sweeps * nint;
+        prof.rounds += rounds;
     }
     return 0;
 }

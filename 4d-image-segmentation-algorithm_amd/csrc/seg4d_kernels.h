@@ -160,6 +160,7 @@ __device__ __forceinline__ bool is_interior(const Geo& G, int i, int j, int k, i
 // ----------------------------------------------------------------------------
 struct SplitBufs {
     double* opnd[2];     // [colour] operand blocks (coefficients + prev)
+    float* opnd32[2];    // [colour] FP32 coefficient copies (or null)
     double* ured[2];     // red ping-pong buffers
     double* ublack;
 };
@@ -186,8 +187,9 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
             sb.ublack[pp] = uc;
         }
         if (is_interior(G, i, j, k, l)) {
-            double* ob = (c ? sb.opnd[1] : sb.opnd[0]) +
-                         opnd_index(G, k, l, (j - 1) * G.H + (i >> 1));
+            const long oi = opnd_index(G, k, l, (j - 1) * G.H + (i >> 1));
+            double* ob = (c ? sb.opnd[1] : sb.opnd[0]) + oi;
+            float* ob32 = sb.opnd32[0] ? (c ? sb.opnd32[1] : sb.opnd32[0]) + oi : nullptr;
             ob[8 * 64] = uc;
             double a[8];
             all_face_sq<POW2>(u, p, G, S, a);
@@ -203,6 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
                 if constexpr (WITH_G) cf = vc * g.p[f][p] * (1.0 / N[f]);
                 else cf = vc * (1.0 / N[f]);
                 ob[f * 64] = cf;
+                if (ob32) ob32[f * 64] = (float)cf;
             }
             acc = uc * uc * h4;
         }
@@ -255,21 +258,24 @@ __device__ __forceinline__ bool map_block(const SweepMap& M, int b, int nb, int&
 }
 
 // Operands of one colour point, gathered before any arithmetic so that the
-// loads of U consecutive l-steps are all in flight together.
+// loads of U consecutive l-steps are all in flight together.  T = double for
+// the reference system, float for the correction equation of the FP32-refined
+// form (there component 8 of the operand block holds the right-hand side).
+template <typename T>
 struct PointIn {
-    double c[8];      // e w n s t b fw bw
-    double pv, uo;    // prev, own value
-    double yo;        // in-row neighbour that is not the column partner
-    double yS, yN, yB, yT;
+    T c[8];          // e w n s t b fw bw
+    T pv, uo;        // right-hand side (prev), own value
+    T yo;            // in-row neighbour that is not the column partner
+    T yS, yN, yB, yT;
     int pp;
-    int sft;          // own i = 2m + sft
+    int sft;         // own i = 2m + sft
     bool act;
 };
 
-__device__ __forceinline__ void load_point(const Geo& G, const double* __restrict__ ob,
-                                           const double* __restrict__ y,
-                                           const double* xin, int pp, int sft, bool act,
-                                           PointIn& P) {
+template <typename T>
+__device__ __forceinline__ void load_point(const Geo& G, const T* __restrict__ ob,
+                                           const T* __restrict__ y, const T* xin, int pp,
+                                           int sft, bool act, PointIn<T>& P) {
     P.pp = pp;
     P.sft = sft;
     P.act = act;
@@ -287,16 +293,18 @@ __device__ __forceinline__ void load_point(const Geo& G, const double* __restric
 }
 
 // One point update, reference operand order (segmentation4D.c:242-256); the
-// residual term (segmentation4D.c:290-307) before (RESID 1) or after (2).
-template <int RESID>
-__device__ __forceinline__ void update_point(const PointIn& P, double ybw, double ypart,
-                                             double yfw, double c, double om, double h4,
-                                             double* xout, double& acc) {
+// residual term r = u (1 + c sum) - c sum coef u_nb - prev (segmentation4D.c:
+// 290-307) before (RESID 1) or after (RESID 2) the update.  RESID 3 does not
+// update: it writes -r (the defect, FP32) to *defect and accumulates r^2 h^4.
+template <typename T, int RESID>
+__device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart, T yfw, T c,
+                                             T om, double h4, T* xout, float* defect,
+                                             double& acc) {
     if (!P.act) return;
-    const double uW = P.sft ? ypart : P.yo;
-    const double uE = P.sft ? P.yo : ypart;
+    const T uW = P.sft ? ypart : P.yo;
+    const T uE = P.sft ? P.yo : ypart;
     // numerator order e, w, s, n, b, t, bw, fw
-    double num = P.c[0] * uE;
+    T num = P.c[0] * uE;
     num = num + P.c[1] * uW;
     num = num + P.c[3] * P.yS;
     num = num + P.c[2] * P.yN;
@@ -305,24 +313,30 @@ __device__ __forceinline__ void update_point(const PointIn& P, double ybw, doubl
     num = num + P.c[7] * ybw;
     num = num + P.c[6] * yfw;
     // diagonal order e, w, n, s, t, b, fw, bw
-    double dg = P.c[0] + P.c[1];
+    T dg = P.c[0] + P.c[1];
     dg = dg + P.c[2];
     dg = dg + P.c[3];
     dg = dg + P.c[4];
     dg = dg + P.c[5];
     dg = dg + P.c[6];
     dg = dg + P.c[7];
-    const double D = 1 + c * dg;
-    if constexpr (RESID == 1) {
-        const double r = P.uo * D - c * num - P.pv;
-        acc += r * r * h4;
+    const T D = 1 + c * dg;
+    if constexpr (RESID == 1 || RESID == 3) {
+        const T r = P.uo * D - c * num - P.pv;
+        const double rd = (double)r;
+        acc += rd * rd * h4;
+        if constexpr (RESID == 3) {
+            *defect = (float)(-r);
+            return;
+        }
     }
-    const double gs = (P.pv + c * num) / D;
-    const double un = P.uo + om * (gs - P.uo);
+    const T gs = (P.pv + c * num) / D;
+    const T un = P.uo + om * (gs - P.uo);
     xout[P.pp] = un;
     if constexpr (RESID == 2) {
-        const double r = un * D - c * num - P.pv;
-        acc += r * r * h4;
+        const T r = un * D - c * num - P.pv;
+        const double rd = (double)r;
+        acc += rd * rd * h4;
     }
 }
 
@@ -337,14 +351,16 @@ __device__ __forceinline__ void update_point(const PointIn& P, double ybw, doubl
 // red pass of iteration z+1 yields the red residual terms of iteration z:
 // red values and black neighbours are then exactly those at the end of z);
 // 2: residual AFTER the update (black pass of iteration z: its red
-// neighbours are final for z).  segmentation4D.c:287-308.
+// neighbours are final for z).  segmentation4D.c:287-308.  3: residual only,
+// the defect -r written as FP32 into component 8 of the operand blocks
+// `defect_ob` (the right-hand side of the FP32 correction equation).
 // ----------------------------------------------------------------------------
-template <int RESID, int U>
+template <typename T, int RESID, int U>
 __global__ __launch_bounds__(kBlock) void k_sor_sweep(
-    Geo G, SweepMap M, double c, double om, double h4, int colour,
-    const double* xin, double* xout, const double* __restrict__ y,
-    const double* __restrict__ opnd, double* __restrict__ partials,
-    const SorCtl* __restrict__ ctl) {
+    Geo G, SweepMap M, T c, T om, double h4, int colour,
+    const T* xin, T* xout, const T* __restrict__ y,
+    const T* __restrict__ opnd, double* __restrict__ partials,
+    const SorCtl* __restrict__ ctl, float* __restrict__ defect_ob) {
     if (ctl != nullptr && ctl->stop) return;
     int bx, k;
     if (!map_block(M, blockIdx.x, gridDim.x, bx, k)) return;
@@ -357,46 +373,72 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(
     double acc = 0.0;
     const int base = (k * G.J + j) * G.H + m;  // l = 0
     // operand block of (k, l = 1, t); the next l plane is ostride further
-    const double* ob0 = opnd + opnd_index(G, k, 1, t);
+    const long ob_off = opnd_index(G, k, 1, t);
+    const T* ob0 = opnd + ob_off;
     const long ostride = (long)(G.K - 2) * G.RP * kOpnd;
+    float* def0 = (RESID == 3) ? defect_ob + ob_off + 8 * 64 : nullptr;
     // window w[q] = other colour at column m, row l-1+q
-    double w[U + 2];
-    w[0] = rowok ? y[base] : 0.0;
-    w[1] = rowok ? y[base + G.pw] : 0.0;
+    T w[U + 2];
+    w[0] = rowok ? y[base] : T(0);
+    w[1] = rowok ? y[base + G.pw] : T(0);
     const int lmax = G.L - 2;
     int l = 1;
     for (; l + U - 1 <= lmax; l += U) {
-        PointIn P[U];
+        PointIn<T> P[U];
 #pragma unroll
         for (int q = 0; q < U; q++) {
             const int lq = l + q;
             const int pp = base + lq * G.pw;
-            w[q + 2] = rowok ? y[pp + G.pw] : 0.0;
+            w[q + 2] = rowok ? y[pp + G.pw] : T(0);
             const int sft = colour ^ ((j + k + lq) & 1);
             const int i = 2 * m + sft;
-            load_point(G, ob0 + (lq - 1) * ostride, y, xin, pp, sft,
-                       rowok && i >= 1 && i <= G.I - 2, P[q]);
+            load_point<T>(G, ob0 + (lq - 1) * ostride, y, xin, pp, sft,
+                          rowok && i >= 1 && i <= G.I - 2, P[q]);
         }
 #pragma unroll
-        for (int q = 0; q < U; q++) update_point<RESID>(P[q], w[q], w[q + 1], w[q + 2], c, om, h4, xout, acc);
+        for (int q = 0; q < U; q++)
+            update_point<T, RESID>(P[q], w[q], w[q + 1], w[q + 2], c, om, h4, xout,
+                                   def0 ? def0 + (l + q - 1) * ostride : nullptr, acc);
         w[0] = w[U];
         w[1] = w[U + 1];
     }
     for (; l <= lmax; l++) {  // remainder, one step at a time
-        PointIn P;
+        PointIn<T> P;
         const int pp = base + l * G.pw;
-        w[2] = rowok ? y[pp + G.pw] : 0.0;
+        w[2] = rowok ? y[pp + G.pw] : T(0);
         const int sft = colour ^ ((j + k + l) & 1);
         const int i = 2 * m + sft;
-        load_point(G, ob0 + (l - 1) * ostride, y, xin, pp, sft,
-                   rowok && i >= 1 && i <= G.I - 2, P);
-        update_point<RESID>(P, w[0], w[1], w[2], c, om, h4, xout, acc);
+        load_point<T>(G, ob0 + (l - 1) * ostride, y, xin, pp, sft,
+                      rowok && i >= 1 && i <= G.I - 2, P);
+        update_point<T, RESID>(P, w[0], w[1], w[2], c, om, h4, xout,
+                               def0 ? def0 + (l - 1) * ostride : nullptr, acc);
         w[0] = w[1];
         w[1] = w[2];
     }
     if constexpr (RESID != 0) {
         const double bs = block_sum256(acc);
         if (threadIdx.x == 0) partials[lb] = bs;
+    }
+}
+
+// u += d over a colour array (the defect-correction update; ghosts of d are 0).
+__global__ __launch_bounds__(kBlock) void k_correct(double* __restrict__ u,
+                                                   const float* __restrict__ d, long n) {
+    for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < n; q += (long)gridDim.x * kBlock)
+        u[q] = u[q] + (double)d[q];
+}
+
+// Sets the control block of one FP32 correction solve: stop when its residual
+// is below max(floor2 * R, frac * tol) of the outer FP64 system.
+__global__ void k_inner_ctl(const SorCtl* __restrict__ outer, SorCtl* inner, double frac,
+                            double floor2) {
+    if (threadIdx.x == 0) {
+        const double a = frac * outer->tol, b2 = floor2 * outer->R;
+        inner->tol = a > b2 ? a : b2;
+        inner->R = 0.0;
+        inner->z = 0;
+        inner->stop = 0;
+        inner->zfinal = 0;
     }
 }
 

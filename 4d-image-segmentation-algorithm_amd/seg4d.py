@@ -143,6 +143,11 @@ class SegLib(_Lib):
         L.seg4dProfileRead.argtypes = [POINTER(c_double), c_int]
         L.seg4dSetSweepVariant.restype = c_int
         L.seg4dSetSweepVariant.argtypes = [c_int]
+        L.seg4dSetPrecision.restype = c_int
+        L.seg4dSetPrecision.argtypes = [c_int]
+        L.seg4dSetRefinement.restype = None
+        L.seg4dSetRefinement.argtypes = [c_int, c_double]
+        L.seg4dLastRounds.restype = c_int
         L.seg4dDeviceInfo.restype = c_int
         L.seg4dDeviceInfo.argtypes = [c_char_p, c_int]
         L.seg4d_gen_preset.restype = c_int
@@ -237,12 +242,25 @@ class SegLib(_Lib):
         self.lib.seg4dProfileReset()
 
     def profile_read(self):
-        out = (c_double * 14)()
-        self.lib.seg4dProfileRead(out, 14)
+        out = (c_double * 23)()
+        self.lib.seg4dProfileRead(out, 23)
         keys = ("red0_ms", "red0_launches", "red1_ms", "red1_launches",
                 "black_ms", "black_launches", "setup_ms", "setup_launches",
-                "step_ms", "steps", "voxel_updates", "iterations", "nred", "nblack")
+                "step_ms", "steps", "voxel_updates", "iterations", "nred", "nblack",
+                "red0_f32_ms", "red0_f32_launches", "red1_f32_ms", "red1_f32_launches",
+                "black_f32_ms", "black_f32_launches", "defect_ms", "defect_launches",
+                "rounds")
         return dict(zip(keys, list(out)))
+
+    def set_precision(self, mode):
+        """0 = FP64 (reference arithmetic), 1 = FP32-refined defect correction."""
+        return self.lib.seg4dSetPrecision(int(mode))
+
+    def set_refinement(self, max_rounds=0, inner_floor=0.0):
+        self.lib.seg4dSetRefinement(int(max_rounds), float(inner_floor))
+
+    def last_rounds(self):
+        return self.lib.seg4dLastRounds()
 
     def set_sweep_variant(self, v):
         return self.lib.seg4dSetSweepVariant(int(v))

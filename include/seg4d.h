@@ -124,8 +124,10 @@ double seg4dLastTolerance(void);
  * fills out[0..13] = {ms, launches} for the three SOR pass kernels (red
  * without residual, red with fused residual, black with fused residual),
  * then setup_ms, setup_launches, step_ms, steps, voxel_updates (interior
- * voxels x SOR iterations), iterations, red points, black points; returns
- * the number of values written. */
+ * voxels x SOR iterations), iterations, red points, black points;
+ * out[14..19] the same three {ms, launches} pairs for the FP32 correction
+ * sweeps, out[20..22] = FP64 defect ms, defect launches, correction rounds.
+ * Returns the number of values written (at most 23). */
 void seg4dProfile(int on);
 int seg4dProfileRead(double *out, int nout);
 void seg4dProfileReset(void);
@@ -134,6 +136,22 @@ void seg4dProfileReset(void);
  * share neighbour planes are co-resident on one XCD).  Default 3.  Returns
  * the previous value, -1 before _allocateMem_. */
 int seg4dSetSweepVariant(int v);
+/* Arithmetic of the SOR solve.  0 (default): FP64, the reference's
+ * arithmetic and bits.  1: FP32-refined -- FP64 defect correction in which
+ * the defect b - A u of the FP64 system is recomputed in FP64 on the GPU and
+ * an FP32 red-black SOR solve of A d = b - A u (coefficients rounded to FP32)
+ * is applied; the stopping test stays the reference's on the FP64 residual
+ * (R <= gsTol, or the relative bound), and maxNoGSIteration caps each FP32
+ * solve.  Returns the previous mode, -1 before _allocateMem_. */
+int seg4dSetPrecision(int mode);
+/* FP32-refined controls: at most max_rounds corrections per solve (default
+ * 8); each FP32 solve stops once its residual is below max(tol/4,
+ * inner_floor^2 * R) of the current FP64 residual R (default 1e-5).
+ * Non-positive values keep the current setting. */
+void seg4dSetRefinement(int max_rounds, double inner_floor);
+/* Correction rounds of the last FP32-refined solve (0 in FP64 mode);
+ * seg4dLastSweeps then counts FP32 sweeps. */
+int seg4dLastRounds(void);
 /* Blocks until all queued device work is done. */
 bool seg4dSynchronize(void);
 /* Device name / CU count / arch into buf; returns 0 on success. */
