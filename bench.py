@@ -173,6 +173,7 @@ def main():
     t1 = time.perf_counter()
     prof = lib.profile_read()
     lib.profile(False)
+    nred, nblack = prof["nred"], prof["nblack"]
     total = t1 - t0
     log(f"[bench] {args.steps} steps in {total:.3f} s, sweeps/step {sweeps}")
     if args.calib_copy:
@@ -238,7 +239,6 @@ def main():
         log(f"[bench] FP32-refined: {t3 - t2:.3f} s ({total / (t3 - t2):.2f}x), "
             f"sweeps {msweeps}, rounds {mrounds}, IoU {mixed['mask_iou_vs_fp64_final']:.5f}")
 
-    nred, nblack = prof["nred"], prof["nblack"]
     red1_avg_s = prof["red1_ms"] / max(prof["red1_launches"], 1) / 1e3
     achieved = nred * BYTES_PER_UPDATE / red1_avg_s / 1e9
     sweep_ms = prof["red0_ms"] + prof["red1_ms"] + prof["black_ms"]
