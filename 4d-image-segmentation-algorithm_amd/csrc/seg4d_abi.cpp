@@ -451,6 +451,13 @@ API void seg4dSetRefinement(int max_rounds, double inner_floor) {
 
 API int seg4dLastRounds(void) { return g_last.rounds; }
 
+API int seg4dSetWarmStart(int mode) {
+    if (!g_eng) return -1;
+    const int old = g_eng->warm_;
+    g_eng->warm_ = (mode == 1) ? 1 : 0;
+    return old;
+}
+
 API int seg4dSetSweepVariant(int v) {
     if (!g_eng) return -1;
     const int old = g_eng->sweep_variant_;

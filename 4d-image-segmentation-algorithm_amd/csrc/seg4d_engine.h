@@ -128,6 +128,10 @@ public:
     // the correction equation inside FP64 defect correction.
     int precision_ = 0;
     int max_rounds_ = 8;           // FP32-refined: correction rounds per solve
+    // SOR start: 0 = u^n (the reference); 1 = 2 u^n - u^(n-1) when the
+    // previous call was a completed time step on the device (hist_ok_).
+    int warm_ = 0;
+    bool hist_ok_ = false;
     double inner_frac_ = 0.25;     // inner stop: R32 <= max(frac*tol, floor2*R)
     double inner_floor2_ = 1e-10;  // (1e-5)^2: what an FP32 solve can resolve
     Profile prof;

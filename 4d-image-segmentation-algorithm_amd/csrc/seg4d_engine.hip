@@ -160,6 +160,7 @@ static Geo make_geo(int I, int J, int K, int L, int H) {
 
 int Engine::upload(ArrayId a, const double* host) {
     if (!full_[a]) return check(hipErrorInvalidValue, "upload: array not allocated");
+    if (a == A_U || a == A_PREV) hist_ok_ = false;
     if (check(hipMemcpyAsync(full_[a], host, (size_t)n_ * sizeof(double),
                              hipMemcpyHostToDevice, st_), "upload"))
         return -1;
@@ -189,6 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_f64(const double* __restrict__ 
 int Engine::copy(ArrayId src, ArrayId dst) {
     if (!full_[src] || !full_[dst]) return check(hipErrorInvalidValue, "copy: array not allocated");
     if (src == dst) return 0;
+    if (dst == A_U || dst == A_PREV) hist_ok_ = false;
     const long nb = (n_ + kBlock - 1) / kBlock;
     k_copy_f64<<<(unsigned)(nb < 65536 ? nb : 65536), kBlock, 0, st_>>>(full_[src], full_[dst], n_);
     return launch_check("k_copy_f64");
@@ -224,19 +226,31 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
         prevf = scratch(A_TMP2);
         if (!prevf) return -1;
     }
-    dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
+    // the extrapolated start needs prev_full = u^(n-1) from a completed step
+    const int warm = (warm_ == 1 && hist_ok_ && write_prev) ? 1 : 0;
+    PlaneMap PM;
+    PM.gx = (int)cdiv((long)I_ * J_, kBlock);
+    PM.gk = K_;
+    PM.gl = L_;
+    PM.tx = 8;
+    PM.tk = 8;
+    PM.tl = 4;
+    PM.ntx = (PM.gx + PM.tx - 1) / PM.tx;
+    PM.ntk = (PM.gk + PM.tk - 1) / PM.tk;
+    PM.ntiles = PM.ntx * PM.ntk * ((PM.gl + PM.tl - 1) / PM.tl);
+    const dim3 grid((unsigned)(((PM.ntiles + 7) / 8) * 8 * PM.tx * PM.tk * PM.tl));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof.on) {
-        hipEventCreate(&e0);
-        hipEventCreate(&e1);
-        hipEventRecord(e0, st_);
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, st_);
     }
     if (with_g) {
-        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_);
-        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_);
+        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_, warm);
+        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_, warm);
     } else {
-        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_);
-        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_);
+        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_, warm);
+        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_, warm);
     }
     if (launch_check("k_setup_step")) return -1;
     if (prof.on) {
@@ -249,7 +263,7 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
         hipEventDestroy(e0);
         hipEventDestroy(e1);
     }
-    const int np = (int)(grid.x * grid.y * grid.z);
+    const int np = PM.gx * PM.gk * PM.gl;  // one partial per (run, k, l)
     k_init_ctl<<<1, 1024, 0, st_>>>(part_, np, ctl_, P.gsTol, P.rel2);
     return launch_check("k_init_ctl");
 }
@@ -586,7 +600,9 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
         hipEventDestroy(e0);
         hipEventDestroy(e1);
     }
-    return check(hipStreamSynchronize(st_), "seg_step");
+    if (check(hipStreamSynchronize(st_), "seg_step")) return -1;
+    hist_ok_ = true;  // prev_full = u^n and u = u^(n+1)
+    return 0;
 }
 
 int Engine::mcf_step(const StepParams& P, double t, SolveStats* st) {
@@ -599,7 +615,9 @@ int Engine::mcf_step(const StepParams& P, double t, SolveStats* st) {
     if (launch_check("k_pack_ghosts")) return -1;
     if (solve(P, st)) return -1;
     if (unpack(nullptr)) return -1;
-    return check(hipStreamSynchronize(st_), "mcf_step");
+    if (check(hipStreamSynchronize(st_), "mcf_step")) return -1;
+    hist_ok_ = true;
+    return 0;
 }
 
 int Engine::reflect(double* d) {
@@ -653,6 +671,7 @@ int Engine::presmooth(double* d, double sigma, double h, double omega, double gs
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     const double c = sigma / (h * h);
     const double h4 = h * h * h * h;
+    hist_ok_ = false;  // prev is overwritten with the image
     if (check(hipMemcpyAsync(full_[A_PREV], d, (size_t)n_ * sizeof(double),
                              hipMemcpyDeviceToDevice, st_), "presmooth copy"))
         return -1;

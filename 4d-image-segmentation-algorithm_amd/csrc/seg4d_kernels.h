@@ -165,13 +165,40 @@ struct SplitBufs {
     double* ublack;
 };
 
+// Block -> (run bx, plane k, frame l) map of full-layout kernels whose
+// stencil reaches the k+-1 and l+-1 planes: every XCD processes whole tiles
+// of tx runs x tk planes x tl frames per residency round, so the planes a
+// 33-point stencil re-reads are fetched once into that XCD's L2.
+struct PlaneMap {
+    int gx, gk, gl;       // runs of kBlock points per (j,i) plane, K, L
+    int tx, tk, tl;
+    int ntx, ntk, ntiles;
+};
+
+__device__ __forceinline__ bool map_plane_block(const PlaneMap& M, int b, int& bx, int& k,
+                                                int& l) {
+    const int x = b % 8, s = b / 8;
+    const int per = M.tx * M.tk * M.tl;
+    const int r = s / per, w = s - r * per;
+    const int T = r * 8 + x;
+    if (T >= M.ntiles) return false;
+    const int tix = T % M.ntx, tik = (T / M.ntx) % M.ntk, til = T / (M.ntx * M.ntk);
+    bx = tix * M.tx + w % M.tx;
+    k = tik * M.tk + (w / M.tx) % M.tk;
+    l = til * M.tl + w / (M.tx * M.tk);
+    return bx < M.gx && k < M.gk && l < M.gl;
+}
+
+// warm: 0 = SOR starts from u^n (the reference); 1 = it starts from the
+// linear extrapolation 2 u^n - u^(n-1) (prev_full holds u^(n-1) on entry).
 template <bool WITH_G, bool POW2>
 __global__ __launch_bounds__(kBlock) void k_setup_step(
-    Geo G, Scale S, double eps2, double h4, const double* __restrict__ u,
+    Geo G, PlaneMap PM, Scale S, double eps2, double h4, const double* __restrict__ u,
     CPtr8 g, SplitBufs sb, double* __restrict__ prev_full,
-    double* __restrict__ partials) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int k = blockIdx.y, l = blockIdx.z;
+    double* __restrict__ partials, int warm) {
+    int bx, k, l;
+    if (!map_plane_block(PM, blockIdx.x, bx, k, l)) return;
+    const int t = bx * kBlock + threadIdx.x;
     double acc = 0.0;
     if (t < G.I * G.J) {
         const int j = t / G.I, i = t - j * G.I;
@@ -179,12 +206,14 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
         const int c = (i + j + k + l) & 1;
         const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
         const double uc = u[p];
+        double start = uc;
+        if (warm == 1 && is_interior(G, i, j, k, l)) start = 2.0 * uc - prev_full[p];
         prev_full[p] = uc;
         if (c == 0) {
-            sb.ured[0][pp] = uc;
-            sb.ured[1][pp] = uc;
+            sb.ured[0][pp] = start;
+            sb.ured[1][pp] = start;
         } else {
-            sb.ublack[pp] = uc;
+            sb.ublack[pp] = start;
         }
         if (is_interior(G, i, j, k, l)) {
             const long oi = opnd_index(G, k, l, (j - 1) * G.H + (i >> 1));
@@ -211,8 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
         }
     }
     const double bs = block_sum256(acc);
-    if (threadIdx.x == 0)
-        partials[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = bs;
+    if (threadIdx.x == 0) partials[(l * PM.gk + k) * PM.gx + bx] = bs;
 }
 
 // ----------------------------------------------------------------------------

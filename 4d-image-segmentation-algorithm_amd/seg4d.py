@@ -148,6 +148,8 @@ class SegLib(_Lib):
         L.seg4dSetRefinement.restype = None
         L.seg4dSetRefinement.argtypes = [c_int, c_double]
         L.seg4dLastRounds.restype = c_int
+        L.seg4dSetWarmStart.restype = c_int
+        L.seg4dSetWarmStart.argtypes = [c_int]
         L.seg4dDeviceInfo.restype = c_int
         L.seg4dDeviceInfo.argtypes = [c_char_p, c_int]
         L.seg4d_gen_preset.restype = c_int
@@ -261,6 +263,10 @@ class SegLib(_Lib):
 
     def last_rounds(self):
         return self.lib.seg4dLastRounds()
+
+    def set_warm_start(self, mode):
+        """0 = start SOR from u^n (reference), 1 = from 2 u^n - u^(n-1)."""
+        return self.lib.seg4dSetWarmStart(int(mode))
 
     def set_sweep_variant(self, v):
         return self.lib.seg4dSetSweepVariant(int(v))

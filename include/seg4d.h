@@ -152,6 +152,13 @@ void seg4dSetRefinement(int max_rounds, double inner_floor);
 /* Correction rounds of the last FP32-refined solve (0 in FP64 mode);
  * seg4dLastSweeps then counts FP32 sweeps. */
 int seg4dLastRounds(void);
+/* Start of the SOR solve.  0 (default): u^n, as the reference.  1: the
+ * linear extrapolation 2 u^n - u^(n-1), used when the previous call was a
+ * completed _segTimeStep_ whose state stayed on the device (resident mode);
+ * the system solved, its right-hand side and the stopping test are
+ * unchanged, only the number of sweeps (and the last bits of u) differ.
+ * Returns the previous mode, -1 before _allocateMem_. */
+int seg4dSetWarmStart(int mode);
 /* Blocks until all queued device work is done. */
 bool seg4dSynchronize(void);
 /* Device name / CU count / arch into buf; returns 0 on success. */
