@@ -73,6 +73,53 @@ def test_fp32_refined_vs_oracle(seglib, oracle, dims):
         lib.deallocate()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_extrapolated_start_vs_oracle(seglib, oracle, precision):
+    """SOR started from 2 u^n - u^(n-1) (resident mode): same stopping test,
+    trajectory within the stated error of the FP64 oracle, fewer sweeps."""
+    lib = seglib
+    dims = (40, 36, 20, 6)
+    shape, img, u0 = _volume(lib, dims, 7)
+    lib.allocate(*dims)
+    try:
+        lib.set(gsTol=0.0, **KW)
+        lib.set_relative_tolerance(REL)
+        lib.arr("i0")[...] = img
+        lib.arr("u")[...] = 0.0
+        lib.arr("Mmask")[...] = 1.0
+        lib.gf("i0", "u")
+        g = [lib.arr("g_" + f).copy() for f in ("e", "w", "n", "s", "t", "b", "fw", "bw")]
+        lib.set_resident(True)
+        lib.arr("u")[...] = u0
+        lib.upload("u")
+        lib.upload("Mmask")
+        lib.set_precision(precision)
+        lib.set_warm_start(1)
+        u, prev, coef = u0.copy(), np.zeros(shape), zeros(shape, 8)
+        cold, warm = [], []
+        for st in range(6):
+            p = params(shape, gsTol=REL * REL * oracle.sumsq_interior(params(shape, **KW), u), **KW)
+            cold.append(oracle.seg_time_step(p, u, prev, g, coef, np.ones(shape))[0])
+            z, R = lib.seg_time_step()
+            warm.append(z)
+            assert R <= lib.lib.seg4dLastTolerance()
+        lib.download("u")
+        ug = lib.arr("u")
+        assert np.abs(ug - u).max() <= 1e-4 * np.abs(u).max()
+        iou = oracle.iou(params(shape), oracle.mask(params(shape), u),
+                         oracle.mask(params(shape), np.ascontiguousarray(ug)))
+        assert iou >= 0.999
+        print(f"  precision {precision}: oracle sweeps {cold}, extrapolated start {warm}", flush=True)
+        if precision == 0:
+            assert sum(warm[2:]) < sum(cold[2:])
+    finally:
+        lib.set_warm_start(0)
+        lib.set_precision(0)
+        lib.set_resident(False)
+        lib.set_relative_tolerance(0.0)
+        lib.deallocate()
+
+
 def test_fp32_refined_round_cap_and_fp64_switch(seglib, oracle):
     """A round cap of 1 still returns a correction; switching back to FP64
     restores bitwise parity with the oracle on the next step."""
