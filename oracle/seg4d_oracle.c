@@ -418,6 +418,119 @@ void orc_rescale(const orc_params *p, double *u, const double *mask,
                         u[orc_ijkl(p, i, j, k, l)] = 0;
 }
 
+/* Ball bounding box of segmentation4D.c:907-912 etc., clamped to the array. */
+static void ball_box(const orc_params *p, double cx, double cy, double cz, double R,
+                     int box[6])
+{
+    box[0] = (int)(cx - R - 0.5); box[1] = (int)(cx + R + 0.5);
+    box[2] = (int)(cy - R - 0.5); box[3] = (int)(cy + R + 0.5);
+    box[4] = (int)(cz - R - 0.5); box[5] = (int)(cz + R + 0.5);
+    const int hi[3] = {p->iMax - 1, p->jMax - 1, p->kMax - 1};
+    for (int a = 0; a < 3; a++) {
+        if (box[2 * a] < 0) box[2 * a] = 0;
+        if (box[2 * a + 1] > hi[a]) box[2 * a + 1] = hi[a];
+    }
+}
+
+/* segmentation4D.c:876-895 -- max over the interior, starting from 0 */
+double orc_get4dmax(const orc_params *p, const double *img)
+{
+    double mx = 0;
+    for (int l = 1; l <= p->lMax - 2; l++)
+        for (int k = 1; k <= p->kMax - 2; k++)
+            for (int j = 1; j <= p->jMax - 2; j++)
+                for (int i = 1; i <= p->iMax - 2; i++)
+                    if (img[orc_ijkl(p, i, j, k, l)] > mx) mx = img[orc_ijkl(p, i, j, k, l)];
+    return mx;
+}
+
+/* segmentation4D.c:897-969 -- min (start 100000) or max (start 0) of frame
+ * l inside the ball |x - c| <= R */
+double orc_ball_minmax(const orc_params *p, const double *img, int l, double cx,
+                       double cy, double cz, double R, int want_max)
+{
+    int box[6];
+    ball_box(p, cx, cy, cz, R, box);
+    double v = want_max ? 0 : 100000;
+    for (int k = box[4]; k <= box[5]; k++)
+        for (int j = box[2]; j <= box[3]; j++)
+            for (int i = box[0]; i <= box[1]; i++) {
+                double dx = i - cx, dy = j - cy, dz = k - cz;
+                if (sqrt((dx * dx) + (dy * dy) + (dz * dz)) <= R) {
+                    double x = img[orc_ijkl(p, i, j, k, l)];
+                    if (want_max ? (x > v) : (x < v)) v = x;
+                }
+            }
+    return v;
+}
+
+/* segmentation4D.c:971-1018 -- inside each ball, voxels brighter than
+ * alpha*min + beta*max (of the ball) are set to the image's global max */
+void orc_threshold(const orc_params *p, const double *in, double *thr,
+                   const orc_balls *balls, double alpha, double beta)
+{
+    const double gmax = orc_get4dmax(p, in);
+    const double R = balls->radius;
+    for (int l = 1; l <= p->lMax - 2; l++)
+        for (int c = 0; c < balls->count[l]; c++) {
+            const double cx = balls->cx[balls->offset[l] + c];
+            const double cy = balls->cy[balls->offset[l] + c];
+            const double cz = balls->cz[balls->offset[l] + c];
+            const double mx = orc_ball_minmax(p, in, l, cx, cy, cz, R, 1);
+            const double mn = orc_ball_minmax(p, in, l, cx, cy, cz, R, 0);
+            const double tv = alpha * mn + beta * mx;
+            int box[6];
+            ball_box(p, cx, cy, cz, R, box);
+            for (int k = box[4]; k <= box[5]; k++)
+                for (int j = box[2]; j <= box[3]; j++)
+                    for (int i = box[0]; i <= box[1]; i++) {
+                        double dx = i - cx, dy = j - cy, dz = k - cz;
+                        if (sqrt((dx * dx) + (dy * dy) + (dz * dz)) <= R &&
+                            in[orc_ijkl(p, i, j, k, l)] > tv)
+                            thr[orc_ijkl(p, i, j, k, l)] = gmax;
+                    }
+        }
+}
+
+/* segmentation4D.c:1106-1148 -- seed function max(u, |1/(d+v) - 1/(Rs+v)|)
+ * inside d <= Rs = 12 (v = 0.5) around every centre, within the
+ * ballRadius box; Mmask = 1 there */
+void orc_generate(const orc_params *p, double *u, double *mask, const orc_balls *balls)
+{
+    const double v = 0.5, Rs = 12.;
+    for (int l = 1; l <= p->lMax - 2; l++)
+        for (int c = 0; c < balls->count[l]; c++) {
+            const double cx = balls->cx[balls->offset[l] + c];
+            const double cy = balls->cy[balls->offset[l] + c];
+            const double cz = balls->cz[balls->offset[l] + c];
+            int box[6];
+            ball_box(p, cx, cy, cz, balls->radius, box);
+            for (int k = box[4]; k <= box[5]; k++)
+                for (int j = box[2]; j <= box[3]; j++)
+                    for (int i = box[0]; i <= box[1]; i++) {
+                        double dx = i - cx, dy = j - cy, dz = k - cz;
+                        double nd = sqrt((dx * dx) + (dy * dy) + (dz * dz));
+                        if (nd <= Rs) {
+                            double nv = fabs((1. / (sqrt((dx * dx) + (dy * dy) + (dz * dz)) + v)) -
+                                             (1. / (Rs + v)));
+                            const long q = orc_ijkl(p, i, j, k, l);
+                            if (u[q] < nv) u[q] = nv;
+                            mask[q] = 1.;
+                        }
+                    }
+        }
+}
+
+/* segmentation4D.c:317-335 -- ghosts included, h^4 multiplied per term */
+double orc_mass(const orc_params *p, const double *a)
+{
+    const double h4 = p->h * p->h * p->h * p->h;
+    const size_t n = orc_size(p);
+    double s = 0;
+    for (size_t q = 0; q < n; q++) s += (a[q] * h4);
+    return s;
+}
+
 /* segmentation4D.c:215-315 */
 int orc_seg_time_step(const orc_params *p, double *u, double *prev,
                       double *const g[8], double *coef[8], const double *mask,

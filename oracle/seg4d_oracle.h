@@ -79,6 +79,18 @@ double orc_residual(const orc_params *p, const double *u, const double *prev,
                     double *const coef[8]);
 /* a10 */ void orc_rescale(const orc_params *p, double *u, const double *mask,
                           const orc_balls *balls);
+
+/* Segmentation initialisation (survey next-row 1), ball boxes clamped to the
+ * array like orc_rescale. */
+double orc_get4dmax(const orc_params *p, const double *img);
+double orc_ball_minmax(const orc_params *p, const double *img, int l, double cx,
+                       double cy, double cz, double R, int want_max);
+void orc_threshold(const orc_params *p, const double *in, double *thr,
+                   const orc_balls *balls, double alpha, double beta);
+void orc_generate(const orc_params *p, double *u, double *mask,
+                  const orc_balls *balls);
+/* sum of a h^4 over the whole padded array, lexicographic order */
+double orc_mass(const orc_params *p, const double *a);
 /* a8 whole step: copy, coefficients, SOR, rescale.  Returns z. */
 int orc_seg_time_step(const orc_params *p, double *u, double *prev,
                       double *const g[8], double *coef[8], const double *mask,

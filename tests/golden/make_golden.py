@@ -92,6 +92,65 @@ def seg_case(name, shape, seed, dparams, iparams, steps, counts=None, centres=No
     print("wrote", name)
 
 
+def init_case(name, shape, seed):
+    """The reference's initialisation chain (segmentation4D.c:1386-1395):
+    _locallyThreshold4Ddata_(i0, u), _generate4DsegmFunct_(u),
+    _rescale4DdataToZeroOne_(u); centres at least ballRadius + 2 voxels from
+    the edges so that the reference's unchecked ball loops stay in bounds."""
+    rng = np.random.default_rng(seed)
+    L_, K_, J_, I_ = shape
+    img = zero_ghosts(np.round(rng.uniform(0, 120, size=shape)) / 255.0)
+    counts = [0] + [int(rng.integers(1, 4)) for _ in range(L_ - 2)] + [0]
+    R = 3.0
+    centres = [[]]
+    for l in range(1, L_ - 1):
+        lst = []
+        for c in range(counts[l]):
+            cx = float(rng.integers(int(R) + 2, I_ - int(R) - 3))
+            cy = float(rng.integers(int(R) + 2, J_ - int(R) - 3))
+            cz = float(rng.integers(int(R) + 2, K_ - int(R) - 3))
+            lst.append((cx, cy, cz))
+            # a bright blob around the centre so that thresholding selects it
+            kk, jj, ii = np.meshgrid(np.arange(K_), np.arange(J_), np.arange(I_), indexing="ij")
+            blob = ((ii - cx) ** 2 + (jj - cy) ** 2 + (kk - cz) ** 2) <= 4.0
+            img[l][blob] = np.round(rng.uniform(180, 255, size=blob.sum())) / 255.0
+        centres.append(lst)
+    centres.append([])
+    ref = RefSeg()
+    ref.allocate(shape, counts)
+    ref.set_centres(centres)
+    ref.setd(ballRadius=R, alpha=0.85, beta=0.15)
+    ref.arr("i0")[...] = img
+    thr = ref.fn("_locallyThreshold4Ddata_", argtypes=[DP, DP])
+    gen = ref.fn("_generate4DsegmFunct_", argtypes=[DP])
+    resc = ref.fn("_rescale4DdataToZeroOne_", argtypes=[DP])
+    gmax = ref.fn("_get4DMax_", restype=c_double, argtypes=[DP])
+    bmax = ref.fn("_get3DMaxWithinR_", restype=c_double,
+                  argtypes=[DP, c_int, c_double, c_double, c_double, c_double])
+    bmin = ref.fn("_get3DMinWithinR_", restype=c_double,
+                  argtypes=[DP, c_int, c_double, c_double, c_double, c_double])
+    mass = ref.fn("_mass4D_", restype=c_double, argtypes=[DP])
+    out = dict(i0=img, counts=np.array(counts, np.int32),
+               centres=np.array([c for lst in centres for c in lst]).reshape(-1, 3),
+               radius=np.float64(R), alpha=np.float64(0.85), beta=np.float64(0.15))
+    l1, c1 = 1, centres[1][0]
+    out["gmax"] = np.float64(gmax(ref.ptr("i0")))
+    out["bmax"] = np.float64(bmax(ref.ptr("i0"), l1, *c1, R))
+    out["bmin"] = np.float64(bmin(ref.ptr("i0"), l1, *c1, R))
+    ref.setd(h=0.5)
+    out["mass"] = np.float64(mass(ref.ptr("i0")))
+    assert thr(ref.ptr("i0"), ref.ptr("u"))
+    out["thr"] = ref.arr("u").copy()
+    assert gen(ref.ptr("u"))
+    out["seed"] = ref.arr("u").copy()
+    out["mask"] = ref.arr("Mmask").copy()
+    assert resc(ref.ptr("u"))
+    out["rescaled"] = ref.arr("u").copy()
+    ref.deallocate()
+    np.savez_compressed(os.path.join(HERE, name), **out)
+    print("wrote", name)
+
+
 def mcf_case(name, m, steps):
     ref = RefMcf()
     ref.seti(iMax=m, jMax=m, kMax=m, lMax=m, maxNoGSIteration=1000)
@@ -137,6 +196,7 @@ def main():
                   maxNoGSIteration_sigma=10.0),
              dict(maxNoGSIteration=5), steps=2)
     mcf_case("mcf.npz", 7, 3)
+    init_case("init.npz", (5, 16, 18, 20), 13)
 
 
 if __name__ == "__main__":

@@ -102,6 +102,14 @@ class Oracle:
         L.orc_iou.argtypes = [P, POINTER(ctypes.c_ubyte), POINTER(ctypes.c_ubyte)]
         L.orc_wtime.restype = c_double
         L.orc_max_threads.restype = c_int
+        L.orc_get4dmax.restype = c_double
+        L.orc_get4dmax.argtypes = [P, DP]
+        L.orc_ball_minmax.restype = c_double
+        L.orc_ball_minmax.argtypes = [P, DP, c_int, c_double, c_double, c_double, c_double, c_int]
+        L.orc_threshold.argtypes = [P, DP, DP, POINTER(OrcBalls), c_double, c_double]
+        L.orc_generate.argtypes = [P, DP, DP, POINTER(OrcBalls)]
+        L.orc_mass.restype = c_double
+        L.orc_mass.argtypes = [P, DP]
 
     @staticmethod
     def _arr8(arrs):
@@ -186,6 +194,22 @@ class Oracle:
         z = self.L.orc_mcf_time_step(ctypes.byref(p), dptr(u), dptr(prev), self._arr8(coef),
                                      step, ctypes.byref(R))
         return z, R.value
+
+    def get4dmax(self, p, a):
+        return self.L.orc_get4dmax(ctypes.byref(p), dptr(a))
+
+    def ball_minmax(self, p, a, l, c, R, want_max):
+        return self.L.orc_ball_minmax(ctypes.byref(p), dptr(a), l, *c, R, int(want_max))
+
+    def threshold(self, p, a, thr, balls, alpha, beta):
+        self.L.orc_threshold(ctypes.byref(p), dptr(a), dptr(thr), ctypes.byref(balls),
+                             alpha, beta)
+
+    def generate(self, p, u, mask, balls):
+        self.L.orc_generate(ctypes.byref(p), dptr(u), dptr(mask), ctypes.byref(balls))
+
+    def mass(self, p, a):
+        return self.L.orc_mass(ctypes.byref(p), dptr(a))
 
     def mask(self, p, u):
         m = np.zeros(u.shape, dtype=np.uint8)

@@ -75,3 +75,32 @@ def test_mcf_golden(oracle):
         oracle.mcf_exact(p, ex, st * hh * hh)
         assert oracle.l2norm(p, ex, u) == fx["err"][st - 1]
     assert np.array_equal(ex, fx["exact_last"])
+
+
+def init_balls(oracle, fx):
+    return balls_of(oracle, fx)
+
+
+def test_init_golden(oracle):
+    """Initialisation chain (segmentation4D.c:1386-1395): local threshold, seed
+    function with Mmask, rescale; plus the helper reductions."""
+    fx = load("init.npz")
+    shape = fx["i0"].shape
+    p = params(shape)
+    balls = init_balls(oracle, fx)
+    R, alpha, beta = float(fx["radius"]), float(fx["alpha"]), float(fx["beta"])
+    img = fx["i0"].copy()
+    assert oracle.get4dmax(p, img) == fx["gmax"]
+    c1 = tuple(fx["centres"][0])
+    assert oracle.ball_minmax(p, img, 1, c1, R, 1) == fx["bmax"]
+    assert oracle.ball_minmax(p, img, 1, c1, R, 0) == fx["bmin"]
+    assert oracle.mass(params(shape, h=0.5), img) == fx["mass"]
+    u = np.zeros(shape)
+    oracle.threshold(p, img, u, balls, alpha, beta)
+    assert np.array_equal(u, fx["thr"])
+    mask = np.zeros(shape)
+    oracle.generate(p, u, mask, balls)
+    assert np.array_equal(u, fx["seed"])
+    assert np.array_equal(mask, fx["mask"])
+    oracle.rescale(p, u, np.ones(shape), balls)
+    assert np.array_equal(u, fx["rescaled"])

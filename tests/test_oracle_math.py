@@ -364,3 +364,45 @@ def test_mask_and_iou_brute_force(oracle):
     union = np.sum(m | m2)
     assert oracle.iou(p, m, m2) == pytest.approx(inter / union)
     assert oracle.iou(p, m, m) == 1.0
+
+
+def test_init_functions_brute_force(oracle):
+    """Seed function and ball reductions against numpy brute force, with
+    centres near and beyond the edges (the oracle clamps the ball boxes)."""
+    rng = np.random.default_rng(3)
+    shape = (4, 9, 10, 11)
+    img = rng.uniform(size=shape)
+    p = params(shape)
+    counts = [0, 2, 1, 0]
+    centres = [[], [(0.3, 8.7, 4.0), (5.2, 4.9, 3.5)], [(10.0, 0.0, 8.0)], []]
+    R = 2.6
+    balls = oracle.make_balls(counts, centres, R)
+    kk, jj, ii = np.meshgrid(np.arange(shape[1]), np.arange(shape[2]), np.arange(shape[3]),
+                             indexing="ij")
+    u = rng.uniform(0, 0.05, size=shape)
+    u_ref = u.copy()
+    mask = np.zeros(shape)
+    mask_ref = mask.copy()
+    for l in range(4):
+        for c in centres[l][: counts[l]]:
+            d = np.sqrt((ii - c[0]) ** 2 + (jj - c[1]) ** 2 + (kk - c[2]) ** 2)
+            ball = d <= R
+            assert oracle.ball_minmax(p, img, l, c, R, 1) == img[l][ball].max()
+            assert oracle.ball_minmax(p, img, l, c, R, 0) == img[l][ball].min()
+            # the seed's box is the ballRadius box, its radius test is d <= 12
+            lo =[max(int(x - R - 0.5), 0) for x in c]
+            hi = [int(x + R + 0.5) for x in c]
+            box = ((ii >= lo[0]) & (ii <= hi[0]) & (jj >= lo[1]) & (jj <= hi[1]) &
+                   (kk >= lo[2]) & (kk <= hi[2]))
+            nv = np.abs(1.0 / (d + 0.5) - 1.0 / 12.5)
+            u_ref[l][box] = np.maximum(u_ref[l][box], nv[box])
+            mask_ref[l][box] = 1.0
+    oracle.generate(p, u, mask, balls)
+    assert np.array_equal(mask, mask_ref)
+    np.testing.assert_allclose(u, u_ref, rtol=1e-15, atol=0)
+    # at the centre the seed is 1/v - 1/(Rs + v) = 2 - 2/25
+    p2 = params((3, 5, 5, 5))
+    u2, m2 = np.zeros((3, 5, 5, 5)), np.zeros((3, 5, 5, 5))
+    oracle.generate(p2, u2, m2, oracle.make_balls([0, 1, 0], [[], [(2.0, 2.0, 2.0)], []], 1.0))
+    assert u2[1, 2, 2, 2] == 2.0 - 1.0 / 12.5
+    assert oracle.get4dmax(p, -img) == 0.0  # the reference's max starts at 0
