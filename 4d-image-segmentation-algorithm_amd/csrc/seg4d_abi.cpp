@@ -366,6 +366,89 @@ API bool _rescale4DdataToZeroOne_(double* a) {
     return g_eng->sync() == 0 ? true : fail("_rescale4DdataToZeroOne_");
 }
 
+// ---- segmentation4D.c:317-335 ------------------------------------------------
+API double _mass4D_(double* a) {
+    if (!a) return false;
+    if (!need_engine("_mass4D_")) return 0.0;
+    double* d = device_of(a, s4d::A_TMP, true);
+    double r = 0.0;
+    if (!d || g_eng->mass(d, h, &r)) {
+        fail("_mass4D_");
+        return 0.0;
+    }
+    return r;
+}
+
+// ---- segmentation4D.c:876-969 ------------------------------------------------
+static void exit_on_null(const void* p, const char* fn) {
+    if (p) return;
+    std::fprintf(stderr, "seg4d: %s: NULL image\n", fn);
+    std::exit(1);
+}
+
+API double _get4DMax_(double* a) {
+    exit_on_null(a, "_get4DMax_");
+    if (!need_engine("_get4DMax_")) return 0.0;
+    double* d = device_of(a, s4d::A_TMP, true);
+    double r = 0.0;
+    if (!d || g_eng->max_interior(d, &r)) {
+        fail("_get4DMax_");
+        return 0.0;
+    }
+    return r;
+}
+
+static double ball_extreme(double* a, int l, double cx, double cy, double cz, double R,
+                           bool want_max, const char* fn) {
+    exit_on_null(a, fn);
+    if (!need_engine(fn)) return 0.0;
+    double* d = device_of(a, s4d::A_TMP, true);
+    double mn = 0.0, mx = 0.0;
+    if (!d || g_eng->ball_minmax(d, l, cx, cy, cz, R, &mn, &mx)) {
+        fail(fn);
+        return 0.0;
+    }
+    return want_max ? mx : mn;
+}
+
+API double _get3DMaxWithinR_(double* a, int l, double cx, double cy, double cz, double R) {
+    return ball_extreme(a, l, cx, cy, cz, R, true, "_get3DMaxWithinR_");
+}
+
+API double _get3DMinWithinR_(double* a, int l, double cx, double cy, double cz, double R) {
+    return ball_extreme(a, l, cx, cy, cz, R, false, "_get3DMinWithinR_");
+}
+
+// ---- segmentation4D.c:971-1018 -----------------------------------------------
+API bool _locallyThreshold4Ddata_(double* in, double* thr) {
+    if (!in || !thr) return false;
+    if (!need_engine("_locallyThreshold4Ddata_")) return false;
+    double* din = device_of(in, s4d::A_TMP, true);
+    double* dthr = thr == in ? din : device_of(thr, s4d::A_TMP2, true);
+    if (!din || !dthr) return fail("_locallyThreshold4Ddata_: cannot resolve arguments");
+    if (g_eng->threshold(din, dthr, collect_balls(), alpha, beta))
+        return fail("_locallyThreshold4Ddata_");
+    if (!g_resident && !download_to(thr, dthr)) return fail("_locallyThreshold4Ddata_: download");
+    return g_eng->sync() == 0 ? true : fail("_locallyThreshold4Ddata_");
+}
+
+// ---- segmentation4D.c:1106-1148 ----------------------------------------------
+API bool _generate4DsegmFunct_(double* a) {
+    if (!a) return false;
+    if (!need_engine("_generate4DsegmFunct_")) return false;
+    double* d = device_of(a, s4d::A_TMP, true);
+    if (!d) return fail("_generate4DsegmFunct_: cannot resolve argument");
+    if (!g_resident && a != Mmask && g_eng->upload(s4d::A_MASK, Mmask))
+        return fail("_generate4DsegmFunct_: upload mask");
+    if (g_eng->seed(d, g_eng->full(s4d::A_MASK), collect_balls()))
+        return fail("_generate4DsegmFunct_");
+    if (!g_resident) {
+        if (!download_to(a, d) || g_eng->download(s4d::A_MASK, Mmask))
+            return fail("_generate4DsegmFunct_: download");
+    }
+    return g_eng->sync() == 0 ? true : fail("_generate4DsegmFunct_");
+}
+
 // ---- extensions ------------------------------------------------------------------
 API int seg4dSetResident(int on) {
     const int old = g_resident;

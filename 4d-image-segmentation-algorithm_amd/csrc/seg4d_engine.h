@@ -16,6 +16,7 @@ namespace s4d {
 struct Geo;
 struct SorCtl;
 struct SweepMap;
+struct BallList;
 
 struct StepParams {
     double h = 1.0, tau = 0.5, eps2 = 1e-6;
@@ -33,7 +34,7 @@ struct SolveStats {
     int rounds = 0;         // FP32-refined: defect-correction rounds
 };
 
-struct Balls {              // rescale-epilogue balls, host copies
+struct Balls {              // ball centres per frame (host copies)
     std::vector<int> count, offset;
     std::vector<double> cx, cy, cz;
     double radius = 0.0;
@@ -99,6 +100,19 @@ public:
     // Colour layout -> u (interior), with the Mmask pass fused if mask.
     int unpack(const double* dmask);
     int rescale(double* d, const Balls& balls, bool apply_mask);
+
+    // --- initialisation (segmentation4D.c:876-1148) -----------------------------
+    // thr = global interior max of `in` where `in` exceeds the ball's
+    // alpha*min + beta*max, inside every ball (thr may alias in).
+    int threshold(const double* in, double* thr, const Balls& balls, double alpha,
+                  double beta);
+    // Seed function max(u, |1/(d+1/2) - 1/12.5|) and mask = 1 around centres.
+    int seed(double* d, double* mask, const Balls& balls);
+    int max_interior(const double* d, double* out);
+    int ball_minmax(const double* d, int l, double cx, double cy, double cz, double R,
+                    double* mn, double* mx);
+    // sum of d h^4 over the whole padded array
+    int mass(const double* d, double h, double* out);
     // Lazily allocated full-layout scratch array (A_TMP or A_TMP2).
     double* scratch(ArrayId a);
     // Whole segmentation step: setup, solve, unpack, rescale epilogue.
@@ -171,6 +185,12 @@ public:
     float* db_ = nullptr;        // correction, black
     SorCtl* ctl2_ = nullptr;     // control block of the correction solves
     SorCtl* hctl2_ = nullptr;
+    // device ball list (count, offset, cx, cy, cz), re-sent when it changes
+    int put_balls(const Balls& b, BallList* out);
+    char* balls_buf_ = nullptr;
+    size_t balls_cap_ = 0;
+    Balls balls_sent_;
+    bool balls_valid_ = false;
 };
 
 // Last error of the calling library (set by the ABI layers).

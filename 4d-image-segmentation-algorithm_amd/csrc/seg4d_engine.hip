@@ -134,6 +134,10 @@ void Engine::release() {
         op32_[c] = dr_[c] = nullptr;
     }
     if (db_) hipFree(db_);
+    if (balls_buf_) hipFree(balls_buf_);
+    balls_buf_ = nullptr;
+    balls_cap_ = 0;
+    balls_valid_ = false;
     if (ctl2_) hipFree(ctl2_);
     if (hctl2_) hipHostFree(hctl2_);
     db_ = nullptr;
@@ -538,33 +542,71 @@ double* Engine::scratch(ArrayId a) {
     return full_[a];
 }
 
+int Engine::put_balls(const Balls& b, BallList* out) {
+    const size_t ni = (size_t)L_ * sizeof(int);
+    const size_t nd = (size_t)(b.total > 0 ? b.total : 1) * sizeof(double);
+    const size_t bytes = 2 * ni + 3 * nd;
+    const bool same = balls_valid_ && b.total == balls_sent_.total &&
+                      b.count == balls_sent_.count && b.offset == balls_sent_.offset &&
+                      b.cx == balls_sent_.cx && b.cy == balls_sent_.cy && b.cz == balls_sent_.cz;
+    if (!same) {
+        if ((int)b.count.size() != L_ || (int)b.offset.size() != L_ ||
+            (int)b.cx.size() < b.total || (int)b.cy.size() < b.total ||
+            (int)b.cz.size() < b.total)
+            return check(hipErrorInvalidValue, "ball list sizes");
+        for (int l = 0; l < L_; l++)
+            if (b.count[l] < 0 || b.offset[l] < 0 || b.offset[l] + b.count[l] > b.total)
+                return check(hipErrorInvalidValue, "ball list offsets");
+        if (bytes > balls_cap_) {
+            if (balls_buf_) {
+                hipStreamSynchronize(st_);
+                hipFree(balls_buf_);
+                balls_buf_ = nullptr;
+                balls_cap_ = 0;
+            }
+            if (check(hipMalloc((void**)&balls_buf_, bytes), "hipMalloc balls")) return -1;
+            balls_cap_ = bytes;
+        }
+        // copies ordered after earlier kernels that read the old list, then
+        // waited for: the host vectors need not outlive this call
+        int rc = 0;
+        auto put = [&](size_t off, const void* src, size_t n) {
+            rc |= check(hipMemcpyAsync(balls_buf_ + off, src, n, hipMemcpyHostToDevice, st_),
+                        "balls");
+        };
+        put(0, b.count.data(), ni);
+        put(ni, b.offset.data(), ni);
+        if (b.total > 0) {
+            const size_t nb = (size_t)b.total * sizeof(double);
+            put(2 * ni, b.cx.data(), nb);
+            put(2 * ni + nd, b.cy.data(), nb);
+            put(2 * ni + 2 * nd, b.cz.data(), nb);
+        }
+        rc |= check(hipStreamSynchronize(st_), "balls sync");
+        if (rc) {
+            balls_valid_ = false;
+            return -1;
+        }
+        balls_sent_ = b;
+        balls_valid_ = true;
+    }
+    out->count = (const int*)balls_buf_;
+    out->offset = (const int*)(balls_buf_ + ni);
+    out->cx = (const double*)(balls_buf_ + 2 * ni);
+    out->cy = (const double*)(balls_buf_ + 2 * ni + nd);
+    out->cz = (const double*)(balls_buf_ + 2 * ni + 2 * nd);
+    out->R = b.radius;
+    return 0;
+}
+
 int Engine::rescale(double* d, const Balls& balls, bool apply_mask) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
+    hist_ok_ = false;  // u changed outside a time step (seg_step sets it again)
     if (balls.total > 0) {
-        // device copies of the ball lists
-        const size_t ni = (size_t)L_ * sizeof(int), nd = (size_t)balls.total * sizeof(double);
-        char* buf = nullptr;
-        const size_t bytes = 2 * ni + 3 * nd;
-        if (check(hipMalloc((void**)&buf, bytes), "hipMalloc balls")) return -1;
-        int* dcount = (int*)buf;
-        int* doff = (int*)(buf + ni);
-        double* dcx = (double*)(buf + 2 * ni);
-        double* dcy = (double*)(buf + 2 * ni + nd);
-        double* dcz = (double*)(buf + 2 * ni + 2 * nd);
-        int rc = 0;
-        rc |= check(hipMemcpyAsync(dcount, balls.count.data(), ni, hipMemcpyHostToDevice, st_), "balls");
-        rc |= check(hipMemcpyAsync(doff, balls.offset.data(), ni, hipMemcpyHostToDevice, st_), "balls");
-        rc |= check(hipMemcpyAsync(dcx, balls.cx.data(), nd, hipMemcpyHostToDevice, st_), "balls");
-        rc |= check(hipMemcpyAsync(dcy, balls.cy.data(), nd, hipMemcpyHostToDevice, st_), "balls");
-        rc |= check(hipMemcpyAsync(dcz, balls.cz.data(), nd, hipMemcpyHostToDevice, st_), "balls");
-        if (!rc) {
-            k_rescale_balls<<<L_ - 2, kBlock, 0, st_>>>(G, d, dcount, doff, dcx, dcy, dcz,
-                                                      balls.radius);
-            rc |= launch_check("k_rescale_balls");
-        }
-        hipStreamSynchronize(st_);
-        hipFree(buf);
-        if (rc) return -1;
+        BallList B;
+        if (put_balls(balls, &B)) return -1;
+        k_rescale_balls<<<L_ - 2, kBlock, 0, st_>>>(G, d, B);
+        if (launch_check("k_rescale_balls")) return -1;
     }
     if (apply_mask) {
         if (!full_[A_MASK]) return check(hipErrorInvalidValue, "rescale: no mask array");
@@ -573,6 +615,83 @@ int Engine::rescale(double* d, const Balls& balls, bool apply_mask) {
         if (launch_check("k_apply_mask")) return -1;
     }
     return 0;
+}
+
+// Interior max into dscal_[1] (device) -- shared by threshold and max_interior.
+static int max_interior_dev(Engine& E, const double* d) {
+    const Geo G = make_geo(E.I_, E.J_, E.K_, E.L_, E.H_);
+    dim3 grid(cdiv((long)(E.I_ - 2) * (E.J_ - 2), kBlock), E.K_ - 2, E.L_ - 2);
+    k_max_interior<<<grid, kBlock, 0, E.st_>>>(G, d, E.part_);
+    if (E.launch_check("k_max_interior")) return -1;
+    k_reduce_max<<<1, 1024, 0, E.st_>>>(E.part_, (int)(grid.x * grid.y * grid.z), E.dscal_ + 1);
+    return E.launch_check("k_reduce_max");
+}
+
+int Engine::threshold(const double* in, double* thr, const Balls& balls, double alpha,
+                      double beta) {
+    hist_ok_ = false;
+    if (max_interior_dev(*this, in)) return -1;
+    if (balls.total <= 0) return 0;
+    BallList B;
+    if (put_balls(balls, &B)) return -1;
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    k_threshold_balls<<<L_ - 2, kBlock, 0, st_>>>(G, in, thr, B, alpha, beta, dscal_ + 1);
+    return launch_check("k_threshold_balls");
+}
+
+int Engine::seed(double* d, double* mask, const Balls& balls) {
+    hist_ok_ = false;
+    if (balls.total <= 0) return 0;
+    BallList B;
+    if (put_balls(balls, &B)) return -1;
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    k_seed_balls<<<L_ - 2, kBlock, 0, st_>>>(G, d, mask, B);
+    return launch_check("k_seed_balls");
+}
+
+int Engine::max_interior(const double* d, double* out) {
+    if (max_interior_dev(*this, d)) return -1;
+    if (check(hipMemcpyAsync(hscal_ + 1, dscal_ + 1, sizeof(double), hipMemcpyDeviceToHost, st_),
+              "scalar copy") ||
+        check(hipStreamSynchronize(st_), "max sync"))
+        return -1;
+    *out = hscal_[1];
+    return 0;
+}
+
+int Engine::ball_minmax(const double* d, int l, double cx, double cy, double cz, double R,
+                        double* mn, double* mx) {
+    if (l < 0 || l >= L_) return check(hipErrorInvalidValue, "ball_minmax: frame out of range");
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    k_ball_minmax<<<1, kBlock, 0, st_>>>(G, d, l, cx, cy, cz, R, dscal_ + 2);
+    if (launch_check("k_ball_minmax")) return -1;
+    if (check(hipMemcpyAsync(hscal_ + 2, dscal_ + 2, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                             st_),
+              "scalar copy") ||
+        check(hipStreamSynchronize(st_), "ball sync"))
+        return -1;
+    *mn = hscal_[2];
+    *mx = hscal_[3];
+    return 0;
+}
+
+namespace {
+__global__ __launch_bounds__(kBlock) void k_scaled_sum(const double* __restrict__ a, long n,
+                                                      double h4, double* __restrict__ partials) {
+    double acc = 0.0;
+    for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < n; q += (long)gridDim.x * kBlock)
+        acc += (a[q] * h4);
+    const double bs = block_sum256(acc);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+}  // namespace
+
+int Engine::mass(const double* d, double h, double* out) {
+    const double h4 = h * h * h * h;
+    const int nblk = 2048;
+    k_scaled_sum<<<nblk, kBlock, 0, st_>>>(d, n_, h4, part_);
+    if (launch_check("k_scaled_sum")) return -1;
+    return reduce_to_host(nblk, out);
 }
 
 int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {

@@ -798,59 +798,108 @@ __global__ __launch_bounds__(kBlock) void k_unpack_coef(Geo G, const double* __r
 }
 
 // ----------------------------------------------------------------------------
-// Per-ball rescale (segmentation4D.c:1020-1092): one block per frame l,
-// balls processed in order c = 0..count-1 (ball 0 overwrites, later balls
-// keep the larger value; each ball's min/max sees the earlier balls'
-// writes).  Bounding boxes are clamped to the padded array.
+// Ball kernels of the segmentation's initialisation and epilogue.  A frame's
+// balls are processed by one block in the reference's order c = 0..count-1
+// (segmentation4D.c:984, :1037, :1117): later balls see the earlier balls'
+// writes, and no two blocks write the same frame.  Ball boxes follow
+// segmentation4D.c:907-912 -- (int) truncation of centre -/+ (R + 0.5) --
+// clamped to the padded array (the reference leaves them unchecked).
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_rescale_balls(Geo G, double* u,
-                                                         const int* __restrict__ count,
-                                                         const int* __restrict__ offset,
-                                                         const double* __restrict__ cx,
-                                                         const double* __restrict__ cy,
-                                                         const double* __restrict__ cz,
-                                                         double R) {
+struct BallList {                // centres of frame l: c[offset[l] + q], q < count[l]
+    const int* count;
+    const int* offset;
+    const double* cx;
+    const double* cy;
+    const double* cz;
+    double R;
+};
+
+struct BallBox {
+    int i0, j0, k0, ni, nj, nk, tot;
+    __device__ __forceinline__ void point(int q, int& i, int& j, int& k) const {
+        i = i0 + q % ni;
+        j = j0 + (q / ni) % nj;
+        k = k0 + q / (ni * nj);
+    }
+};
+
+__device__ __forceinline__ BallBox ball_box(const Geo& G, double x0, double y0, double z0,
+                                            double R) {
+    // lower ends raised to 0, upper ends lowered to the last index; a box
+    // entirely outside the array comes out empty
+    BallBox b;
+    b.k0 = max((int)(z0 - R - 0.5), 0);
+    b.j0 = max((int)(y0 - R - 0.5), 0);
+    b.i0 = max((int)(x0 - R - 0.5), 0);
+    b.nk = min((int)(z0 + R + 0.5), G.K - 1) - b.k0 + 1;
+    b.nj = min((int)(y0 + R + 0.5), G.J - 1) - b.j0 + 1;
+    b.ni = min((int)(x0 + R + 0.5), G.I - 1) - b.i0 + 1;
+    b.tot = (b.ni > 0 && b.nj > 0 && b.nk > 0) ? b.ni * b.nj * b.nk : 0;
+    return b;
+}
+
+__device__ __forceinline__ double ball_dist(int i, int j, int k, double x0, double y0,
+                                            double z0) {
+    const double dx = i - x0, dy = j - y0, dz = k - z0;
+    return sqrt((dx * dx) + (dy * dy) + (dz * dz));
+}
+
+// Block-wide min (from 100000) and max (from 0) of frame l of `a` over the
+// points of box `bb` within distance R of the centre (segmentation4D.c:
+// 897-969).  Every thread receives both; the block is synchronised on exit,
+// so writes that follow cannot overtake a read of this reduction.
+__device__ void ball_minmax_block(const Geo& G, const double* a, int l, const BallBox& bb,
+                                  double x0, double y0, double z0, double R, double* mn_out,
+                                  double* mx_out) {
+    __shared__ double smn[kBlock / 64], smx[kBlock / 64];
+    double mn = 100000, mx = 0;
+    for (int q = threadIdx.x; q < bb.tot; q += kBlock) {
+        int i, j, k;
+        bb.point(q, i, j, k);
+        if (ball_dist(i, j, k, x0, y0, z0) <= R) {
+            const double v = a[((l * G.K + k) * G.J + j) * G.I + i];
+            if (v < mn) mn = v;
+            if (v > mx) mx = v;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, off, 64));
+        mx = fmax(mx, __shfl_xor(mx, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        smn[threadIdx.x >> 6] = mn;
+        smx[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    mn = smn[0];
+    mx = smx[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; w++) {
+        mn = fmin(mn, smn[w]);
+        mx = fmax(mx, smx[w]);
+    }
+    *mn_out = mn;
+    *mx_out = mx;
+    __syncthreads();
+}
+
+// Per-ball rescale to [0, 1] (segmentation4D.c:1020-1092): ball 0 of a frame
+// overwrites, later balls keep the larger value.
+__global__ __launch_bounds__(kBlock) void k_rescale_balls(Geo G, double* u, BallList B) {
     const int l = blockIdx.x + 1;
-    __shared__ double smin[kBlock], smax[kBlock];
-    for (int c = 0; c < count[l]; c++) {
-        const double x0 = cx[offset[l] + c], y0 = cy[offset[l] + c], z0 = cz[offset[l] + c];
-        const int k0 = clampi((int)(z0 - R - 0.5), 0, G.K - 1);
-        const int k1 = clampi((int)(z0 + R + 0.5), 0, G.K - 1);
-        const int j0 = clampi((int)(y0 - R - 0.5), 0, G.J - 1);
-        const int j1 = clampi((int)(y0 + R + 0.5), 0, G.J - 1);
-        const int i0 = clampi((int)(x0 - R - 0.5), 0, G.I - 1);
-        const int i1 = clampi((int)(x0 + R + 0.5), 0, G.I - 1);
-        const int ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
-        const int tot = (ni > 0 && nj > 0 && nk > 0) ? ni * nj * nk : 0;
-        double mn = 100000, mx = 0;
-        for (int q = threadIdx.x; q < tot; q += kBlock) {
-            const int i = i0 + q % ni, j = j0 + (q / ni) % nj, k = k0 + q / (ni * nj);
-            const double dx = i - x0, dy = j - y0, dz = k - z0;
-            const double nd = sqrt((dx * dx) + (dy * dy) + (dz * dz));
-            if (nd <= R) {
-                const double v = u[((l * G.K + k) * G.J + j) * G.I + i];
-                if (v < mn) mn = v;
-                if (v > mx) mx = v;
-            }
-        }
-        smin[threadIdx.x] = mn;
-        smax[threadIdx.x] = mx;
-        __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) {
-                smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
-                smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + s]);
-            }
-            __syncthreads();
-        }
-        const double quot = 1. / (smax[0] - smin[0]);
-        const double offs = smin[0] * quot;
-        __syncthreads();
-        for (int q = threadIdx.x; q < tot; q += kBlock) {
-            const int i = i0 + q % ni, j = j0 + (q / ni) % nj, k = k0 + q / (ni * nj);
-            const double dx = i - x0, dy = j - y0, dz = k - z0;
-            const double nd = sqrt((dx * dx) + (dy * dy) + (dz * dz));
-            if (nd <= R) {
+    for (int c = 0; c < B.count[l]; c++) {
+        const int qc = B.offset[l] + c;
+        const double x0 = B.cx[qc], y0 = B.cy[qc], z0 = B.cz[qc];
+        const BallBox bb = ball_box(G, x0, y0, z0, B.R);
+        double mn, mx;
+        ball_minmax_block(G, u, l, bb, x0, y0, z0, B.R, &mn, &mx);
+        const double quot = 1. / (mx - mn);
+        const double offs = mn * quot;
+        for (int q = threadIdx.x; q < bb.tot; q += kBlock) {
+            int i, j, k;
+            bb.point(q, i, j, k);
+            if (ball_dist(i, j, k, x0, y0, z0) <= B.R) {
                 const int p = ((l * G.K + k) * G.J + j) * G.I + i;
                 const double nv = (quot * u[p]) - offs;
                 if (c == 0) u[p] = nv;
@@ -858,6 +907,112 @@ __global__ __launch_bounds__(kBlock) void k_rescale_balls(Geo G, double* u,
             }
         }
         __syncthreads();
+    }
+}
+
+// Local thresholding (segmentation4D.c:971-1018): inside each ball, points of
+// `in` brighter than alpha*min + beta*max of that ball are set to the
+// image's interior maximum *gmax in `thr` (which may alias `in`).
+__global__ __launch_bounds__(kBlock) void k_threshold_balls(Geo G, const double* in,
+                                                           double* thr, BallList B,
+                                                           double alpha, double beta,
+                                                           const double* gmax_p) {
+    const int l = blockIdx.x + 1;
+    const double gmax = *gmax_p;
+    for (int c = 0; c < B.count[l]; c++) {
+        const int qc = B.offset[l] + c;
+        const double x0 = B.cx[qc], y0 = B.cy[qc], z0 = B.cz[qc];
+        const BallBox bb = ball_box(G, x0, y0, z0, B.R);
+        double mn, mx;
+        ball_minmax_block(G, in, l, bb, x0, y0, z0, B.R, &mn, &mx);
+        const double tv = alpha * mn + beta * mx;
+        for (int q = threadIdx.x; q < bb.tot; q += kBlock) {
+            int i, j, k;
+            bb.point(q, i, j, k);
+            const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+            if (ball_dist(i, j, k, x0, y0, z0) <= B.R && in[p] > tv) thr[p] = gmax;
+        }
+        __syncthreads();
+    }
+}
+
+// Seed of the segmentation function (segmentation4D.c:1106-1148): within the
+// ball box and distance d <= 12, u = max(u, |1/(d + 1/2) - 1/(12 + 1/2)|)
+// and Mmask = 1.
+__global__ __launch_bounds__(kBlock) void k_seed_balls(Geo G, double* u, double* mask,
+                                                      BallList B) {
+    const int l = blockIdx.x + 1;
+    const double v = 0.5, Rs = 12.;
+    for (int c = 0; c < B.count[l]; c++) {
+        const int qc = B.offset[l] + c;
+        const double x0 = B.cx[qc], y0 = B.cy[qc], z0 = B.cz[qc];
+        const BallBox bb = ball_box(G, x0, y0, z0, B.R);
+        for (int q = threadIdx.x; q < bb.tot; q += kBlock) {
+            int i, j, k;
+            bb.point(q, i, j, k);
+            const double d = ball_dist(i, j, k, x0, y0, z0);
+            if (d <= Rs) {
+                const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+                const double nv = fabs((1. / (d + v)) - (1. / (Rs + v)));
+                if (u[p] < nv) u[p] = nv;
+                mask[p] = 1.;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// One ball's min and max (segmentation4D.c:897-969) into out[0], out[1].
+__global__ __launch_bounds__(kBlock) void k_ball_minmax(Geo G, const double* a, int l,
+                                                       double x0, double y0, double z0,
+                                                       double R, double* out) {
+    const BallBox bb = ball_box(G, x0, y0, z0, R);
+    double mn, mx;
+    ball_minmax_block(G, a, l, bb, x0, y0, z0, R, &mn, &mx);
+    if (threadIdx.x == 0) {
+        out[0] = mn;
+        out[1] = mx;
+    }
+}
+
+// Interior maximum from 0 (segmentation4D.c:876-895), per-block partials;
+// the order of a max does not change its value.
+__global__ __launch_bounds__(kBlock) void k_max_interior(Geo G, const double* __restrict__ a,
+                                                        double* __restrict__ partials) {
+    __shared__ double sm[kBlock / 64];
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y + 1, l = blockIdx.z + 1;
+    const int w = G.I - 2;
+    double mx = 0;
+    if (t < w * (G.J - 2)) {
+        const int j = t / w + 1, i = t - (j - 1) * w + 1;
+        const double v = a[((l * G.K + k) * G.J + j) * G.I + i];
+        if (v > mx) mx = v;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sm[0];
+        for (int q = 1; q < kBlock / 64; q++) r = fmax(r, sm[q]);
+        partials[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_max(const double* __restrict__ partials,
+                                                    int np, double* out) {
+    __shared__ double sm[16];
+    double m = 0;
+    for (int q = threadIdx.x; q < np; q += 1024) m = fmax(m, partials[q]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sm[0];
+        for (int q = 1; q < 16; q++) r = fmax(r, sm[q]);
+        *out = r;
     }
 }
 

@@ -116,13 +116,21 @@ class SegLib(_Lib):
             getattr(L, fn).restype = c_bool
             getattr(L, fn).argtypes = []
         for fn in ("_presmoothStep_", "_rescale4DdataToZeroOne_", "_reflection4Ddata_",
-                   "seg4dUpload", "seg4dDownload"):
+                   "_generate4DsegmFunct_", "seg4dUpload", "seg4dDownload"):
             getattr(L, fn).restype = c_bool
             getattr(L, fn).argtypes = [POINTER(c_double)]
         L._gf_inputImage_plus_thresholdedImage_.restype = c_bool
         L._gf_inputImage_plus_thresholdedImage_.argtypes = [POINTER(c_double)] * 2
         L._l2norm_.restype = c_double
         L._l2norm_.argtypes = [POINTER(c_double)] * 2
+        L._locallyThreshold4Ddata_.restype = c_bool
+        L._locallyThreshold4Ddata_.argtypes = [POINTER(c_double)] * 2
+        for fn in ("_get4DMax_", "_mass4D_"):
+            getattr(L, fn).restype = c_double
+            getattr(L, fn).argtypes = [POINTER(c_double)]
+        for fn in ("_get3DMaxWithinR_", "_get3DMinWithinR_"):
+            getattr(L, fn).restype = c_double
+            getattr(L, fn).argtypes = [POINTER(c_double), c_int] + [c_double] * 4
         L._copy4DdataFromSrcToDest_.restype = None
         L._copy4DdataFromSrcToDest_.argtypes = [POINTER(c_double)] * 2
         L.ijkl.restype = c_int
@@ -215,6 +223,34 @@ class SegLib(_Lib):
 
     def l2norm(self, a, b):
         return self.lib._l2norm_(self.ptr(a), self.ptr(b))
+
+    def set_balls(self, centres, radius):
+        """centres[l] = list of (cx, cy, cz) for l = 0..lMax-1, written into
+        the centre arrays _allocateMem_ sized from no_of_centers."""
+        cx, cy, cz = self.centre_arrays()
+        for l, lst in enumerate(centres):
+            for c, (x, y, z) in enumerate(lst):
+                cx[l][c], cy[l][c], cz[l][c] = x, y, z
+        self.set(ballRadius=float(radius))
+
+    def locally_threshold(self, src="i0", dst="u"):
+        self.check(self.lib._locallyThreshold4Ddata_(self.ptr(src), self.ptr(dst)),
+                   "_locallyThreshold4Ddata_")
+
+    def generate_seed(self, name="u"):
+        self.check(self.lib._generate4DsegmFunct_(self.ptr(name)), "_generate4DsegmFunct_")
+
+    def get4dmax(self, name):
+        return self.lib._get4DMax_(self.ptr(name))
+
+    def ball_max(self, name, l, c, R):
+        return self.lib._get3DMaxWithinR_(self.ptr(name), l, *c, R)
+
+    def ball_min(self, name, l, c, R):
+        return self.lib._get3DMinWithinR_(self.ptr(name), l, *c, R)
+
+    def mass(self, name):
+        return self.lib._mass4D_(self.ptr(name))
 
     def copy(self, src, dst):
         self.lib._copy4DdataFromSrcToDest_(self.ptr(src), self.ptr(dst))

@@ -38,6 +38,7 @@
 #include <stdbool.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #include "seg4d_gen.h"
 
@@ -100,7 +101,59 @@ bool _rescale4DdataToZeroOne_(double *imagePtr);
 /* segmentation4D.h:56 / segmentation4D.c:826-864 */
 bool _reflection4Ddata_(double *u);
 
+/* ---- initialisation of the segmentation function ----------------------- */
+/* The balls are the per-frame centres center_x/y/z[l][c], c < no_of_centers[l],
+ * with radius ballRadius; boxes are clamped to the array as in the rescale. */
+/* segmentation4D.h:26 / segmentation4D.c:971-1018: inside every ball, points
+ * of the input brighter than alpha*min + beta*max of that ball are set to
+ * the input's interior maximum (from 0) in the output (which may alias the
+ * input). */
+bool _locallyThreshold4Ddata_(double *inputImagePtr, double *thresholdedImagePtr);
+/* segmentation4D.h:22 / segmentation4D.c:1106-1148: within each ball's box
+ * and distance d <= 12: value = max(value, |1/(d + 0.5) - 1/12.5|) and
+ * Mmask = 1. */
+bool _generate4DsegmFunct_(double *segmFuntionPtr);
+/* segmentation4D.h:38 / segmentation4D.c:876-895: interior maximum, starting
+ * from 0.  Exits the process on NULL, as the reference. */
+double _get4DMax_(double *imagePtr);
+/* segmentation4D.h:40 / segmentation4D.c:934-969: maximum (from 0) of frame
+ * l over |x - c| <= R.  Exits the process on NULL, as the reference. */
+double _get3DMaxWithinR_(double *imagePtr, int l, double center_x, double center_y,
+                         double center_z, double R);
+/* segmentation4D.h:42 / segmentation4D.c:897-932: minimum (from 100000). */
+double _get3DMinWithinR_(double *imagePtr, int l, double center_x, double center_y,
+                         double center_z, double R);
+/* segmentation4D.h:62 / segmentation4D.c:317-335: sum of value * h^4 over
+ * the whole padded array (device reduction: equal to the reference's
+ * sequential sum to rounding, ~1e-15 relative). */
+double _mass4D_(double *dataArray3DPtr1);
+
+/* ---- file formats (host; csrc/seg4d_io.c) -------------------------------- */
+/* They read and write the host arrays.  In resident mode, seg4dUpload after
+ * reading and seg4dDownload before writing. */
+extern FILE *input_file, *input_center_file, *output_file; /* segmentation4D.c:60 */
+/* segmentation4D.h:52 / segmentation4D.c:1149-1216: frames l = 1..lMax-2
+ * from <dir>/movingBalls_457_at_time_%03d.vtk (bytes / 255.) and centres
+ * from <dir>/movingBalls_457_centers_at_timestep_%03d.vtk, each mapped to
+ * (double)(int)(x / spacing + 0.5).  Exits the process on a missing or
+ * malformed file, as the reference (with a message). */
+void _readData_(double *inputImage, unsigned char *fileDirectory);
+/* segmentation4D.h:48 / segmentation4D.c:1282-1313: one frame's bytes after
+ * ten header lines into frame l's interior. */
+bool _load3dDataArrayVTK_(double *imageDataPtr, int l, unsigned char *pathPtr);
+/* segmentation4D.h:50 / segmentation4D.c:1218-1253: <path>_at_time_%03d.vtk
+ * per frame, bytes (unsigned char)(255 v + 0.5).  False if a file cannot be
+ * created. */
+bool _writeData_(double *array4DPtr, unsigned char *pathPtr);
+/* segmentation4D.h:44 / segmentation4D.c:1255-1266 */
+void _readNumberFromFile_(char *dim, FILE *input_file);
+/* segmentation4D.h:46 / segmentation4D.c:1268-1280 */
+void _readMetaData_(FILE *input_file);
+
 /* ---- extensions (no counterpart in the reference) ---------------------- */
+/* The mask writer of the MPI program (segmentation4D_mpi.c:1429-1480): the
+ * _writeData_ files with bytes 1 where value >= 0.5, else 0. */
+bool seg4dWriteMaskVTK(const double *array4DPtr, const char *pathPtr);
 /* 0 = mirror (default), 1 = resident.  Returns the previous mode. */
 int seg4dSetResident(int on);
 /* > 0: before every SOR solve (time step and presmoothing) the library

@@ -104,3 +104,37 @@ def test_init_golden(oracle):
     assert np.array_equal(mask, fx["mask"])
     oracle.rescale(p, u, np.ones(shape), balls)
     assert np.array_equal(u, fx["rescaled"])
+
+
+def test_main_flow_golden(oracle):
+    """The reference main()'s flow: threshold, edge indicator, seed, rescale,
+    three time steps with its own parameters."""
+    fx = load("main_flow.npz")
+    shape = fx["i0"].shape
+    kw = {k[2:]: fx[k].item() for k in fx.files if k.startswith("p_")}
+    alpha, beta, R = kw.pop("alpha"), kw.pop("beta"), kw.pop("ballRadius")
+    p = params(shape, **kw)
+    counts = [int(c) for c in fx["counts"]]
+    flat = [tuple(r) for r in fx["centres"]]
+    centres, q = [], 0
+    for c in counts:
+        centres.append(flat[q:q + c])
+        q += c
+    balls = oracle.make_balls(counts, centres, R)
+    i0, u = fx["i0"].copy(), np.zeros(shape)
+    oracle.threshold(p, i0, u, balls, alpha, beta)
+    prev = np.zeros(shape)
+    g = zeros(shape, 8)
+    oracle.gf(p, i0, u, prev, g)
+    assert np.array_equal(u, fx["thr_s"]) and np.array_equal(i0, fx["i0_s"])
+    for f, gf in zip(FACES, g):
+        assert np.array_equal(gf, fx["g_" + f]), f
+    mask = np.zeros(shape)
+    oracle.generate(p, u, mask, balls)
+    oracle.rescale(p, u, mask, balls)
+    assert np.array_equal(u, fx["u0"]) and np.array_equal(mask, fx["mask"])
+    coef = zeros(shape, 8)
+    for st in range(int(fx["steps"])):
+        oracle.seg_time_step(p, u, prev, g, coef, mask, balls)
+        assert np.array_equal(u, fx[f"u{st + 1}"]), f"step {st + 1}"
+        assert oracle.l2norm(p, prev, u) == fx["l2"][st]
