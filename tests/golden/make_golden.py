@@ -14,6 +14,7 @@ Each .npz holds inputs and the reference's outputs (float64, bitwise):
                mass helpers on a volume with bright blobs at the centres
   main_flow.npz the reference main()'s flow and parameters end to end
   vtk.npz      _readData_ of the files in vtk/ and _writeData_ output bytes
+  mcf_eoc.npz  the MCF program's L2 error at T = 0.0625 for n = 10, 20, 40
 
 Run:  make -C oracle ref && python tests/golden/make_golden.py
 """
@@ -316,6 +317,34 @@ def mcf_case(name, m, steps):
     print("wrote", name)
 
 
+def mcf_eoc_case(name, ns, T):
+    """The MCF program's loop (mcf_4D.c:473-524) at n = ns to the final time
+    T: the L2 error after the last step and the SOR iterations per level."""
+    errs, sweeps = [], []
+    for m in ns:
+        hh = 2.5 / m
+        steps = int(round(T / (hh * hh)))
+        ref = RefMcf()
+        ref.seti(iMax=m + 2, jMax=m + 2, kMax=m + 2, lMax=m + 2, maxNoGSIteration=1000)
+        ref.setd(h=hh, tau=hh * hh, eps2=hh * hh, gsTol=1e-12, omega_c=1.8, mcfTol=1e-12)
+        assert ref.fn("_allocateMem_mcf")()
+        exact = ref.fn("_exactSolution_mcf", restype=None, argtypes=[c_double])
+        exact(0.0)
+        ref.arr("u")[...] = ref.arr("exact_u")
+        step = ref.fn("_mcfTimeStep")
+        for st in range(1, steps + 1):
+            c_int.in_dll(ref.L, "numberOfTimeStep").value = st
+            assert step()
+        exact(steps * hh * hh)
+        l2 = ref.fn("_l2norm_mcf", restype=c_double, argtypes=[DP, DP])
+        errs.append(l2(ref.ptr("exact_u"), ref.ptr("u")))
+        ref.fn("_deallocateMem_mcf")()
+        print(f"  n = {m}: {steps} steps, error {errs[-1]:.6e}", flush=True)
+    np.savez_compressed(os.path.join(HERE, name), n=np.array(ns, np.int32),
+                        T=np.float64(T), err=np.array(errs))
+    print("wrote", name)
+
+
 def main():
     shape = (5, 6, 7, 8)  # padded (lMax, kMax, jMax, iMax)
     counts = [0, 1, 2, 1, 0]
@@ -336,6 +365,7 @@ def main():
     init_case("init.npz", (5, 16, 18, 20), 13)
     main_case("main_flow.npz", (6, 18, 20, 22), 17, 3)
     vtk_case("vtk.npz", (6, 5, 4, 3), 19)
+    mcf_eoc_case("mcf_eoc.npz", (10, 20, 40), 0.0625)
 
 
 if __name__ == "__main__":
