@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "seg4d_engine.h"
+#include "seg4d_net.h"
 
 #define API extern "C" __attribute__((visibility("default")))
 
@@ -38,6 +39,7 @@ API double ballRadius = 0;
 namespace {
 
 s4d::Engine* g_eng = nullptr;
+s4d::Net* g_net = nullptr;   // learned warm start (survives re-allocation)
 int g_resident = 0;
 int g_verbose = 1;
 double g_rel = 0.0;
@@ -211,6 +213,7 @@ API bool _allocateMem_(void) {
     if (!g_eng) g_eng = new s4d::Engine();
     if (g_eng->init(iMax, jMax, kMax, lMax, true, false))
         return fail("_allocateMem_: device allocation failed");
+    g_eng->net_ = g_net;
     g_g_dirty = true;
     return true;
 }
@@ -536,9 +539,143 @@ API int seg4dLastRounds(void) { return g_last.rounds; }
 
 API int seg4dSetWarmStart(int mode) {
     if (!g_eng) return -1;
+    if (mode == 2 && !(g_net && g_net->ready())) {
+        s4d::set_error("seg4dSetWarmStart: no network (seg4dNetInit or seg4dNetLoad first)");
+        return -1;
+    }
     const int old = g_eng->warm_;
-    g_eng->warm_ = (mode == 1) ? 1 : 0;
+    g_eng->warm_ = (mode == 1 || mode == 2) ? mode : 0;
     return old;
+}
+
+// ---- learned warm start ------------------------------------------------------------
+static bool need_net(const char* fn) {
+    if (g_net && g_net->ready()) return true;
+    s4d::set_error(std::string(fn) + ": no network (seg4dNetInit or seg4dNetLoad first)");
+    return false;
+}
+
+static int net_fail(const char* fn) {
+    std::string m = std::string(fn) + ": " + (g_net ? g_net->err : std::string("no network"));
+    s4d::set_error(m);
+    std::fprintf(stderr, "seg4d: %s\n", m.c_str());
+    return -1;
+}
+
+API int seg4dNetInit(uint64_t seed) {
+    if (!g_net) g_net = new s4d::Net();
+    if (g_net->init(seed)) return net_fail("seg4dNetInit");
+    if (g_eng) g_eng->net_ = g_net;
+    return 0;
+}
+
+API int seg4dNetLoad(const char* path) {
+    if (!g_net) g_net = new s4d::Net();
+    if (g_net->load(path)) return net_fail("seg4dNetLoad");
+    if (g_eng) g_eng->net_ = g_net;
+    return 0;
+}
+
+API int seg4dNetSave(const char* path) {
+    if (!need_net("seg4dNetSave")) return -1;
+    return g_net->save(path) ? net_fail("seg4dNetSave") : 0;
+}
+
+API int seg4dNetRecord(int new_run) {
+    if (!need_net("seg4dNetRecord") || !need_engine("seg4dNetRecord")) return -1;
+    if (!g_resident && g_eng->upload(s4d::A_U, u)) {
+        fail("seg4dNetRecord: upload");
+        return -1;
+    }
+    return g_net->record(*g_eng, new_run != 0) ? net_fail("seg4dNetRecord") : 0;
+}
+
+API long seg4dNetPoolPairs(void) { return g_net ? g_net->pool_pairs() : 0; }
+
+API double seg4dNetTrain(int iters, int batch, double lr, uint64_t seed, int log_every) {
+    if (!need_net("seg4dNetTrain")) return -1.0;
+    double loss = 0.0;
+    if (g_net->train(iters, batch, (float)lr, seed, log_every, &loss)) {
+        net_fail("seg4dNetTrain");
+        return -1.0;
+    }
+    return loss;
+}
+
+API int seg4dNetParams(float* out, int n) {
+    if (!need_net("seg4dNetParams")) return -1;
+    if (!out || n <= 0) return g_net->nparam;
+    const int m = n < g_net->nparam ? n : g_net->nparam;
+    if (hipMemcpy(out, g_net->wm_, (size_t)m * 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return g_net->nparam;
+}
+
+API int seg4dNetSetParams(const float* in, int n, float s_d, float s_t) {
+    if (!need_net("seg4dNetSetParams")) return -1;
+    if (!in || n != g_net->nparam) {
+        s4d::set_error("seg4dNetSetParams: need exactly the network's parameter count");
+        return -1;
+    }
+    // write the FP32 master weights, then re-derive the BF16 and transposed
+    // copies the kernels read
+    if (hipMemcpy(g_net->wm_, in, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return -1;
+    g_net->s_d = s_d;
+    g_net->s_t = s_t;
+    return g_net->refresh() ? net_fail("seg4dNetSetParams") : 0;
+}
+
+API int seg4dNetGrad(int I, int J, int K, int L, int P, const uint16_t* feat_bf16,
+                     const float* target, const unsigned char* lmask, float* grad_out,
+                     double* loss) {
+    if (!need_net("seg4dNetGrad")) return -1;
+    const size_t pts = (size_t)I * J * K * L;
+    void* df = nullptr;
+    float* dt = nullptr;
+    unsigned char* dm = nullptr;
+    int rc = 0;
+    if (hipMalloc(&df, pts * 16) != hipSuccess || hipMalloc((void**)&dt, pts * 4) != hipSuccess ||
+        hipMalloc((void**)&dm, pts) != hipSuccess ||
+        hipMemcpy(df, feat_bf16, pts * 16, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dt, target, pts * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dm, lmask, pts, hipMemcpyHostToDevice) != hipSuccess) {
+        s4d::set_error("seg4dNetGrad: device buffers");
+        rc = -1;
+    }
+    if (!rc && g_net->grad_batch(I, J, K, L, P, df, dt, dm, loss)) rc = net_fail("seg4dNetGrad");
+    if (!rc && hipMemcpy(grad_out, g_net->grad_, (size_t)g_net->nparam * 4,
+                         hipMemcpyDeviceToHost) != hipSuccess)
+        rc = -1;
+    if (df) (void)hipFree(df);
+    if (dt) (void)hipFree(dt);
+    if (dm) (void)hipFree(dm);
+    return rc;
+}
+
+API int seg4dNetForward(int I, int J, int K, int L, int P, const uint16_t* feat_bf16,
+                        float* out) {
+    if (!need_net("seg4dNetForward")) return -1;
+    const size_t pts = (size_t)I * J * K * L;
+    void* df = nullptr;
+    float* dout = nullptr;
+    int rc = 0;
+    if (hipMalloc(&df, pts * 16) != hipSuccess || hipMalloc((void**)&dout, pts * 4) != hipSuccess ||
+        hipMemcpy(df, feat_bf16, pts * 16, hipMemcpyHostToDevice) != hipSuccess) {
+        s4d::set_error("seg4dNetForward: device buffers");
+        rc = -1;
+    }
+    if (!rc && g_net->forward_batch(I, J, K, L, P, df, dout)) rc = net_fail("seg4dNetForward");
+    if (!rc && hipMemcpy(out, dout, pts * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
+    if (df) (void)hipFree(df);
+    if (dout) (void)hipFree(dout);
+    return rc;
+}
+
+API double seg4dNetPredictMs(void) { return g_net ? g_net->predict_ms : 0.0; }
+API long seg4dNetPredicts(void) { return g_net ? g_net->predicts : 0; }
+API void seg4dNetScales(float* s_d, float* s_t) {
+    if (s_d) *s_d = g_net ? g_net->s_d : 0.f;
+    if (s_t) *s_t = g_net ? g_net->s_t : 0.f;
 }
 
 API int seg4dSetSweepVariant(int v) {

@@ -17,6 +17,7 @@ struct Geo;
 struct SorCtl;
 struct SweepMap;
 struct BallList;
+class Net;
 
 struct StepParams {
     double h = 1.0, tau = 0.5, eps2 = 1e-6;
@@ -143,9 +144,12 @@ public:
     int precision_ = 0;
     int max_rounds_ = 8;           // FP32-refined: correction rounds per solve
     // SOR start: 0 = u^n (the reference); 1 = 2 u^n - u^(n-1) when the
-    // previous call was a completed time step on the device (hist_ok_).
+    // previous call was a completed time step on the device (hist_ok_);
+    // 2 = the prediction of net_ (learned warm start).
     int warm_ = 0;
     bool hist_ok_ = false;
+    Net* net_ = nullptr;           // not owned
+    long g_version_ = 0;           // bumped whenever the g arrays change
     double inner_frac_ = 0.25;     // inner stop: R32 <= max(frac*tol, floor2*R)
     double inner_floor2_ = 1e-10;  // (1e-5)^2: what an FP32 solve can resolve
     Profile prof;

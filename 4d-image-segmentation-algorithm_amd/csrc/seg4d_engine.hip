@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "seg4d_kernels.h"
+#include "seg4d_net.h"
 
 namespace s4d {
 
@@ -165,6 +166,7 @@ static Geo make_geo(int I, int J, int K, int L, int H) {
 int Engine::upload(ArrayId a, const double* host) {
     if (!full_[a]) return check(hipErrorInvalidValue, "upload: array not allocated");
     if (a == A_U || a == A_PREV) hist_ok_ = false;
+    if (a >= A_G0 && a <= A_G7) g_version_++;
     if (check(hipMemcpyAsync(full_[a], host, (size_t)n_ * sizeof(double),
                              hipMemcpyHostToDevice, st_), "upload"))
         return -1;
@@ -195,6 +197,7 @@ int Engine::copy(ArrayId src, ArrayId dst) {
     if (!full_[src] || !full_[dst]) return check(hipErrorInvalidValue, "copy: array not allocated");
     if (src == dst) return 0;
     if (dst == A_U || dst == A_PREV) hist_ok_ = false;
+    if (dst >= A_G0 && dst <= A_G7) g_version_++;
     const long nb = (n_ + kBlock - 1) / kBlock;
     k_copy_f64<<<(unsigned)(nb < 65536 ? nb : 65536), kBlock, 0, st_>>>(full_[src], full_[dst], n_);
     return launch_check("k_copy_f64");
@@ -230,8 +233,17 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
         prevf = scratch(A_TMP2);
         if (!prevf) return -1;
     }
-    // the extrapolated start needs prev_full = u^(n-1) from a completed step
-    const int warm = (warm_ == 1 && hist_ok_ && write_prev) ? 1 : 0;
+    // the extrapolated and the learned start need prev_full = u^(n-1) from a
+    // completed step; without it the solve starts from u^n
+    int warm = (warm_ == 1 && hist_ok_ && write_prev) ? 1 : 0;
+    const double* startf = nullptr;
+    if (warm_ == 2 && hist_ok_ && write_prev && net_ && net_->ready()) {
+        double* st = scratch(A_TMP);
+        if (!st) return -1;
+        if (net_->predict(*this, st, true)) return check(hipErrorUnknown, net_->err.c_str());
+        startf = st;
+        warm = 2;
+    }
     PlaneMap PM;
     PM.gx = (int)cdiv((long)I_ * J_, kBlock);
     PM.gk = K_;
@@ -250,11 +262,11 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
         (void)hipEventRecord(e0, st_);
     }
     if (with_g) {
-        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_, warm);
-        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_, warm);
+        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, part_, warm);
+        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, part_, warm);
     } else {
-        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_, warm);
-        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, part_, warm);
+        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, part_, warm);
+        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, part_, warm);
     }
     if (launch_check("k_setup_step")) return -1;
     if (prof.on) {
@@ -824,6 +836,7 @@ int Engine::gfun(const double* a, const double* b, double h, double K, double th
                  double rho) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     const Scale S = make_scale(h);
+    g_version_++;
     Ptr8 g;
     for (int f = 0; f < 8; f++) {
         g.p[f] = full_[A_G0 + f];

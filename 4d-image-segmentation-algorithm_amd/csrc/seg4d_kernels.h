@@ -190,12 +190,13 @@ __device__ __forceinline__ bool map_plane_block(const PlaneMap& M, int b, int& b
 }
 
 // warm: 0 = SOR starts from u^n (the reference); 1 = it starts from the
-// linear extrapolation 2 u^n - u^(n-1) (prev_full holds u^(n-1) on entry).
+// linear extrapolation 2 u^n - u^(n-1) (prev_full holds u^(n-1) on entry);
+// 2 = from start_full (interior points; the learned start).
 template <bool WITH_G, bool POW2>
 __global__ __launch_bounds__(kBlock) void k_setup_step(
     Geo G, PlaneMap PM, Scale S, double eps2, double h4, const double* __restrict__ u,
     CPtr8 g, SplitBufs sb, double* __restrict__ prev_full,
-    double* __restrict__ partials, int warm) {
+    const double* __restrict__ start_full, double* __restrict__ partials, int warm) {
     int bx, k, l;
     if (!map_plane_block(PM, blockIdx.x, bx, k, l)) return;
     const int t = bx * kBlock + threadIdx.x;
@@ -208,6 +209,7 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
         const double uc = u[p];
         double start = uc;
         if (warm == 1 && is_interior(G, i, j, k, l)) start = 2.0 * uc - prev_full[p];
+        if (warm == 2 && is_interior(G, i, j, k, l)) start = start_full[p];
         prev_full[p] = uc;
         if (c == 0) {
             sb.ured[0][pp] = start;

@@ -1,0 +1,126 @@
+// seg4d_net.h -- the learned warm start of the SOR solve.
+//
+// A separable 4D convolutional network predicts the next time step's
+// solution from (u^n, u^n - u^(n-1), g) and the solver starts SOR from that
+// prediction instead of u^n; the stopping test is unchanged, so the plain
+// method stays the judge and a better start only shows up as fewer sweeps.
+//
+// Network (parameters FP32, kernels BF16 with FP32 accumulation):
+//   features x0 = [u^n, s_d (u^n - u^(n-1)), g_mean, 0 x 5]   (8 channels)
+//   4 x { 3x3x3 spatial conv -> 3-tap temporal conv + bias -> ReLU }, 16 ch
+//   1x1 projection -> c;  start = u^n + (u^n - u^(n-1)) + c / s_t
+// i.e. it learns the correction to the linear extrapolation.  Training
+// pairs come from the solver itself: record() snapshots the device u after
+// each step of a run; train() samples patches of those runs (core 64^3
+// points, all frames, a 4-point halo of real data) at levels n >= 1 and
+// minimises the mean squared error of c against s_t (u^(n+1) - 2 u^n +
+// u^(n-1)) with Adam.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace s4d {
+
+class Engine;
+
+namespace net {
+struct NGeo;
+struct Taps;
+}  // namespace net
+
+class Net {
+public:
+    Net() = default;
+    ~Net();
+    Net(const Net&) = delete;
+    Net& operator=(const Net&) = delete;
+
+    // He-initialised weights from `seed`; empties the training pool.
+    int init(uint64_t seed);
+    bool ready() const { return wm_ != nullptr; }
+    void release();
+
+    // Training pool: snapshot of the device u of `E` (FP32), appended to the
+    // current run; `new_run` starts a run (and snapshots E's mean g).
+    int record(Engine& E, bool new_run);
+    long pool_pairs() const;
+    // `iters` Adam steps on batches of `batch` patches; mean loss of the last
+    // 10% of steps in *loss.  Prints progress every `log_every` steps (0: no).
+    int train(int iters, int batch, float lr, uint64_t seed, int log_every, double* loss);
+    int save(const char* path);
+    int load(const char* path);
+    // Re-derives the BF16 and transposed copies after wm_ was written.
+    int refresh();
+
+    // FP64 start of the next solve of E into `start` (full layout, interior
+    // points); `history`: E's prev array holds u^(n-1) (else the increment is
+    // taken as zero).  The engine only asks with history.
+    int predict(Engine& E, double* start, bool history);
+
+    // One forward pass on a caller-built batch (tests): feat BF16
+    // [points][8], out FP32 [points].
+    int forward_batch(int I, int J, int K, int L, int P, const void* feat_dev, float* out_dev);
+    // Forward and backward on a caller-built batch (tests): leaves the
+    // gradient of the mean squared error over the lmask points in grad_
+    // (parameter layout) and the loss in *loss; no update.
+    int grad_batch(int I, int J, int K, int L, int P, const void* feat_dev,
+                   const float* target_dev, const unsigned char* lmask_dev, double* loss);
+
+    std::string err;
+    double predict_ms = 0.0;
+    long predicts = 0;
+    float s_d = 1.f, s_t = 1.f;  // feature / target scales
+
+    // parameter offsets (floats) into the parameter vector
+    static constexpr int kLayers = 4;
+    int off_ws[kLayers] = {}, off_wt[kLayers] = {}, off_b[kLayers] = {};
+    int off_wp = 0, off_bp = 0, nparam = 0;
+
+    // device parameters
+    float* wm_ = nullptr;   // FP32 master weights
+    __bf16* wb_ = nullptr;  // BF16 copy (same offsets)
+    __bf16* wtr_ = nullptr; // per-tap transposed copies for the backward-data convs
+    float* m1_ = nullptr;
+    float* m2_ = nullptr;
+    float* grad_ = nullptr;
+    long adam_t_ = 0;
+
+private:
+    int check(hipError_t e, const char* what);
+    int layout();
+    int ensure_acts(int I, int J, int K, int L, bool training);
+    int forward(const net::NGeo& G, hipStream_t st, bool training);
+    int backward(const net::NGeo& G, hipStream_t st, float inv_n);
+    int compute_scales(hipStream_t st);
+
+    struct Run {
+        int I = 0, J = 0, K = 0, L = 0;
+        __bf16* gmean = nullptr;       // BF16 [points]
+        std::vector<float*> snaps;     // FP32 [points] per time level
+    };
+    std::vector<Run> runs_;
+
+    // activations of the current geometry
+    int aI_ = 0, aJ_ = 0, aK_ = 0, aL_ = 0;
+    bool a_training_ = false;
+    __bf16* feat_ = nullptr;          // [pts][8]
+    __bf16* S_[kLayers] = {};         // spatial conv outputs [pts][16]
+    __bf16* Y_[kLayers] = {};         // layer outputs [pts][16]
+    __bf16* dA_ = nullptr;            // gradient buffers [pts][16]
+    __bf16* dB_ = nullptr;
+    float* out_ = nullptr;            // [pts]
+    float* target_ = nullptr;         // [pts]
+    unsigned char* lmask_ = nullptr;  // [pts]
+    float* partial_ = nullptr;        // reduction partials
+    long npartial_ = 0;
+    double* dscal_ = nullptr;
+    double* hscal_ = nullptr;
+    long g_version_ = -1;             // engine g generation of gmean_
+    __bf16* gmean_ = nullptr;         // inference: mean g of the engine [pts]
+    long gpts_ = 0;
+};
+
+}  // namespace s4d

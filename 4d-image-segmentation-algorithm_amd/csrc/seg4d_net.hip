@@ -1,0 +1,846 @@
+// seg4d_net.hip -- the learned warm start (seg4d_net.h): parameters, the
+// training pool, forward / backward passes on the kernels of
+// seg4d_net_kernels.h, Adam, and the prediction used by Engine::setup.
+#include "seg4d_net.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "seg4d_engine.h"
+#include "seg4d_net_kernels.h"
+
+namespace s4d {
+
+using net::bf16;
+using net::bf16x8;
+using net::kCh;
+using net::NGeo;
+using net::Taps;
+
+namespace {
+
+constexpr int kPatchCore = 64;  // loss points per axis of a training patch
+constexpr int kHalo = 4;        // receptive field of the four spatial convs
+constexpr int kPatch = kPatchCore + 2 * kHalo + 2;  // padded patch extent (74)
+constexpr int kSpatialTaps = 27, kTemporalTaps = 3;
+
+long cdivl(long a, long b) { return (a + b - 1) / b; }
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double uniform01(uint64_t& s) { return (splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+// 3x3x3 taps, t = 9 (dz+1) + 3 (dy+1) + (dx+1); rows q = 3 (dz+1) + (dy+1)
+Taps spatial_taps(const NGeo& G, int sign) {
+    Taps T;
+    std::memset(&T, 0, sizeof T);
+    T.n = kSpatialTaps;
+    T.nrow = 9;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++) {
+            const int q = 3 * (dz + 1) + (dy + 1);
+            T.rowoff[q] = sign * (G.I * (dy + G.J * dz));
+            for (int dx = -1; dx <= 1; dx++) {
+                const int t = 9 * (dz + 1) + 3 * (dy + 1) + (dx + 1);
+                T.off[t] = sign * (dx + G.I * (dy + G.J * dz));
+                T.trow[t] = (signed char)q;
+                T.tdx[t] = (signed char)(sign * dx);
+            }
+        }
+    return T;
+}
+
+// 3-tap temporal conv, t = dl + 1
+Taps temporal_taps(const NGeo& G, int sign) {
+    Taps T;
+    std::memset(&T, 0, sizeof T);
+    T.n = kTemporalTaps;
+    T.nrow = 3;
+    for (int dl = -1; dl <= 1; dl++) {
+        const int t = dl + 1;
+        T.off[t] = sign * (G.I * G.J * G.K * dl);
+        T.rowoff[t] = T.off[t];
+        T.trow[t] = (signed char)t;
+        T.tdx[t] = 0;
+    }
+    return T;
+}
+
+// ---- small kernels -------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gmean(NGeo G, net::bf16* __restrict__ out,
+                                               const double* g0, const double* g1,
+                                               const double* g2, const double* g3,
+                                               const double* g4, const double* g5,
+                                               const double* g6, const double* g7) {
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    const long nv = (long)G.I * G.J * G.K * G.L;
+    if (v >= nv) return;
+    const double s = ((((((g0[v] + g1[v]) + g2[v]) + g3[v]) + g4[v]) + g5[v]) + g6[v]) + g7[v];
+    out[v] = (bf16)(float)(s * 0.125);
+}
+
+__device__ __forceinline__ bool point_of(const NGeo& G, long v, int& i, int& j, int& k, int& l) {
+    i = (int)(v % G.I);
+    const long r1 = v / G.I;
+    j = (int)(r1 % G.J);
+    const long r2 = r1 / G.J;
+    k = (int)(r2 % G.K);
+    l = (int)(r2 / G.K);
+    return net::interior(G, i, j, k, l);
+}
+
+__global__ __launch_bounds__(256) void k_features(NGeo G, const double* __restrict__ u,
+                                                  const double* __restrict__ prev,
+                                                  const bf16* __restrict__ gm, float sd,
+                                                  bf16* __restrict__ feat) {
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    if (v >= (long)G.I * G.J * G.K * G.L) return;
+    int i, j, k, l;
+    if (!point_of(G, v, i, j, k, l)) return;
+    const double uc = u[v];
+    const double d = prev ? uc - prev[v] : 0.0;
+    bf16x8 f;
+    f[0] = (bf16)(float)uc;
+    f[1] = (bf16)((float)d * sd);
+    f[2] = gm[v];
+#pragma unroll
+    for (int q = 3; q < 8; q++) f[q] = (bf16)0.0f;
+    *reinterpret_cast<bf16x8*>(feat + v * 8) = f;
+}
+
+__global__ __launch_bounds__(256) void k_start(NGeo G, const double* __restrict__ u,
+                                               const double* __restrict__ prev,
+                                               const float* __restrict__ out, float inv_st,
+                                               double* __restrict__ start) {
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    if (v >= (long)G.I * G.J * G.K * G.L) return;
+    int i, j, k, l;
+    if (!point_of(G, v, i, j, k, l)) return;
+    const double uc = u[v];
+    const double d = prev ? uc - prev[v] : 0.0;
+    start[v] = uc + d + (double)(out[v] * inv_st);
+}
+
+__global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ a, long n,
+                                                float* __restrict__ b) {
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    if (v < n) b[v] = (float)a[v];
+}
+
+// One patch of a training batch: patch point (pi, pj, pk, pl) of batch slot
+// b reads run point (ax + pi, ay + pj, az + pk, pl).  Patch ghosts are zero;
+// loss points are the interior points at least kHalo + 1 from a patch
+// ghost that is not also a ghost of the run.
+__global__ __launch_bounds__(256) void k_gather_patch(
+    NGeo PG, int b, int RI, int RJ, int RK, const float* __restrict__ um1,
+    const float* __restrict__ u0, const float* __restrict__ up1,
+    const bf16* __restrict__ gm, int ax, int ay, int az, float sd, float st,
+    bf16* __restrict__ feat, float* __restrict__ target, unsigned char* __restrict__ lmask) {
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    const long per = (long)PG.I * PG.J * PG.K * PG.P;
+    if (q >= per) return;
+    const int pi = (int)(q % PG.I);
+    const long r1 = q / PG.I;
+    const int pj = (int)(r1 % PG.J);
+    const long r2 = r1 / PG.J;
+    const int pk = (int)(r2 % PG.K);
+    const int pl = (int)(r2 / PG.K);
+    const long v = (((long)(b * PG.P + pl) * PG.K + pk) * PG.J + pj) * PG.I + pi;
+    const bool ghost = pi == 0 || pi == PG.I - 1 || pj == 0 || pj == PG.J - 1 || pk == 0 ||
+                       pk == PG.K - 1 || pl == 0 || pl == PG.P - 1;
+    bf16x8 f;
+#pragma unroll
+    for (int c = 0; c < 8; c++) f[c] = (bf16)0.0f;
+    float tg = 0.f;
+    unsigned char lm = 0;
+    if (!ghost) {
+        const long rv = (((long)pl * RK + (az + pk)) * RJ + (ay + pj)) * RI + (ax + pi);
+        const float uc = u0[rv], um = um1[rv], up = up1[rv];
+        f[0] = (bf16)uc;
+        f[1] = (bf16)((uc - um) * sd);
+        f[2] = gm[rv];
+        tg = ((up - uc) - (uc - um)) * st;
+        auto ok = [](int p, int n, int a, int rn) {
+            return (p >= kHalo + 1 || a == 0) && (p <= n - kHalo - 2 || a + n == rn);
+        };
+        lm = ok(pi, PG.I, ax, RI) && ok(pj, PG.J, ay, RJ) && ok(pk, PG.K, az, RK) ? 1 : 0;
+    }
+    *reinterpret_cast<bf16x8*>(feat + v * 8) = f;
+    target[v] = tg;
+    lmask[v] = lm;
+}
+
+// Per-block partial sums (FP64) of d^2 and c^2 over the interior of a run
+// pair: d = u0 - um1, c = (up1 - u0) - d.
+__global__ __launch_bounds__(256) void k_moments(NGeo G, const float* __restrict__ um1,
+                                                 const float* __restrict__ u0,
+                                                 const float* __restrict__ up1,
+                                                 double* __restrict__ partial) {
+    __shared__ double s1[256], s2[256];
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    double a = 0, c = 0;
+    int i, j, k, l;
+    if (v < (long)G.I * G.J * G.K * G.L && point_of(G, v, i, j, k, l)) {
+        const double d = (double)u0[v] - um1[v];
+        const double e = ((double)up1[v] - u0[v]) - d;
+        a = d * d;
+        c = e * e;
+    }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s1[threadIdx.x] += s1[threadIdx.x + s];
+            s2[threadIdx.x] += s2[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = s1[0];
+        partial[2 * blockIdx.x + 1] = s2[0];
+    }
+}
+
+// out[e] = sum_b partial[b * stride + col(e)], one block per output, fixed
+// order (strided lane sums, then a tree).  col(e) maps the output index to
+// the partial layout: for a weight gradient of `cin` input channels the
+// partial rows are 16 per tap.
+__global__ __launch_bounds__(256) void k_reduce_cols(const float* __restrict__ partial, int nb,
+                                                     long stride, int cin, int n, float scale,
+                                                     float* __restrict__ out) {
+    __shared__ float sm[256];
+    const int e = blockIdx.x;
+    if (e >= n) return;
+    int col = e;
+    if (cin > 0) {
+        const int co = e % kCh, r = e / kCh, ci = r % cin, t = r / cin;
+        col = (t * 16 + ci) * kCh + co;
+    }
+    float s = 0.f;
+    for (int b = threadIdx.x; b < nb; b += 256) s += partial[(long)b * stride + col];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[e] = scale * sm[0];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_d(const double* __restrict__ partial, int nb,
+                                                  int ncol, double* __restrict__ out) {
+    __shared__ double sm[256];
+    for (int c = 0; c < ncol; c++) {
+        double s = 0;
+        for (int b = threadIdx.x; b < nb; b += 256) s += partial[(long)b * ncol + c];
+        sm[threadIdx.x] = s;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[c] = sm[0];
+        __syncthreads();
+    }
+}
+
+// wt[t][co][ci] = w[t][ci][co] for 16 x 16 taps
+__global__ __launch_bounds__(256) void k_transpose_taps(const bf16* __restrict__ w, int ntaps,
+                                                        bf16* __restrict__ wt) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= ntaps * 256) return;
+    const int t = e / 256, ci = (e / 16) % 16, co = e % 16;
+    wt[(t * 16 + co) * 16 + ci] = w[e];
+}
+
+__global__ __launch_bounds__(256) void k_f32_to_bf16(const float* __restrict__ a, int n,
+                                                     bf16* __restrict__ b) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e < n) b[e] = (bf16)a[e];
+}
+
+}  // namespace
+
+// ---- Net -------------------------------------------------------------------------
+Net::~Net() { release(); }
+
+int Net::check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return 0;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    return -1;
+}
+
+int Net::layout() {
+    int o = 0;
+    for (int L = 0; L < kLayers; L++) {
+        const int cin = L == 0 ? net::kCin0 : kCh;
+        off_ws[L] = o;
+        o += net::kMaxTaps * cin * kCh;
+        off_wt[L] = o;
+        o += 4 * kCh * kCh;
+        off_b[L] = o;
+        o += kCh;
+    }
+    off_wp = o;
+    o += kCh;
+    off_bp = o;
+    o += 1;
+    nparam = o;
+    return o;
+}
+
+void Net::release() {
+    auto fr = [](auto*& p) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    };
+    fr(wm_);
+    fr(wb_);
+    fr(wtr_);
+    fr(m1_);
+    fr(m2_);
+    fr(grad_);
+    for (Run& r : runs_) {
+        fr(r.gmean);
+        for (float*& s : r.snaps) fr(s);
+    }
+    runs_.clear();
+    fr(feat_);
+    for (int L = 0; L < kLayers; L++) {
+        fr(S_[L]);
+        fr(Y_[L]);
+    }
+    fr(dA_);
+    fr(dB_);
+    fr(out_);
+    fr(target_);
+    fr(lmask_);
+    fr(partial_);
+    fr(dscal_);
+    fr(gmean_);
+    if (hscal_) (void)hipHostFree(hscal_);
+    hscal_ = nullptr;
+    aI_ = aJ_ = aK_ = aL_ = 0;
+    g_version_ = -1;
+    gpts_ = 0;
+    adam_t_ = 0;
+}
+
+int Net::init(uint64_t seed) {
+    release();
+    layout();
+    const size_t nb = (size_t)nparam;
+    if (check(hipMalloc((void**)&wm_, nb * 4), "hipMalloc net") ||
+        check(hipMalloc((void**)&wb_, nb * 2), "hipMalloc net") ||
+        check(hipMalloc((void**)&wtr_, nb * 2), "hipMalloc net") ||
+        check(hipMalloc((void**)&m1_, nb * 4), "hipMalloc net") ||
+        check(hipMalloc((void**)&m2_, nb * 4), "hipMalloc net") ||
+        check(hipMalloc((void**)&grad_, nb * 4), "hipMalloc net") ||
+        check(hipMalloc((void**)&dscal_, 64 * sizeof(double)), "hipMalloc net") ||
+        check(hipHostMalloc((void**)&hscal_, 64 * sizeof(double), hipHostMallocDefault),
+              "hipHostMalloc net"))
+        return -1;
+    // He initialisation: N(0, 2 / fan_in) as a sum of uniforms (Irwin-Hall 12)
+    std::vector<float> w(nb, 0.f);
+    uint64_t s = seed;
+    auto normal = [&]() {
+        double a = 0;
+        for (int q = 0; q < 12; q++) a += uniform01(s);
+        return a - 6.0;
+    };
+    for (int L = 0; L < kLayers; L++) {
+        const int cin = L == 0 ? net::kCin0 : kCh;
+        const int cin_used = L == 0 ? 3 : kCh;
+        const double ss = std::sqrt(2.0 / (kSpatialTaps * cin_used));
+        for (int t = 0; t < kSpatialTaps; t++)
+            for (int ci = 0; ci < cin_used; ci++)
+                for (int co = 0; co < kCh; co++)
+                    w[off_ws[L] + (t * cin + ci) * kCh + co] = (float)(ss * normal());
+        const double st = std::sqrt(2.0 / (kTemporalTaps * kCh));
+        for (int t = 0; t < kTemporalTaps; t++)
+            for (int ci = 0; ci < kCh; ci++)
+                for (int co = 0; co < kCh; co++)
+                    w[off_wt[L] + (t * kCh + ci) * kCh + co] = (float)(st * normal());
+    }
+    // the projection starts at zero: the initial prediction is the linear
+    // extrapolation itself
+    if (check(hipMemcpy(wm_, w.data(), nb * 4, hipMemcpyHostToDevice), "net weights") ||
+        check(hipMemset(m1_, 0, nb * 4), "net memset") ||
+        check(hipMemset(m2_, 0, nb * 4), "net memset") ||
+        check(hipMemset(grad_, 0, nb * 4), "net memset") ||
+        check(hipMemset(wb_, 0, nb * 2), "net memset") ||
+        check(hipMemset(wtr_, 0, nb * 2), "net memset"))
+        return -1;
+    s_d = s_t = 1.f;
+    adam_t_ = 0;
+    return refresh();
+}
+
+int Net::ensure_acts(int I, int J, int K, int L, bool training) {
+    if (I == aI_ && J == aJ_ && K == aK_ && L == aL_ && training == a_training_ && feat_)
+        return 0;
+    auto fr = [](auto*& p) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    };
+    fr(feat_);
+    for (int q = 0; q < kLayers; q++) {
+        fr(S_[q]);
+        fr(Y_[q]);
+    }
+    fr(dA_);
+    fr(dB_);
+    fr(out_);
+    fr(target_);
+    fr(lmask_);
+    fr(partial_);
+    aI_ = aJ_ = aK_ = aL_ = 0;
+    const size_t pts = (size_t)I * J * K * L;
+    auto al = [&](auto** p, size_t bytes) {
+        if (check(hipMalloc((void**)p, bytes), "hipMalloc net activations")) return -1;
+        return check(hipMemset(*p, 0, bytes), "net memset");
+    };
+    if (al(&feat_, pts * 8 * 2)) return -1;
+    const int nS = training ? kLayers : 1, nY = training ? kLayers : 2;
+    for (int q = 0; q < nS; q++)
+        if (al(&S_[q], pts * kCh * 2)) return -1;
+    for (int q = 0; q < nY; q++)
+        if (al(&Y_[q], pts * kCh * 2)) return -1;
+    if (al(&out_, pts * 4)) return -1;
+    if (training) {
+        if (al(&dA_, pts * kCh * 2) || al(&dB_, pts * kCh * 2) || al(&target_, pts * 4) ||
+            al(&lmask_, pts))
+            return -1;
+    }
+    // partials: weight gradients (2048 blocks x 28 x 16 x 16) or loss rows
+    const long nb_loss = cdivl((long)pts, 256);
+    npartial_ = std::max<long>(2048L * net::kMaxTaps * 16 * kCh, nb_loss * 18);
+    if (al(&partial_, (size_t)npartial_ * 4)) return -1;
+    aI_ = I;
+    aJ_ = J;
+    aK_ = K;
+    aL_ = L;
+    a_training_ = training;
+    return 0;
+}
+
+static int conv_waves(long nrow, int* rows_per_wave) {
+    const long waves = std::min<long>(nrow, 256L * 16);
+    *rows_per_wave = (int)cdivl(nrow, waves);
+    return (int)cdivl(nrow, *rows_per_wave);
+}
+
+int Net::forward(const NGeo& G, hipStream_t st, bool training) {
+    const long nrow = (long)(G.J - 2) * (G.K - 2) * ((G.L / G.P) * (G.P - 2));
+    int rpw = 1;
+    const int waves = conv_waves(nrow, &rpw);
+    const dim3 grid((unsigned)cdivl(waves, 4));
+    const Taps Ts = spatial_taps(G, 1), Tt = temporal_taps(G, 1);
+    const bf16* X = feat_;
+    for (int L = 0; L < kLayers; L++) {
+        bf16* S = training ? S_[L] : S_[0];
+        bf16* Y = training ? Y_[L] : Y_[L & 1];
+        if (L == 0)
+            net::k_conv<8, kSpatialTaps, net::EPI_NONE><<<grid, 256, 0, st>>>(G, Ts, X, wb_ + off_ws[L],
+                                                                nullptr, nullptr, S, rpw);
+        else
+            net::k_conv<16, kSpatialTaps, net::EPI_NONE><<<grid, 256, 0, st>>>(G, Ts, X, wb_ + off_ws[L],
+                                                                 nullptr, nullptr, S, rpw);
+        net::k_conv<16, kTemporalTaps, net::EPI_BIAS_RELU><<<grid, 256, 0, st>>>(
+            G, Tt, S, wb_ + off_wt[L], wm_ + off_b[L], nullptr, Y, rpw);
+        X = Y;
+    }
+    const long nv = (long)G.I * G.J * G.K * G.L;
+    net::k_project<<<(unsigned)cdivl(nv, 256), 256, 0, st>>>(G, X, wm_ + off_wp,
+                                                             wm_ + off_bp, out_);
+    return check(hipGetLastError(), "net forward");
+}
+
+int Net::backward(const NGeo& G, hipStream_t st, float inv_n) {
+    const long nv = (long)G.I * G.J * G.K * G.L;
+    const long nrow = (long)(G.J - 2) * (G.K - 2) * ((G.L / G.P) * (G.P - 2));
+    int rpw = 1;
+    const int waves = conv_waves(nrow, &rpw);
+    const dim3 grid((unsigned)cdivl(waves, 4));
+    const Taps Ts = spatial_taps(G, 1), Tt = temporal_taps(G, 1);
+    const Taps Tsb = spatial_taps(G, -1), Ttb = temporal_taps(G, -1);
+    const int nbl = (int)cdivl(nv, 256);
+    // loss and projection
+    net::k_loss_project_bwd<<<nbl, 256, 0, st>>>(G, out_, target_, lmask_, Y_[kLayers - 1],
+                                                 wm_ + off_wp, 2.f * inv_n, dA_, partial_);
+    // partial columns: 0 = sum d^2, 1..16 = sum d a_c, 17 = sum d; the loss
+    // is sum d^2 / N, so the projection's gradients are (2/N) times the sums
+    k_reduce_cols<<<kCh, 256, 0, st>>>(partial_ + 1, nbl, 18, 0, kCh, 2.f * inv_n, grad_ + off_wp);
+    k_reduce_cols<<<1, 256, 0, st>>>(partial_ + 17, nbl, 18, 0, 1, 2.f * inv_n, grad_ + off_bp);
+    k_reduce_cols<<<1, 256, 0, st>>>(partial_, nbl, 18, 0, 1, 1.f, (float*)dscal_);
+    const int nblk = (int)std::min<long>(2048, nrow);
+    const int rpb = (int)cdivl(nrow, nblk);
+    const long dw_stride = (long)net::kMaxTaps * 16 * kCh;
+    for (int L = kLayers - 1; L >= 0; L--) {
+        // dA_ holds the gradient at the pre-activation of layer L
+        net::k_bias_grad<<<nblk, 256, 0, st>>>(G, dA_, partial_, rpb);
+        k_reduce_cols<<<kCh, 256, 0, st>>>(partial_, nblk, kCh, 0, kCh, 1.f, grad_ + off_b[L]);
+        net::k_conv_dw<16, kTemporalTaps><<<nblk, 64, 0, st>>>(G, Tt, S_[L], dA_, partial_, rpb);
+        k_reduce_cols<<<kTemporalTaps * 256, 256, 0, st>>>(partial_, nblk, dw_stride, 16,
+                                                          kTemporalTaps * 256, 1.f,
+                                                          grad_ + off_wt[L]);
+        net::k_conv<16, kTemporalTaps, net::EPI_NONE><<<grid, 256, 0, st>>>(G, Ttb, dA_, wtr_ + off_wt[L],
+                                                             nullptr, nullptr, dB_, rpw);
+        if (L == 0) {
+            net::k_conv_dw<8, kSpatialTaps><<<nblk, 64, 0, st>>>(G, Ts, feat_, dB_, partial_, rpb);
+            k_reduce_cols<<<kSpatialTaps * 8 * kCh, 256, 0, st>>>(
+                partial_, nblk, dw_stride, 8, kSpatialTaps * 8 * kCh, 1.f, grad_ + off_ws[L]);
+        } else {
+            net::k_conv_dw<16, kSpatialTaps><<<nblk, 64, 0, st>>>(G, Ts, Y_[L - 1], dB_, partial_, rpb);
+            k_reduce_cols<<<kSpatialTaps * 256, 256, 0, st>>>(
+                partial_, nblk, dw_stride, 16, kSpatialTaps * 256, 1.f, grad_ + off_ws[L]);
+            net::k_conv<16, kSpatialTaps, net::EPI_MASK><<<grid, 256, 0, st>>>(
+                G, Tsb, dB_, wtr_ + off_ws[L], nullptr, Y_[L - 1], dA_, rpw);
+        }
+    }
+    return check(hipGetLastError(), "net backward");
+}
+
+int Net::record(Engine& E, bool new_run) {
+    if (!ready()) {
+        err = "net: not initialised";
+        return -1;
+    }
+    const long pts = (long)E.I_ * E.J_ * E.K_ * E.L_;
+    for (int f = 0; f < 8; f++)
+        if (!E.full(ArrayId(A_G0 + f))) {
+            err = "net: the volume has no edge-indicator arrays";
+            return -1;
+        }
+    if (new_run || runs_.empty()) {
+        Run r;
+        r.I = E.I_;
+        r.J = E.J_;
+        r.K = E.K_;
+        r.L = E.L_;
+        if (check(hipMalloc((void**)&r.gmean, pts * 2), "hipMalloc pool")) return -1;
+        const NGeo G{E.I_, E.J_, E.K_, E.L_, E.L_};
+        const double* g[8];
+        for (int f = 0; f < 8; f++) g[f] = E.full(ArrayId(A_G0 + f));
+        k_gmean<<<(unsigned)cdivl(pts, 256), 256, 0, E.st_>>>(G, r.gmean, g[0], g[1], g[2], g[3],
+                                                              g[4], g[5], g[6], g[7]);
+        runs_.push_back(r);
+    }
+    Run& r = runs_.back();
+    if (r.I != E.I_ || r.J != E.J_ || r.K != E.K_ || r.L != E.L_) {
+        err = "net: record with other dimensions than the run";
+        return -1;
+    }
+    float* s = nullptr;
+    if (check(hipMalloc((void**)&s, pts * 4), "hipMalloc pool")) return -1;
+    k_to_f32<<<(unsigned)cdivl(pts, 256), 256, 0, E.st_>>>(E.full(A_U), pts, s);
+    r.snaps.push_back(s);
+    return check(hipStreamSynchronize(E.st_), "net record");
+}
+
+// Training pairs: levels n >= 1 with a successor (n = 0 has no increment
+// u^n - u^(n-1), and the learned start is not used there).
+long Net::pool_pairs() const {
+    long n = 0;
+    for (const Run& r : runs_) n += r.snaps.size() > 2 ? (long)r.snaps.size() - 2 : 0;
+    return n;
+}
+
+int Net::compute_scales(hipStream_t st) {
+    double sd = 0, sc = 0;
+    long npts = 0;
+    std::vector<double> host(2);
+    for (const Run& r : runs_) {
+        const NGeo G{r.I, r.J, r.K, r.L, r.L};
+        const long pts = (long)r.I * r.J * r.K * r.L;
+        const int nb = (int)cdivl(pts, 256);
+        double* part = nullptr;
+        if (check(hipMalloc((void**)&part, (size_t)nb * 2 * sizeof(double)), "hipMalloc")) return -1;
+        for (size_t n = 1; n + 1 < r.snaps.size(); n++) {
+            k_moments<<<nb, 256, 0, st>>>(G, r.snaps[n - 1], r.snaps[n], r.snaps[n + 1], part);
+            k_reduce_d<<<1, 256, 0, st>>>(part, nb, 2, dscal_);
+            if (check(hipMemcpyAsync(hscal_, dscal_, 2 * sizeof(double), hipMemcpyDeviceToHost, st),
+                      "moments") ||
+                check(hipStreamSynchronize(st), "moments")) {
+                (void)hipFree(part);
+                return -1;
+            }
+            sd += hscal_[0];
+            sc += hscal_[1];
+            npts += (long)(r.I - 2) * (r.J - 2) * (r.K - 2) * (r.L - 2);
+        }
+        (void)hipFree(part);
+    }
+    if (npts == 0 || !(sd > 0) || !(sc > 0)) {
+        err = "net: training pool needs runs of at least 3 time levels with motion";
+        return -1;
+    }
+    s_d = (float)(1.0 / std::sqrt(sd / npts));
+    s_t = (float)(1.0 / std::sqrt(sc / npts));
+    return 0;
+}
+
+int Net::train(int iters, int batch, float lr, uint64_t seed, int log_every, double* loss) {
+    if (!ready()) {
+        err = "net: not initialised";
+        return -1;
+    }
+    if (runs_.empty() || pool_pairs() == 0) {
+        err = "net: empty training pool";
+        return -1;
+    }
+    const int P = runs_[0].L;
+    for (const Run& r : runs_)
+        if (r.L != P || r.I < kPatch || r.J < kPatch || r.K < kPatch) {
+            err = "net: runs need equal frame counts and at least 74 padded points per axis";
+            return -1;
+        }
+    if (batch < 1 || iters < 1) {
+        err = "net: batch and iterations must be positive";
+        return -1;
+    }
+    hipStream_t st = nullptr;
+    if (check(hipStreamCreate(&st), "stream")) return -1;
+    int rc = 0;
+    if (adam_t_ == 0) rc = compute_scales(st);
+    const NGeo G{kPatch, kPatch, kPatch, batch * P, P};
+    if (!rc) rc = ensure_acts(G.I, G.J, G.K, G.L, true);
+    // (run, level) pairs with a predecessor and a successor
+    std::vector<std::pair<int, int>> pairs;
+    for (int q = 0; q < (int)runs_.size(); q++)
+        for (int n = 1; n + 1 < (int)runs_[q].snaps.size(); n++) pairs.push_back({q, n});
+    uint64_t s = seed;
+    double tail = 0;
+    int ntail = 0;
+    const int tail_from = iters - std::max(1, iters / 10);
+    const long per = (long)G.I * G.J * G.K * P;
+    for (int it = 0; it < iters && !rc; it++) {
+        long nloss = 0;
+        for (int b = 0; b < batch; b++) {
+            const auto pr = pairs[(size_t)(uniform01(s) * pairs.size())];
+            const Run& r = runs_[pr.first];
+            const int n = pr.second;
+            const int ax = (int)(uniform01(s) * (r.I - kPatch + 1));
+            const int ay = (int)(uniform01(s) * (r.J - kPatch + 1));
+            const int az = (int)(uniform01(s) * (r.K - kPatch + 1));
+            const float* um1 = r.snaps[n - 1];
+            k_gather_patch<<<(unsigned)cdivl(per, 256), 256, 0, st>>>(
+                G, b, r.I, r.J, r.K, um1, r.snaps[n], r.snaps[n + 1], r.gmean, ax, ay, az, s_d, s_t,
+                feat_, target_, lmask_);
+            auto cnt = [&](int a, int rn) {
+                const int lo = a == 0 ? 1 : kHalo + 1;
+                const int hi = a + kPatch == rn ? kPatch - 2 : kPatch - kHalo - 2;
+                return (long)(hi - lo + 1);
+            };
+            nloss += cnt(ax, r.I) * cnt(ay, r.J) * cnt(az, r.K) * (P - 2);
+        }
+        const float inv_n = 1.f / (float)nloss;
+        rc = forward(G, st, true);
+        if (!rc) rc = backward(G, st, inv_n);
+        if (rc) break;
+        adam_t_++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)adam_t_);
+        const float c2 = 1.f - std::pow(b2, (float)adam_t_);
+        net::k_adam<<<(unsigned)cdivl(nparam, 256), 256, 0, st>>>(wm_, m1_, m2_, grad_, nparam, lr,
+                                                                  b1, b2, 1e-8f, c1, c2, wb_);
+        for (int L = 0; L < kLayers; L++) {
+            if (L > 0)
+                k_transpose_taps<<<(kSpatialTaps * 256 + 255) / 256, 256, 0, st>>>(
+                    wb_ + off_ws[L], kSpatialTaps, wtr_ + off_ws[L]);
+            k_transpose_taps<<<(kTemporalTaps * 256 + 255) / 256, 256, 0, st>>>(
+                wb_ + off_wt[L], kTemporalTaps, wtr_ + off_wt[L]);
+        }
+        rc = check(hipMemcpyAsync(hscal_, dscal_, sizeof(float), hipMemcpyDeviceToHost, st), "loss");
+        if (!rc) rc = check(hipStreamSynchronize(st), "train step");
+        if (rc) break;
+        const double lval = (double)(*(float*)hscal_) * inv_n;
+        if (it >= tail_from) {
+            tail += lval;
+            ntail++;
+        }
+        if (log_every > 0 && (it % log_every == 0 || it == iters - 1)) {
+            std::printf("[net] step %d loss %.5f\n", it, lval);
+            std::fflush(stdout);
+        }
+    }
+    if (!rc) rc = check(hipStreamSynchronize(st), "train");
+    (void)hipStreamDestroy(st);
+    if (loss) *loss = ntail ? tail / ntail : 0.0;
+    return rc;
+}
+
+int Net::forward_batch(int I, int J, int K, int L, int P, const void* feat_dev, float* out_dev) {
+    if (!ready()) {
+        err = "net: not initialised";
+        return -1;
+    }
+    const NGeo G{I, J, K, L, P};
+    if (ensure_acts(I, J, K, L, false)) return -1;
+    const size_t pts = (size_t)I * J * K * L;
+    if (check(hipMemcpy(feat_, feat_dev, pts * 16, hipMemcpyDeviceToDevice), "feat copy") ||
+        forward(G, nullptr, false) ||
+        check(hipMemcpy(out_dev, out_, pts * 4, hipMemcpyDeviceToDevice), "out copy"))
+        return -1;
+    return check(hipDeviceSynchronize(), "forward_batch");
+}
+
+int Net::grad_batch(int I, int J, int K, int L, int P, const void* feat_dev,
+                    const float* target_dev, const unsigned char* lmask_dev, double* loss) {
+    if (!ready()) {
+        err = "net: not initialised";
+        return -1;
+    }
+    const NGeo G{I, J, K, L, P};
+    if (ensure_acts(I, J, K, L, true)) return -1;
+    const size_t pts = (size_t)I * J * K * L;
+    std::vector<unsigned char> lm(pts);
+    if (check(hipMemcpy(lm.data(), lmask_dev, pts, hipMemcpyDeviceToHost), "lmask copy"))
+        return -1;
+    long n = 0;
+    for (unsigned char c : lm) n += c ? 1 : 0;
+    if (n == 0) {
+        err = "net: no loss points";
+        return -1;
+    }
+    const float inv_n = 1.f / (float)n;
+    if (check(hipMemcpy(feat_, feat_dev, pts * 16, hipMemcpyDeviceToDevice), "feat copy") ||
+        check(hipMemcpy(target_, target_dev, pts * 4, hipMemcpyDeviceToDevice), "target copy") ||
+        check(hipMemcpy(lmask_, lmask_dev, pts, hipMemcpyDeviceToDevice), "lmask copy") ||
+        check(hipMemset(grad_, 0, (size_t)nparam * 4), "grad clear") ||
+        forward(G, nullptr, true) || backward(G, nullptr, inv_n) ||
+        check(hipDeviceSynchronize(), "grad_batch"))
+        return -1;
+    float lsum = 0.f;
+    if (check(hipMemcpy(&lsum, dscal_, 4, hipMemcpyDeviceToHost), "loss")) return -1;
+    if (loss) *loss = (double)lsum * inv_n;
+    return 0;
+}
+
+int Net::predict(Engine& E, double* start, bool history) {
+    if (!ready()) {
+        err = "net: not initialised";
+        return -1;
+    }
+    const NGeo G{E.I_, E.J_, E.K_, E.L_, E.L_};
+    const long pts = (long)E.I_ * E.J_ * E.K_ * E.L_;
+    for (int f = 0; f < 8; f++)
+        if (!E.full(ArrayId(A_G0 + f))) {
+            err = "net: the volume has no edge-indicator arrays";
+            return -1;
+        }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipEventRecord(e0, E.st_);
+    if (ensure_acts(E.I_, E.J_, E.K_, E.L_, false)) return -1;
+    if (g_version_ != E.g_version_ || gpts_ != pts) {
+        if (gmean_) (void)hipFree(gmean_);
+        gmean_ = nullptr;
+        if (check(hipMalloc((void**)&gmean_, pts * 2), "hipMalloc gmean")) return -1;
+        gpts_ = pts;
+        const double* g[8];
+        for (int f = 0; f < 8; f++) g[f] = E.full(ArrayId(A_G0 + f));
+        k_gmean<<<(unsigned)cdivl(pts, 256), 256, 0, E.st_>>>(G, gmean_, g[0], g[1], g[2], g[3],
+                                                              g[4], g[5], g[6], g[7]);
+        g_version_ = E.g_version_;
+    }
+    const double* prev = history ? E.full(A_PREV) : nullptr;
+    k_features<<<(unsigned)cdivl(pts, 256), 256, 0, E.st_>>>(G, E.full(A_U), prev, gmean_, s_d,
+                                                             feat_);
+    if (forward(G, E.st_, false)) return -1;
+    k_start<<<(unsigned)cdivl(pts, 256), 256, 0, E.st_>>>(G, E.full(A_U), prev, out_,
+                                                          1.f / s_t, start);
+    (void)hipEventRecord(e1, E.st_);
+    if (check(hipEventSynchronize(e1), "net predict")) return -1;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    predict_ms += ms;
+    predicts++;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return check(hipGetLastError(), "net predict");
+}
+
+int Net::save(const char* path) {
+    if (!ready()) {
+        err = "net: not initialised";
+        return -1;
+    }
+    std::vector<float> w((size_t)nparam);
+    if (check(hipMemcpy(w.data(), wm_, w.size() * 4, hipMemcpyDeviceToHost), "net save")) return -1;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) {
+        err = std::string("net: cannot create ") + path;
+        return -1;
+    }
+    const char magic[8] = {'S', '4', 'D', 'N', 'E', 'T', '0', '1'};
+    const int32_t np = nparam;
+    const int64_t t = adam_t_;
+    bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(&np, 4, 1, f) == 1 &&
+              std::fwrite(&s_d, 4, 1, f) == 1 && std::fwrite(&s_t, 4, 1, f) == 1 &&
+              std::fwrite(&t, 8, 1, f) == 1 && std::fwrite(w.data(), 4, w.size(), f) == w.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) err = "net: write failed";
+    return ok ? 0 : -1;
+}
+
+int Net::load(const char* path) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) {
+        err = std::string("net: cannot open ") + path;
+        return -1;
+    }
+    char magic[8];
+    int32_t np = 0;
+    float sd = 1, stv = 1;
+    int64_t t = 0;
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "S4DNET01", 8) == 0 &&
+              std::fread(&np, 4, 1, f) == 1 && std::fread(&sd, 4, 1, f) == 1 &&
+              std::fread(&stv, 4, 1, f) == 1 && std::fread(&t, 8, 1, f) == 1;
+    if (!ok || init(0)) {
+        std::fclose(f);
+        if (ok) return -1;
+        err = "net: not a weights file";
+        return -1;
+    }
+    if (np != nparam) {
+        std::fclose(f);
+        err = "net: weights file of another network shape";
+        return -1;
+    }
+    std::vector<float> w((size_t)nparam);
+    ok = std::fread(w.data(), 4, w.size(), f) == w.size();
+    std::fclose(f);
+    if (!ok) {
+        err = "net: truncated weights file";
+        return -1;
+    }
+    if (check(hipMemcpy(wm_, w.data(), w.size() * 4, hipMemcpyHostToDevice), "net load")) return -1;
+    s_d = sd;
+    s_t = stv;
+    adam_t_ = t;
+    return refresh();
+}
+
+int Net::refresh() {
+    k_f32_to_bf16<<<(unsigned)cdivl(nparam, 256), 256>>>(wm_, nparam, wb_);
+    for (int L = 0; L < kLayers; L++) {
+        if (L > 0)
+            k_transpose_taps<<<(kSpatialTaps * 256 + 255) / 256, 256>>>(wb_ + off_ws[L],
+                                                                     kSpatialTaps,
+                                                                     wtr_ + off_ws[L]);
+        k_transpose_taps<<<(kTemporalTaps * 256 + 255) / 256, 256>>>(wb_ + off_wt[L],
+                                                                  kTemporalTaps,
+                                                                  wtr_ + off_wt[L]);
+    }
+    return check(hipDeviceSynchronize(), "net refresh");
+}
+
+}  // namespace s4d

@@ -1,0 +1,377 @@
+// seg4d_net_kernels.h -- HIP kernels of the learned warm start: a separable
+// 4D convolutional network (3x3x3 spatial conv, then 3-tap temporal conv,
+// bias, ReLU; four such layers of 16 channels and a 1x1 projection) that
+// predicts the SOR start of a time step from (u^n, u^n - u^(n-1), g).
+//
+// Activations live on the solver's padded grid (I x J x K x L points, x
+// fastest), channel-last in BF16: element (v, c) at v * C + c with C = 8 for
+// the input features and 16 otherwise.  Ghost points hold zeros and are
+// never written, so they are the convolutions' zero padding.  A training
+// batch stacks patches along the frame axis: frames with l % P in {0, P-1}
+// are ghosts (P = L for one volume).
+//
+// The convolutions are implicit GEMMs on v_mfma_f32_16x16x32_bf16 (gfx950):
+// one wave computes a tile of 16 consecutive x points by 16 output channels
+// with FP32 accumulation.  For that instruction, lane ln holds
+//   A[m = ln % 16][k = 8 (ln / 16) .. + 7]   (8 consecutive k, BF16)
+//   B[k = 8 (ln / 16) .. + 7][n = ln % 16]
+//   C[m = 4 (ln / 16) + r][n = ln % 16], r = 0..3  (FP32)
+// With channel-last data, the 8 k of an A operand are 8 channels of one
+// point (16 contiguous bytes).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace s4d {
+namespace net {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kCh = 16;       // hidden channels
+constexpr int kCin0 = 8;      // input feature channels (3 used)
+constexpr int kMaxTaps = 28;  // 27 spatial taps padded to an even count
+constexpr int kWave = 64;
+
+struct NGeo {
+    int I, J, K, L;  // padded dims of the (batched) grid
+    int P;           // frame period: l % P in {0, P-1} are ghost frames
+};
+
+// Taps of one convolution as linear point offsets (dx + I (dy + J (dz +
+// K dl))), built on the host for the grid at hand.  For the weight gradient
+// the taps are also grouped by row: `rowoff` lists the distinct (dy, dz, dl)
+// rows (dx = 0) and tap t reads row trow[t] shifted by tdx[t].
+struct Taps {
+    int n;
+    int off[kMaxTaps];
+    int nrow;
+    int rowoff[9];
+    signed char trow[kMaxTaps];
+    signed char tdx[kMaxTaps];
+};
+
+__device__ __forceinline__ long nvox(const NGeo& G, int i, int j, int k, int l) {
+    return ((long)(l * G.K + k) * G.J + j) * G.I + i;
+}
+
+__device__ __forceinline__ bool interior(const NGeo& G, int i, int j, int k, int l) {
+    const int lp = l % G.P;
+    return i >= 1 && i <= G.I - 2 && j >= 1 && j <= G.J - 2 && k >= 1 && k <= G.K - 2 &&
+           lp >= 1 && lp <= G.P - 2;
+}
+
+// Row r of the interior rows (j, k, l) of the grid, l counting interior
+// frames of every period.
+__device__ __forceinline__ void row_of(const NGeo& G, long r, int& j, int& k, int& l) {
+    const int nj = G.J - 2, nk = G.K - 2, nlp = G.P - 2;
+    j = (int)(r % nj) + 1;
+    r /= nj;
+    k = (int)(r % nk) + 1;
+    r /= nk;
+    const int lq = (int)r;
+    l = (lq / nlp) * G.P + (lq % nlp) + 1;
+}
+
+__device__ __forceinline__ long n_rows(const NGeo& G) {
+    return (long)(G.J - 2) * (G.K - 2) * ((G.L / G.P) * (G.P - 2));
+}
+
+// Epilogue of a convolution: 0 = store, 1 = + bias and ReLU,
+// 2 = multiply by (mask > 0) (ReLU derivative in the backward pass).
+enum { EPI_NONE = 0, EPI_BIAS_RELU = 1, EPI_MASK = 2 };
+
+// ----------------------------------------------------------------------------
+// Convolution Y = sum_tap W[tap] X[v + d(tap)], output tile 16 x-points by 16
+// channels per wave.  W is BF16 [tap][ci][co] (CIN x 16 per tap, taps padded
+// with zeros to a multiple of 32 / CIN).  Each wave walks `rows_per_wave`
+// rows; the weights stay in registers.
+// ----------------------------------------------------------------------------
+template <int CIN, int NT, int EPI>
+__global__ __launch_bounds__(256) void k_conv(NGeo G, Taps T, const bf16* __restrict__ X,
+                                               const bf16* __restrict__ W,
+                                               const float* __restrict__ bias,
+                                               const bf16* __restrict__ mask,
+                                               bf16* __restrict__ Y, int rows_per_wave) {
+    constexpr int TPS = 32 / CIN;               // taps per MFMA K-step
+    constexpr int NS = (NT + TPS - 1) / TPS;     // K-steps
+    const int ln = threadIdx.x & 63;
+    const int g = ln >> 4, m = ln & 15;
+    // B operands: lane holds W[tap][ci][co = m] for k = 8 g .. 8 g + 7
+    bf16x8 wreg[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        bf16x8 b;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int kk = s * 32 + 8 * g + q;
+            const int tap = kk / CIN, ci = kk % CIN;
+            b[q] = tap < NT ? W[(tap * CIN + ci) * kCh + m] : (bf16)0.0f;
+        }
+        wreg[s] = b;
+    }
+    const int tap_in_step = (8 * g) / CIN, ci0 = (8 * g) % CIN;
+    // this lane's tap offsets, one per K-step (uniform loads, lane selects)
+    int loff[NS];
+    bool lok[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int o = 0;
+#pragma unroll
+        for (int q = 0; q < TPS; q++)
+            if (tap_in_step == q && s * TPS + q < NT) o = T.off[s * TPS + q];
+        loff[s] = o;
+        lok[s] = s * TPS + tap_in_step < NT;
+    }
+    const float bco = (EPI == EPI_BIAS_RELU) ? bias[m] : 0.0f;
+    const long nrow = n_rows(G);
+    const int tiles = (G.I - 2 + 15) / 16;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    for (int rr = 0; rr < rows_per_wave; rr++) {
+        const long r = wave * rows_per_wave + rr;
+        if (r >= nrow) break;
+        int j, k, l;
+        row_of(G, r, j, k, l);
+        for (int tl = 0; tl < tiles; tl++) {
+            const int i = 1 + tl * 16 + m;  // this lane's A row (x point)
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                bf16x8 a;
+                const bool ok = lok[s] && i <= G.I - 2;
+                if (ok) {
+                    const long src = (nvox(G, i, j, k, l) + loff[s]) * CIN + ci0;
+                    a = *reinterpret_cast<const bf16x8*>(X + src);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; q++) a[q] = (bf16)0.0f;
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wreg[s], acc, 0, 0, 0);
+            }
+            // C[x = 4 g + rr2][co = m]
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int io = 1 + tl * 16 + 4 * g + q;
+                if (io > G.I - 2) continue;
+                const long dst = nvox(G, io, j, k, l) * kCh + m;
+                float v = acc[q];
+                if (EPI == EPI_BIAS_RELU) v = fmaxf(v + bco, 0.0f);
+                if (EPI == EPI_MASK) v = ((float)mask[dst] > 0.0f) ? v : 0.0f;
+                Y[dst] = (bf16)v;
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Weight gradient dW[tap][ci][co] = sum_v X[v + d(tap)][ci] dY[v][co] over the
+// interior points of a range of rows, as per-block partials (FP32, summed by
+// k_reduce_rows).  K of the MFMA is 32 x points of one row; X and dY are
+// staged in LDS so that each lane can gather 8 consecutive points of one
+// channel.  One wave per block; each block handles `rows_per_block` rows.
+// ----------------------------------------------------------------------------
+template <int CIN, int NT>
+__global__ __launch_bounds__(64) void k_conv_dw(NGeo G, Taps T, const bf16* __restrict__ X,
+                                                 const bf16* __restrict__ dY,
+                                                 float* __restrict__ partial,
+                                                 int rows_per_block) {
+    // staged rows: the distinct (dy, dz, dl) rows of the taps, 34 points each
+    // (x offsets -1..+1 around a 32-point chunk)
+    constexpr int kRowsMax = 9;
+    __shared__ bf16 sx[kRowsMax][34][CIN];
+    __shared__ bf16 sy[32][kCh];
+    const int ln = threadIdx.x;
+    const int g = ln >> 4, m = ln & 15;
+    // accumulators: per tap, a 16 (ci) x 16 (co) tile; CIN = 8 uses rows 0..7
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const long nrow = n_rows(G);
+    for (int rr = 0; rr < rows_per_block; rr++) {
+        const long r = (long)blockIdx.x * rows_per_block + rr;
+        if (r >= nrow) break;
+        int j, k, l;
+        row_of(G, r, j, k, l);
+        for (int x0 = 1; x0 <= G.I - 2; x0 += 32) {
+            __syncthreads();
+            // stage dY[x0 .. x0+31][0..15] (zero past the interior)
+            for (int e = ln; e < 32 * kCh; e += 64) {
+                const int xi = e / kCh, c = e % kCh;
+                const int x = x0 + xi;
+                sy[xi][c] = x <= G.I - 2 ? dY[nvox(G, x, j, k, l) * kCh + c] : (bf16)0.0f;
+            }
+            // stage X rows, points x0-1 .. x0+32 (within the padded row)
+            const long rbase = nvox(G, 0, j, k, l);
+            for (int q = 0; q < T.nrow; q++) {
+                const long base = rbase + T.rowoff[q];
+                for (int e = ln; e < 34 * CIN; e += 64) {
+                    const int xi = e / CIN, c = e % CIN;
+                    const int x = x0 - 1 + xi;
+                    sx[q][xi][c] = x <= G.I - 1 ? X[(base + x) * CIN + c] : (bf16)0.0f;
+                }
+            }
+            __syncthreads();
+            // B = dY chunk: lane holds dY[x = 8 g + q][co = m]
+            bf16x8 b;
+#pragma unroll
+            for (int q = 0; q < 8; q++) b[q] = sy[8 * g + q][m];
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                // A = X^T: lane holds X[x = 8 g + q + dx][ci = m]
+                bf16x8 a;
+                const int dx = T.tdx[t];
+                const int tr = T.trow[t];
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    a[q] = m < CIN ? sx[tr][8 * g + q + dx + 1][m < CIN ? m : 0] : (bf16)0.0f;
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    // partial[block][tap][ci][co]; C[ci = 4 g + q][co = m]
+    float* out = partial + (long)blockIdx.x * kMaxTaps * 16 * kCh;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) out[(t * 16 + 4 * g + q) * kCh + m] = acc[t][q];
+    }
+}
+
+// Sum of per-block partials: out[e] = sum_b partial[b][e], fixed order.
+__global__ __launch_bounds__(256) void k_reduce_rows(const float* __restrict__ partial, int nb,
+                                                     int n, float* __restrict__ out) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    float s = 0.f;
+    for (int b = 0; b < nb; b++) s += partial[(long)b * n + e];
+    out[e] = s;
+}
+
+// Bias gradient partials: sum over interior points of dY[v][co].
+__global__ __launch_bounds__(256) void k_bias_grad(NGeo G, const bf16* __restrict__ dY,
+                                                   float* __restrict__ partial,
+                                                   int rows_per_block) {
+    __shared__ float sm[256];
+    const int co = threadIdx.x & 15, lane16 = threadIdx.x >> 4;
+    float s = 0.f;
+    const long nrow = n_rows(G);
+    for (int rr = 0; rr < rows_per_block; rr++) {
+        const long r = (long)blockIdx.x * rows_per_block + rr;
+        if (r >= nrow) break;
+        int j, k, l;
+        row_of(G, r, j, k, l);
+        for (int x = 1 + lane16; x <= G.I - 2; x += 16)
+            s += (float)dY[nvox(G, x, j, k, l) * kCh + co];
+    }
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        float t = 0.f;
+        for (int q = 0; q < 16; q++) t += sm[q * 16 + threadIdx.x];
+        partial[(long)blockIdx.x * kCh + threadIdx.x] = t;
+    }
+}
+
+// 1x1 projection to one channel: out[v] = sum_c w[c] A[v][c] + b[0] (FP32).
+__global__ __launch_bounds__(256) void k_project(NGeo G, const bf16* __restrict__ A,
+                                                 const float* __restrict__ w,
+                                                 const float* __restrict__ b,
+                                                 float* __restrict__ out) {
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    const long nv = (long)G.I * G.J * G.K * G.L;
+    if (v >= nv) return;
+    const int i = (int)(v % G.I);
+    const long r1 = v / G.I;
+    const int j = (int)(r1 % G.J);
+    const long r2 = r1 / G.J;
+    const int k = (int)(r2 % G.K), l = (int)(r2 / G.K);
+    if (!interior(G, i, j, k, l)) return;
+    const bf16x8* a = reinterpret_cast<const bf16x8*>(A + v * kCh);
+    const bf16x8 a0 = a[0], a1 = a[1];
+    float s = b[0];
+#pragma unroll
+    for (int q = 0; q < 8; q++) s += w[q] * (float)a0[q] + w[8 + q] * (float)a1[q];
+    out[v] = s;
+}
+
+// Loss and the gradients of the projection: d = out - target on the loss
+// points (lmask > 0); per-block partials of sum d^2, of sum d A[v][c] and of
+// sum d; dA[v][c] = scale d w[c] where A[v][c] > 0 (the last ReLU), zero
+// elsewhere (BF16).
+__global__ __launch_bounds__(256) void k_loss_project_bwd(
+    NGeo G, const float* __restrict__ out, const float* __restrict__ target,
+    const unsigned char* __restrict__ lmask, const bf16* __restrict__ A,
+    const float* __restrict__ w, float scale, bf16* __restrict__ dA,
+    float* __restrict__ partial) {
+    // partial[block][0] = sum d^2, [1..16] = sum d A_c, [17] = sum d
+    __shared__ float sm[18][256];
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    const long nv = (long)G.I * G.J * G.K * G.L;
+    float d = 0.f;
+    float ac[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++) ac[c] = 0.f;
+    if (v < nv && lmask[v]) {
+        d = out[v] - target[v];
+        const bf16x8* a = reinterpret_cast<const bf16x8*>(A + v * kCh);
+        const bf16x8 a0 = a[0], a1 = a[1];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            ac[q] = (float)a0[q];
+            ac[8 + q] = (float)a1[q];
+        }
+        // A is the ReLU output of the last layer: its derivative masks dA
+        bf16x8 g0, g1;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            g0[q] = (bf16)(ac[q] > 0.f ? scale * d * w[q] : 0.f);
+            g1[q] = (bf16)(ac[8 + q] > 0.f ? scale * d * w[8 + q] : 0.f);
+        }
+        bf16x8* ga = reinterpret_cast<bf16x8*>(dA + v * kCh);
+        ga[0] = g0;
+        ga[1] = g1;
+    } else if (v < nv) {
+        // no loss here: zero gradient (ghost points stay zero as well)
+        bf16x8 z;
+#pragma unroll
+        for (int q = 0; q < 8; q++) z[q] = (bf16)0.0f;
+        bf16x8* ga = reinterpret_cast<bf16x8*>(dA + v * kCh);
+        ga[0] = z;
+        ga[1] = z;
+    }
+    sm[0][threadIdx.x] = d * d;
+#pragma unroll
+    for (int c = 0; c < 16; c++) sm[1 + c][threadIdx.x] = d * ac[c];
+    sm[17][threadIdx.x] = d;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 18; q++) sm[q][threadIdx.x] += sm[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 18) partial[(long)blockIdx.x * 18 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+// Adam step on FP32 master weights; BF16 copy for the kernels (wb may be
+// null for parameters used in FP32).
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ wm, float* __restrict__ m1,
+                                              float* __restrict__ m2,
+                                              const float* __restrict__ grad, int n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2,
+                                              bf16* __restrict__ wb) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const float gr = grad[e];
+    const float a = b1 * m1[e] + (1.f - b1) * gr;
+    const float bq = b2 * m2[e] + (1.f - b2) * gr * gr;
+    m1[e] = a;
+    m2[e] = bq;
+    const float w = wm[e] - lr * (a / c1) / (sqrtf(bq / c2) + eps);
+    wm[e] = w;
+    if (wb) wb[e] = (bf16)w;
+}
+
+}  // namespace net
+}  // namespace s4d

@@ -158,6 +158,33 @@ class SegLib(_Lib):
         L.seg4dLastRounds.restype = c_int
         L.seg4dSetWarmStart.restype = c_int
         L.seg4dSetWarmStart.argtypes = [c_int]
+        L.seg4dNetInit.restype = c_int
+        L.seg4dNetInit.argtypes = [c_uint64]
+        for fn in ("seg4dNetLoad", "seg4dNetSave"):
+            getattr(L, fn).restype = c_int
+            getattr(L, fn).argtypes = [c_char_p]
+        L.seg4dNetRecord.restype = c_int
+        L.seg4dNetRecord.argtypes = [c_int]
+        L.seg4dNetPoolPairs.restype = ctypes.c_long
+        L.seg4dNetTrain.restype = c_double
+        L.seg4dNetTrain.argtypes = [c_int, c_int, c_double, c_uint64, c_int]
+        L.seg4dNetParams.restype = c_int
+        L.seg4dNetParams.argtypes = [POINTER(ctypes.c_float), c_int]
+        L.seg4dNetSetParams.restype = c_int
+        L.seg4dNetSetParams.argtypes = [POINTER(ctypes.c_float), c_int, ctypes.c_float,
+                                        ctypes.c_float]
+        L.seg4dNetScales.restype = None
+        L.seg4dNetScales.argtypes = [POINTER(ctypes.c_float)] * 2
+        L.seg4dNetForward.restype = c_int
+        L.seg4dNetForward.argtypes = [c_int] * 5 + [POINTER(ctypes.c_uint16),
+                                                    POINTER(ctypes.c_float)]
+        L.seg4dNetGrad.restype = c_int
+        L.seg4dNetGrad.argtypes = [c_int] * 5 + [POINTER(ctypes.c_uint16),
+                                                 POINTER(ctypes.c_float),
+                                                 POINTER(ctypes.c_ubyte),
+                                                 POINTER(ctypes.c_float), POINTER(c_double)]
+        L.seg4dNetPredictMs.restype = c_double
+        L.seg4dNetPredicts.restype = ctypes.c_long
         L.seg4dDeviceInfo.restype = c_int
         L.seg4dDeviceInfo.argtypes = [c_char_p, c_int]
         L.seg4d_gen_preset.restype = c_int
@@ -301,8 +328,83 @@ class SegLib(_Lib):
         return self.lib.seg4dLastRounds()
 
     def set_warm_start(self, mode):
-        """0 = start SOR from u^n (reference), 1 = from 2 u^n - u^(n-1)."""
+        """0 = start SOR from u^n (reference), 1 = from 2 u^n - u^(n-1),
+        2 = from the learned network's prediction."""
         return self.lib.seg4dSetWarmStart(int(mode))
+
+    # -- learned warm start ---------------------------------------------------
+    def net_init(self, seed=0):
+        if self.lib.seg4dNetInit(int(seed)) != 0:
+            raise RuntimeError(f"seg4dNetInit failed: {self.last_error()}")
+
+    def net_load(self, path):
+        if self.lib.seg4dNetLoad(str(path).encode()) != 0:
+            raise RuntimeError(f"seg4dNetLoad failed: {self.last_error()}")
+
+    def net_save(self, path):
+        if self.lib.seg4dNetSave(str(path).encode()) != 0:
+            raise RuntimeError(f"seg4dNetSave failed: {self.last_error()}")
+
+    def net_record(self, new_run=False):
+        if self.lib.seg4dNetRecord(1 if new_run else 0) != 0:
+            raise RuntimeError(f"seg4dNetRecord failed: {self.last_error()}")
+
+    def net_pool_pairs(self):
+        return self.lib.seg4dNetPoolPairs()
+
+    def net_train(self, iters, batch=4, lr=1e-3, seed=0, log_every=0):
+        loss = self.lib.seg4dNetTrain(int(iters), int(batch), float(lr), int(seed),
+                                      int(log_every))
+        if loss < 0:
+            raise RuntimeError(f"seg4dNetTrain failed: {self.last_error()}")
+        return loss
+
+    def net_params(self) -> np.ndarray:
+        n = self.lib.seg4dNetParams(None, 0)
+        if n < 0:
+            raise RuntimeError(f"seg4dNetParams failed: {self.last_error()}")
+        out = np.zeros(n, dtype=np.float32)
+        self.lib.seg4dNetParams(out.ctypes.data_as(POINTER(ctypes.c_float)), n)
+        return out
+
+    def net_set_params(self, w, s_d=1.0, s_t=1.0):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if self.lib.seg4dNetSetParams(w.ctypes.data_as(POINTER(ctypes.c_float)), w.size,
+                                      float(s_d), float(s_t)) != 0:
+            raise RuntimeError(f"seg4dNetSetParams failed: {self.last_error()}")
+
+    def net_scales(self):
+        a, b = ctypes.c_float(), ctypes.c_float()
+        self.lib.seg4dNetScales(ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def net_forward(self, feat_bits: np.ndarray, P: int) -> np.ndarray:
+        """feat_bits: uint16 BF16 patterns of shape (L, K, J, I, 8)."""
+        L_, K_, J_, I_, _ = feat_bits.shape
+        f = np.ascontiguousarray(feat_bits, dtype=np.uint16)
+        out = np.zeros((L_, K_, J_, I_), dtype=np.float32)
+        if self.lib.seg4dNetForward(I_, J_, K_, L_, P, f.ctypes.data_as(POINTER(ctypes.c_uint16)),
+                                    out.ctypes.data_as(POINTER(ctypes.c_float))) != 0:
+            raise RuntimeError(f"seg4dNetForward failed: {self.last_error()}")
+        return out
+
+    def net_grad(self, feat_bits, target, lmask, P):
+        L_, K_, J_, I_, _ = feat_bits.shape
+        f = np.ascontiguousarray(feat_bits, dtype=np.uint16)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        m = np.ascontiguousarray(lmask, dtype=np.uint8)
+        g = np.zeros(self.lib.seg4dNetParams(None, 0), dtype=np.float32)
+        loss = c_double()
+        if self.lib.seg4dNetGrad(I_, J_, K_, L_, P, f.ctypes.data_as(POINTER(ctypes.c_uint16)),
+                                 t.ctypes.data_as(POINTER(ctypes.c_float)),
+                                 m.ctypes.data_as(POINTER(ctypes.c_ubyte)),
+                                 g.ctypes.data_as(POINTER(ctypes.c_float)),
+                                 ctypes.byref(loss)) != 0:
+            raise RuntimeError(f"seg4dNetGrad failed: {self.last_error()}")
+        return g, loss.value
+
+    def net_predict_stats(self):
+        return self.lib.seg4dNetPredictMs(), self.lib.seg4dNetPredicts()
 
     def set_sweep_variant(self, v):
         return self.lib.seg4dSetSweepVariant(int(v))

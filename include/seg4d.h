@@ -207,11 +207,52 @@ void seg4dSetRefinement(int max_rounds, double inner_floor);
 int seg4dLastRounds(void);
 /* Start of the SOR solve.  0 (default): u^n, as the reference.  1: the
  * linear extrapolation 2 u^n - u^(n-1), used when the previous call was a
- * completed _segTimeStep_ whose state stayed on the device (resident mode);
+ * completed _segTimeStep_ whose state stayed on the device (resident mode).
+ * 2: the prediction of the learned network (seg4dNet*), under the same
+ * condition as 1 (otherwise the solve starts from u^n).  In every mode
  * the system solved, its right-hand side and the stopping test are
- * unchanged, only the number of sweeps (and the last bits of u) differ.
- * Returns the previous mode, -1 before _allocateMem_. */
+ * unchanged; only the number of sweeps (and the last bits of u) differ.
+ * Returns the previous mode, -1 before _allocateMem_ or (mode 2) without a
+ * network. */
 int seg4dSetWarmStart(int mode);
+
+/* ---- learned warm start (no counterpart in the reference) ---------------
+ * A separable 4D convolutional network (4 x {3x3x3 spatial conv, 3-tap
+ * temporal conv, bias, ReLU}, 16 channels, 1x1 projection; BF16 MFMA
+ * kernels, FP32 accumulation) predicts the correction of the linear
+ * extrapolation 2 u^n - u^(n-1) from (u^n, u^n - u^(n-1), mean g).  It is
+ * trained on solver runs: seg4dNetRecord after each _segTimeStep_ (and once
+ * before the first) fills the pool; seg4dNetTrain fits it with Adam on
+ * patches of 64^3 points x all frames.  The network outlives
+ * _deallocateMem_, so runs of several volumes can be pooled.  Functions
+ * return 0 (or the documented value) on success, -1 on failure. */
+int seg4dNetInit(uint64_t seed);          /* fresh weights, empty pool      */
+int seg4dNetLoad(const char *path);       /* weights file of seg4dNetSave   */
+int seg4dNetSave(const char *path);
+/* Snapshot of the current u (the device copy; uploaded first in mirror
+ * mode); new_run != 0 starts a run and snapshots the mean g. */
+int seg4dNetRecord(int new_run);
+long seg4dNetPoolPairs(void);             /* (u^n, u^(n+1)) pairs pooled    */
+/* iters Adam steps (lr) on batches of `batch` patches; progress line every
+ * log_every steps (0: silent).  Returns the mean loss of the last 10% of
+ * steps, -1 on failure.  Runs need equal frame counts and >= 74 padded
+ * points in x, y, z. */
+double seg4dNetTrain(int iters, int batch, double lr, uint64_t seed, int log_every);
+/* Parameter vector (FP32) out of / into the network; seg4dNetParams(NULL, 0)
+ * returns the count.  s_d, s_t: feature and target scales. */
+int seg4dNetParams(float *out, int n);
+int seg4dNetSetParams(const float *in, int n, float s_d, float s_t);
+void seg4dNetScales(float *s_d, float *s_t);
+/* Test hooks on a caller-built batch of padded dims I, J, K, L (frame
+ * period P): features BF16 bit patterns [points][8]; one forward pass
+ * (out: FP32 [points]) or one forward + backward pass of the mean squared
+ * error over lmask points (gradient in parameter layout, loss). */
+int seg4dNetForward(int I, int J, int K, int L, int P, const uint16_t *feat_bf16, float *out);
+int seg4dNetGrad(int I, int J, int K, int L, int P, const uint16_t *feat_bf16,
+                 const float *target, const unsigned char *lmask, float *grad_out,
+                 double *loss);
+double seg4dNetPredictMs(void);           /* total time of the predictions  */
+long seg4dNetPredicts(void);
 /* Blocks until all queued device work is done. */
 bool seg4dSynchronize(void);
 /* Device name / CU count / arch into buf; returns 0 on success. */
