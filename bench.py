@@ -113,6 +113,15 @@ def main():
     ap.add_argument("--variant", type=int, default=None, help="SOR pass kernel variant")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the FP32-refined run of the same time steps")
+    ap.add_argument("--net", default=None,
+                    help="weights of the learned warm start (default: the package's "
+                         "weights/warmstart.bin when present)")
+    ap.add_argument("--train-net", action="store_true",
+                    help="train the learned warm start first (solver runs + Adam) and "
+                         "save it to --net-out")
+    ap.add_argument("--net-out", default=None)
+    ap.add_argument("--train-iters", type=int, default=2500)
+    ap.add_argument("--train-runs", type=int, default=8)
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one device copy of u (padded volume, "
                          "FP64) as the known-size reference for rocprofv3 --pmc byte counts")
@@ -123,9 +132,18 @@ def main():
         sys.exit(2)
 
     import seg4d
+    import warmstart
 
     lib = seg4d.SegLib()
     lib.set_verbose(False)
+    net_path = args.net or warmstart.DEFAULT_WEIGHTS
+    training = None
+    if args.train_net and not args.no_mixed:
+        log("[bench] training the learned warm start ...")
+        training = warmstart.train(lib, runs=args.train_runs, iters=args.train_iters,
+                                   out=args.net_out or net_path, log=log)
+        net_path = args.net_out or net_path
+        log(f"[bench] trained: {training}")
     spec = lib.gen_spec(args.preset, args.preset)
     nx, ny, nz, nt = spec.nx, spec.ny, spec.nz, spec.nt
     shape = (nt + 2, nz + 2, ny + 2, nx + 2)
@@ -190,7 +208,8 @@ def main():
 
     def run_refined(warm):
         """The same time steps in the FP32-refined form from the same u0;
-        warm = 1 starts every SOR solve from 2 u^n - u^(n-1)."""
+        warm = 1 starts every SOR solve from 2 u^n - u^(n-1), warm = 2 from
+        the learned network's prediction (both from the second step on)."""
         u64 = inner.copy()
         lib.set_precision(1)
         lib.set_warm_start(warm)
@@ -204,6 +223,7 @@ def main():
         lib.profile(True)
         lib.profile_reset()
         msweeps, mrounds = [], []
+        pms0, pn0 = lib.net_predict_stats()
         t2 = time.perf_counter()
         for s in range(args.steps):
             lib.set(numberOfTimeStep=s + 1)
@@ -214,6 +234,7 @@ def main():
         t3 = time.perf_counter()
         mprof = lib.profile_read()
         lib.profile(False)
+        pms1, pn1 = lib.net_predict_stats()
         lib.set_precision(0)
         lib.set_warm_start(0)
         lib.download("u")
@@ -224,7 +245,7 @@ def main():
         union = np.logical_or(mask64, maskm).sum()
         r32 = mprof["red1_f32_ms"] / max(mprof["red1_f32_launches"], 1)
         res = {
-            "warm_start": "2u^n - u^(n-1)" if warm else "u^n",
+            "warm_start": {0: "u^n", 1: "2u^n - u^(n-1)", 2: "learned"}[warm],
             "time_to_tolerance_s": t3 - t2,
             "ms_per_step": (t3 - t2) / args.steps * 1e3,
             "speedup_vs_fp64": total / (t3 - t2),
@@ -233,6 +254,7 @@ def main():
             "red_pass_f32_avg_ms": r32,
             "red_pass_f32_gbps": nred * 48.0 / (r32 / 1e3) / 1e9 if r32 > 0 else None,
             "defect_ms_per_step": mprof["defect_ms"] / args.steps,
+            "predict_ms_per_step": (pms1 - pms0) / max(pn1 - pn0, 1) if warm == 2 else None,
             "max_abs_du_vs_fp64_final": float(np.abs(um - u64).max()),
             "max_abs_u_fp64_final": float(np.abs(u64).max()),
             "mask_iou_vs_fp64_final": float(inter / union) if union else 1.0,
@@ -244,10 +266,17 @@ def main():
             f"IoU {res['mask_iou_vs_fp64_final']:.5f}")
         return res
 
-    mixed = mixed_warm = None
+    mixed = mixed_warm = learned = None
     if not args.no_mixed:
         mixed = run_refined(0)
         mixed_warm = run_refined(1)
+        if os.path.exists(net_path):
+            lib.net_load(net_path)
+            learned = run_refined(2)
+            learned["weights"] = os.path.relpath(net_path, ROOT)
+            learned["training"] = training
+        else:
+            log(f"[bench] no learned warm start weights at {net_path}: skipped")
 
     red1_avg_s = prof["red1_ms"] / max(prof["red1_launches"], 1) / 1e3
     achieved = nred * BYTES_PER_UPDATE / red1_avg_s / 1e9
@@ -305,6 +334,7 @@ def main():
         "mask_fraction": mask_frac,
         "fp32_refined": mixed,
         "fp32_refined_extrapolated_start": mixed_warm,
+        "fp32_refined_learned_start": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
