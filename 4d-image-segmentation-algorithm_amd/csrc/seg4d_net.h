@@ -6,15 +6,18 @@
 // method stays the judge and a better start only shows up as fewer sweeps.
 //
 // Network (parameters FP32, kernels BF16 with FP32 accumulation):
-//   features x0 = [u^n, s_d (u^n - u^(n-1)), g_mean, 0 x 5]   (8 channels)
+//   sigma = rms of d = u^n - u^(n-1) over the interior (this step's motion)
+//   features x0 = [u^n, s_d d / sigma, g_mean, 0 x 5]   (8 channels)
 //   4 x { 3x3x3 spatial conv -> 3-tap temporal conv + bias -> ReLU }, 16 ch
-//   1x1 projection -> c;  start = u^n + (u^n - u^(n-1)) + c / s_t
-// i.e. it learns the correction to the linear extrapolation.  Training
+//   1x1 projection -> c;  start = u^n + d + sigma c / s_t
+// i.e. it learns the correction to the linear extrapolation, in units of the
+// step's own increment, so that one network serves early (fast) and late
+// (slow) steps alike.  Training
 // pairs come from the solver itself: record() snapshots the device u after
 // each step of a run; train() samples patches of those runs (core 64^3
 // points, all frames, a 4-point halo of real data) at levels n >= 1 and
 // minimises the mean squared error of c against s_t (u^(n+1) - 2 u^n +
-// u^(n-1)) with Adam.
+// u^(n-1)) / sigma_n with Adam.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -100,6 +103,7 @@ private:
         int I = 0, J = 0, K = 0, L = 0;
         __bf16* gmean = nullptr;       // BF16 [points]
         std::vector<float*> snaps;     // FP32 [points] per time level
+        std::vector<double> sigma;     // rms of u^n - u^(n-1) over the interior (0 for n = 0)
     };
     std::vector<Run> runs_;
 

@@ -94,9 +94,16 @@ __device__ __forceinline__ bool point_of(const NGeo& G, long v, int& i, int& j, 
     return net::interior(G, i, j, k, l);
 }
 
+// sigma = sqrt(sum / n) of the increment u - prev over the interior, from the
+// device sum *dsum (0 if there is no increment)
+__device__ __forceinline__ double step_sigma(const double* dsum, double inv_n) {
+    return dsum ? sqrt(*dsum * inv_n) : 0.0;
+}
+
 __global__ __launch_bounds__(256) void k_features(NGeo G, const double* __restrict__ u,
                                                   const double* __restrict__ prev,
                                                   const bf16* __restrict__ gm, float sd,
+                                                  const double* __restrict__ dsum, double inv_n,
                                                   bf16* __restrict__ feat) {
     const long v = (long)blockIdx.x * 256 + threadIdx.x;
     if (v >= (long)G.I * G.J * G.K * G.L) return;
@@ -104,9 +111,11 @@ __global__ __launch_bounds__(256) void k_features(NGeo G, const double* __restri
     if (!point_of(G, v, i, j, k, l)) return;
     const double uc = u[v];
     const double d = prev ? uc - prev[v] : 0.0;
+    const double sig = step_sigma(dsum, inv_n);
+    const double inv_sig = sig > 0.0 ? 1.0 / sig : 0.0;
     bf16x8 f;
     f[0] = (bf16)(float)uc;
-    f[1] = (bf16)((float)d * sd);
+    f[1] = (bf16)((float)(d * inv_sig) * sd);
     f[2] = gm[v];
 #pragma unroll
     for (int q = 3; q < 8; q++) f[q] = (bf16)0.0f;
@@ -116,6 +125,7 @@ __global__ __launch_bounds__(256) void k_features(NGeo G, const double* __restri
 __global__ __launch_bounds__(256) void k_start(NGeo G, const double* __restrict__ u,
                                                const double* __restrict__ prev,
                                                const float* __restrict__ out, float inv_st,
+                                               const double* __restrict__ dsum, double inv_n,
                                                double* __restrict__ start) {
     const long v = (long)blockIdx.x * 256 + threadIdx.x;
     if (v >= (long)G.I * G.J * G.K * G.L) return;
@@ -123,7 +133,29 @@ __global__ __launch_bounds__(256) void k_start(NGeo G, const double* __restrict_
     if (!point_of(G, v, i, j, k, l)) return;
     const double uc = u[v];
     const double d = prev ? uc - prev[v] : 0.0;
-    start[v] = uc + d + (double)(out[v] * inv_st);
+    start[v] = uc + d + step_sigma(dsum, inv_n) * (double)(out[v] * inv_st);
+}
+
+// Per-block partial sums (FP64) of (a - b)^2 over the interior points.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sumsq_diff(NGeo G, const T* __restrict__ a,
+                                                    const T* __restrict__ b,
+                                                    double* __restrict__ partial) {
+    __shared__ double sm[256];
+    const long v = (long)blockIdx.x * 256 + threadIdx.x;
+    double s = 0;
+    int i, j, k, l;
+    if (v < (long)G.I * G.J * G.K * G.L && point_of(G, v, i, j, k, l)) {
+        const double d = (double)a[v] - (double)b[v];
+        s = d * d;
+    }
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
 }
 
 __global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ a, long n,
@@ -139,7 +171,7 @@ __global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ a, lo
 __global__ __launch_bounds__(256) void k_gather_patch(
     NGeo PG, int b, int RI, int RJ, int RK, const float* __restrict__ um1,
     const float* __restrict__ u0, const float* __restrict__ up1,
-    const bf16* __restrict__ gm, int ax, int ay, int az, float sd, float st,
+    const bf16* __restrict__ gm, int ax, int ay, int az, float sd, float st, float inv_sig,
     bf16* __restrict__ feat, float* __restrict__ target, unsigned char* __restrict__ lmask) {
     const long q = (long)blockIdx.x * 256 + threadIdx.x;
     const long per = (long)PG.I * PG.J * PG.K * PG.P;
@@ -162,9 +194,9 @@ __global__ __launch_bounds__(256) void k_gather_patch(
         const long rv = (((long)pl * RK + (az + pk)) * RJ + (ay + pj)) * RI + (ax + pi);
         const float uc = u0[rv], um = um1[rv], up = up1[rv];
         f[0] = (bf16)uc;
-        f[1] = (bf16)((uc - um) * sd);
+        f[1] = (bf16)((uc - um) * inv_sig * sd);
         f[2] = gm[rv];
-        tg = ((up - uc) - (uc - um)) * st;
+        tg = ((up - uc) - (uc - um)) * inv_sig * st;
         auto ok = [](int p, int n, int a, int rn) {
             return (p >= kHalo + 1 || a == 0) && (p <= n - kHalo - 2 || a + n == rn);
         };
@@ -541,11 +573,27 @@ int Net::record(Engine& E, bool new_run) {
     if (check(hipMalloc((void**)&s, pts * 4), "hipMalloc pool")) return -1;
     k_to_f32<<<(unsigned)cdivl(pts, 256), 256, 0, E.st_>>>(E.full(A_U), pts, s);
     r.snaps.push_back(s);
+    // rms increment of this level over the interior (0 for the first level)
+    double sig = 0.0;
+    if (r.snaps.size() >= 2) {
+        const NGeo G{r.I, r.J, r.K, r.L, r.L};
+        const int nb = (int)cdivl(pts, 256);
+        double* part = nullptr;
+        if (check(hipMalloc((void**)&part, (size_t)nb * sizeof(double)), "hipMalloc")) return -1;
+        k_sumsq_diff<float><<<nb, 256, 0, E.st_>>>(G, s, r.snaps[r.snaps.size() - 2], part);
+        k_reduce_d<<<1, 256, 0, E.st_>>>(part, nb, 1, dscal_);
+        const int rc = check(hipMemcpyAsync(hscal_, dscal_, sizeof(double), hipMemcpyDeviceToHost,
+                                            E.st_), "record") ||
+                       check(hipStreamSynchronize(E.st_), "record");
+        (void)hipFree(part);
+        if (rc) return -1;
+        const double nint = (double)(r.I - 2) * (r.J - 2) * (r.K - 2) * (r.L - 2);
+        sig = std::sqrt(hscal_[0] / nint);
+    }
+    r.sigma.push_back(sig);
     return check(hipStreamSynchronize(E.st_), "net record");
 }
 
-// Training pairs: levels n >= 1 with a successor (n = 0 has no increment
-// u^n - u^(n-1), and the learned start is not used there).
 long Net::pool_pairs() const {
     long n = 0;
     for (const Run& r : runs_) n += r.snaps.size() > 2 ? (long)r.snaps.size() - 2 : 0;
@@ -571,8 +619,10 @@ int Net::compute_scales(hipStream_t st) {
                 (void)hipFree(part);
                 return -1;
             }
-            sd += hscal_[0];
-            sc += hscal_[1];
+            // features and targets are divided by the level's rms increment
+            const double s2 = r.sigma[n] > 0 ? r.sigma[n] * r.sigma[n] : 1.0;
+            sd += hscal_[0] / s2;
+            sc += hscal_[1] / s2;
             npts += (long)(r.I - 2) * (r.J - 2) * (r.K - 2) * (r.L - 2);
         }
         (void)hipFree(part);
@@ -632,7 +682,7 @@ int Net::train(int iters, int batch, float lr, uint64_t seed, int log_every, dou
             const float* um1 = r.snaps[n - 1];
             k_gather_patch<<<(unsigned)cdivl(per, 256), 256, 0, st>>>(
                 G, b, r.I, r.J, r.K, um1, r.snaps[n], r.snaps[n + 1], r.gmean, ax, ay, az, s_d, s_t,
-                feat_, target_, lmask_);
+                r.sigma[n] > 0 ? (float)(1.0 / r.sigma[n]) : 0.f, feat_, target_, lmask_);
             auto cnt = [&](int a, int rn) {
                 const int lo = a == 0 ? 1 : kHalo + 1;
                 const int hi = a + kPatch == rn ? kPatch - 2 : kPatch - kHalo - 2;
@@ -752,11 +802,20 @@ int Net::predict(Engine& E, double* start, bool history) {
         g_version_ = E.g_version_;
     }
     const double* prev = history ? E.full(A_PREV) : nullptr;
+    const double inv_n = 1.0 / ((double)(E.I_ - 2) * (E.J_ - 2) * (E.K_ - 2) * (E.L_ - 2));
+    const double* dsum = nullptr;
+    if (prev) {
+        // rms increment of this step, left on the device for the two kernels
+        const int nb = (int)cdivl(pts, 256);
+        k_sumsq_diff<double><<<nb, 256, 0, E.st_>>>(G, E.full(A_U), prev, (double*)partial_);
+        k_reduce_d<<<1, 256, 0, E.st_>>>((double*)partial_, nb, 1, dscal_ + 8);
+        dsum = dscal_ + 8;
+    }
     k_features<<<(unsigned)cdivl(pts, 256), 256, 0, E.st_>>>(G, E.full(A_U), prev, gmean_, s_d,
-                                                             feat_);
+                                                             dsum, inv_n, feat_);
     if (forward(G, E.st_, false)) return -1;
     k_start<<<(unsigned)cdivl(pts, 256), 256, 0, E.st_>>>(G, E.full(A_U), prev, out_,
-                                                          1.f / s_t, start);
+                                                          1.f / s_t, dsum, inv_n, start);
     (void)hipEventRecord(e1, E.st_);
     if (check(hipEventSynchronize(e1), "net predict")) return -1;
     float ms = 0.f;
