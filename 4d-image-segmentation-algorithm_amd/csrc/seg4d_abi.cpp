@@ -681,8 +681,16 @@ API void seg4dNetScales(float* s_d, float* s_t) {
 API int seg4dSetSweepVariant(int v) {
     if (!g_eng) return -1;
     const int old = g_eng->sweep_variant_;
-    g_eng->sweep_variant_ = (v >= 0 && v <= 3) ? v : 0;
+    g_eng->sweep_variant_ = (v >= 0 && v <= 31 && (v & 24) != 24) ? v : 0;
     return old;
+}
+
+API int seg4dSetSweepTile(int runs, int planes) {
+    if (!g_eng) return -1;
+    if (runs < 1 || planes < 1) return -1;
+    g_eng->sweep_tb_ = runs;
+    g_eng->sweep_tk_ = planes;
+    return 0;
 }
 
 API bool seg4dSynchronize(void) {

@@ -136,9 +136,14 @@ public:
 
     hipStream_t stream() const { return st_; }
     // SOR pass kernel variant (tuning knob, identical results), bit flags:
-    // 1 = two l-steps per load phase, 2 = tiled XCD-resident launch order.
-    // 3 measured fastest on MI355X at 256x256x128x16.
+    // 1 = two l-steps per load phase, 2 = tiled XCD-resident launch order,
+    // 4 = nontemporal operand-block loads, 8 = two k planes per block,
+    // 16 = four k planes per block.
     int sweep_variant_ = 3;
+    // tile of the tiled order in blocks (runs x k groups); 16 x 8 measured
+    // fastest at 256x256x128x16 (8 x 32: 1.401 ms, 16 x 16: 1.352 ms, 16 x 8:
+    // 1.347 ms per red pass)
+    int sweep_tb_ = 16, sweep_tk_ = 8;
     // 0 = FP64 (the reference's arithmetic); 1 = FP32-refined: FP32 SOR on
     // the correction equation inside FP64 defect correction.
     int precision_ = 0;

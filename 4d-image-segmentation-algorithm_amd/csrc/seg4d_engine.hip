@@ -286,11 +286,12 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
 
 SweepMap Engine::sweep_map() const {
     SweepMap M;
-    M.gx = (int)cdiv((long)H_ * (J_ - 2), kBlock);
-    M.gk = K_ - 2;
+    M.kp = (sweep_variant_ & 16) ? 4 : (sweep_variant_ & 8) ? 2 : 1;
+    M.gx = (int)cdiv((long)H_ * (J_ - 2), kBlock / M.kp);
+    M.gk = (int)cdiv((long)(K_ - 2), M.kp);
     M.tiled = (sweep_variant_ & 2) ? 1 : 0;
-    M.tb = 8;
-    M.tk = 32;
+    M.tb = sweep_tb_;
+    M.tk = sweep_tk_;
     M.ntb = (M.gx + M.tb - 1) / M.tb;
     M.ntiles = M.ntb * ((M.gk + M.tk - 1) / M.tk);
     return M;
@@ -336,26 +337,37 @@ int sor_run(Engine& E, const SorJob<T>& J, int* zout) {
         return r;
     };
     const bool u2 = (E.sweep_variant_ & 1) != 0;
+    const bool nto = (E.sweep_variant_ & 4) != 0;
+    // one colour pass: template variant by (resid kind, U, nontemporal operands)
+    auto pass = [&](int rk, int colour, const T* xin, T* xout, const T* y, const T* ob,
+                    double* part, const SorCtl* ctl) {
+#define S4D_PASS(RK, UU, NT)                                                              \
+    k_sor_sweep<T, RK, UU, NT><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, colour, xin, \
+                                                        xout, y, ob, part, ctl, nullptr)
+        if (rk == 0) {
+            if (u2) { if (nto) S4D_PASS(0, 2, true); else S4D_PASS(0, 2, false); }
+            else { if (nto) S4D_PASS(0, 1, true); else S4D_PASS(0, 1, false); }
+        } else if (rk == 1) {
+            if (u2) { if (nto) S4D_PASS(1, 2, true); else S4D_PASS(1, 2, false); }
+            else { if (nto) S4D_PASS(1, 1, true); else S4D_PASS(1, 1, false); }
+        } else {
+            if (u2) { if (nto) S4D_PASS(2, 2, true); else S4D_PASS(2, 2, false); }
+            else { if (nto) S4D_PASS(2, 1, true); else S4D_PASS(2, 1, false); }
+        }
+#undef S4D_PASS
+    };
     auto red = [&](int z, bool resid) -> int {
         hipEvent_t* e = E.prof.on ? ev_pair(J.kind0 + (resid ? 1 : 0)) : nullptr;
         if (e) (void)hipEventRecord(e[0], st);
-        const T* xin = J.red[(z - 1) & 1];
-        T* xout = J.red[z & 1];
-        if (resid) {
-            if (u2) k_sor_sweep<T, 1, 2><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 0, xin, xout, J.black, J.opnd[0], part_r, nullptr, nullptr);
-            else k_sor_sweep<T, 1, 1><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 0, xin, xout, J.black, J.opnd[0], part_r, nullptr, nullptr);
-        } else {
-            if (u2) k_sor_sweep<T, 0, 2><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 0, xin, xout, J.black, J.opnd[0], nullptr, nullptr, nullptr);
-            else k_sor_sweep<T, 0, 1><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 0, xin, xout, J.black, J.opnd[0], nullptr, nullptr, nullptr);
-        }
+        pass(resid ? 1 : 0, 0, J.red[(z - 1) & 1], J.red[z & 1], J.black, J.opnd[0],
+             resid ? part_r : nullptr, nullptr);
         if (e) (void)hipEventRecord(e[1], st);
         return E.launch_check("k_sor_sweep red");
     };
     auto black = [&](int z) -> int {
         hipEvent_t* e = E.prof.on ? ev_pair(J.kind0 + 2) : nullptr;
         if (e) (void)hipEventRecord(e[0], st);
-        if (u2) k_sor_sweep<T, 2, 2><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 1, J.black, J.black, J.red[z & 1], J.opnd[1], part_b, J.ctl, nullptr);
-        else k_sor_sweep<T, 2, 1><<<grid, kBlock, 0, st>>>(G, M, J.c, J.om, J.h4, 1, J.black, J.black, J.red[z & 1], J.opnd[1], part_b, J.ctl, nullptr);
+        pass(2, 1, J.black, J.black, J.red[z & 1], J.opnd[1], part_b, J.ctl);
         if (e) (void)hipEventRecord(e[1], st);
         return E.launch_check("k_sor_sweep black");
     };

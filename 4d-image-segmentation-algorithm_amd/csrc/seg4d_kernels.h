@@ -263,7 +263,10 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
 //          per residency round, so that the blocks sharing the other colour's
 //          k+-1 planes are co-resident on one XCD (one L2) and march l in step.
 struct SweepMap {
-    int gx, gk;      // runs of kBlock points per row plane, interior k planes
+    int gx, gk;      // block grid: runs of kBlock / kp points per row plane,
+                     // groups of kp consecutive interior k planes
+    int kp;          // k planes per block (1, 2, 4): the block's planes share
+                     // their other-colour neighbour planes through L1/L2
     int tiled;
     int tb, tk;      // tile shape; tb * tk blocks per tile
     int ntb, ntiles;
@@ -302,7 +305,10 @@ struct PointIn {
     bool act;
 };
 
-template <typename T>
+// NTO: the operand-block loads carry the nontemporal hint (the stream is read
+// once per pass; the hint keeps it from displacing the other colour's planes,
+// which neighbouring blocks re-read, from L2).
+template <typename T, bool NTO = false>
 __device__ __forceinline__ void load_point(const Geo& G, const T* __restrict__ ob,
                                            const T* __restrict__ y, const T* xin, int pp,
                                            int sft, bool act, PointIn<T>& P) {
@@ -315,9 +321,15 @@ __device__ __forceinline__ void load_point(const Geo& G, const T* __restrict__ o
         P.yN = y[pp + G.py];
         P.yB = y[pp - G.pz];
         P.yT = y[pp + G.pz];
+        if constexpr (NTO) {
 #pragma unroll
-        for (int f = 0; f < 8; f++) P.c[f] = ob[f * 64];
-        P.pv = ob[8 * 64];
+            for (int f = 0; f < 8; f++) P.c[f] = __builtin_nontemporal_load(ob + f * 64);
+            P.pv = __builtin_nontemporal_load(ob + 8 * 64);
+        } else {
+#pragma unroll
+            for (int f = 0; f < 8; f++) P.c[f] = ob[f * 64];
+            P.pv = ob[8 * 64];
+        }
         P.uo = xin[pp];
     }
 }
@@ -385,21 +397,28 @@ __device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart
 // the defect -r written as FP32 into component 8 of the operand blocks
 // `defect_ob` (the right-hand side of the FP32 correction equation).
 // ----------------------------------------------------------------------------
-template <typename T, int RESID, int U>
+template <typename T, int RESID, int U, bool NTO = false>
 __global__ __launch_bounds__(kBlock) void k_sor_sweep(
     Geo G, SweepMap M, T c, T om, double h4, int colour,
     const T* xin, T* xout, const T* __restrict__ y,
     const T* __restrict__ opnd, double* __restrict__ partials,
     const SorCtl* __restrict__ ctl, float* __restrict__ defect_ob) {
     if (ctl != nullptr && ctl->stop) return;
-    int bx, k;
-    if (!map_block(M, blockIdx.x, gridDim.x, bx, k)) return;
-    const int lb = (k - 1) * M.gx + bx;
-    const int t = bx * kBlock + threadIdx.x;
+    int bx, kb;
+    if (!map_block(M, blockIdx.x, gridDim.x, bx, kb)) return;
+    const int lb = (kb - 1) * M.gx + bx;
+    // the block's kp planes: threads [sub * per, (sub + 1) * per) take plane
+    // (kb - 1) kp + sub + 1, run bx of `per` points
+    const int per = kBlock / M.kp;
+    const int sub = threadIdx.x / per;
+    const int kk = (kb - 1) * M.kp + sub + 1;
+    const bool kok = kk <= G.K - 2;
+    const int k = kok ? kk : G.K - 2;  // indices stay in range for idle threads
+    const int t = bx * per + (threadIdx.x - sub * per);
     const int jr = t / G.H;
     const int m = t - jr * G.H;
     const int j = jr + 1;
-    const bool rowok = j <= G.J - 2;
+    const bool rowok = kok && j <= G.J - 2;
     double acc = 0.0;
     const int base = (k * G.J + j) * G.H + m;  // l = 0
     // operand block of (k, l = 1, t); the next l plane is ostride further
@@ -422,8 +441,8 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(
             w[q + 2] = rowok ? y[pp + G.pw] : T(0);
             const int sft = colour ^ ((j + k + lq) & 1);
             const int i = 2 * m + sft;
-            load_point<T>(G, ob0 + (lq - 1) * ostride, y, xin, pp, sft,
-                          rowok && i >= 1 && i <= G.I - 2, P[q]);
+            load_point<T, NTO>(G, ob0 + (lq - 1) * ostride, y, xin, pp, sft,
+                               rowok && i >= 1 && i <= G.I - 2, P[q]);
         }
 #pragma unroll
         for (int q = 0; q < U; q++)
@@ -438,8 +457,8 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(
         w[2] = rowok ? y[pp + G.pw] : T(0);
         const int sft = colour ^ ((j + k + l) & 1);
         const int i = 2 * m + sft;
-        load_point<T>(G, ob0 + (l - 1) * ostride, y, xin, pp, sft,
-                      rowok && i >= 1 && i <= G.I - 2, P);
+        load_point<T, NTO>(G, ob0 + (l - 1) * ostride, y, xin, pp, sft,
+                           rowok && i >= 1 && i <= G.I - 2, P);
         update_point<T, RESID>(P, w[0], w[1], w[2], c, om, h4, xout,
                                def0 ? def0 + (l - 1) * ostride : nullptr, acc);
         w[0] = w[1];

@@ -186,9 +186,14 @@ int seg4dProfileRead(double *out, int nout);
 void seg4dProfileReset(void);
 /* SOR pass kernel variant (tuning knob, identical results), bit flags:
  * 1 = two l-steps per load phase; 2 = tiled launch order (the blocks that
- * share neighbour planes are co-resident on one XCD).  Default 3.  Returns
- * the previous value, -1 before _allocateMem_. */
+ * share neighbour planes are co-resident on one XCD); 4 = nontemporal loads
+ * of the coefficient stream; 8 = two, 16 = four consecutive k planes per
+ * block (8 and 16 exclusive).  Default 3.  Returns the previous value, -1
+ * before _allocateMem_. */
 int seg4dSetSweepVariant(int v);
+/* Tile of the tiled launch order in blocks: `runs` runs x `planes` k-plane
+ * groups per XCD residency round (default 16 x 8).  0 on success. */
+int seg4dSetSweepTile(int runs, int planes);
 /* Arithmetic of the SOR solve.  0 (default): FP64, the reference's
  * arithmetic and bits.  1: FP32-refined -- FP64 defect correction in which
  * the defect b - A u of the FP64 system is recomputed in FP64 on the GPU and

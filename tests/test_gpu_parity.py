@@ -162,10 +162,12 @@ def test_seg_random_vs_oracle(seglib, oracle, dims):
         lib.deallocate()
 
 
-@pytest.mark.parametrize("sweeps,variant", [(1, 0), (2, 0), (7, 0), (7, 1), (7, 2), (7, 3)])
+@pytest.mark.parametrize("sweeps,variant", [(1, 0), (2, 0), (7, 0), (7, 1), (7, 2), (7, 3),
+                                            (7, 7), (7, 11), (7, 15), (7, 19), (7, 23), (7, 9)])
 def test_fixed_sweep_count(seglib, oracle, sweeps, variant):
     """gsTol < 0 forces exactly maxNoGSIteration sweeps (sweep cap path);
-    every SOR pass kernel variant gives the same bits."""
+    every SOR pass kernel variant gives the same bits (bits 8 / 16: two /
+    four k planes per block; nz = 9 leaves the last block partly idle)."""
     nx, ny, nz, nt = 12, 10, 9, 4
     shape = (nt + 2, nz + 2, ny + 2, nx + 2)
     rng = np.random.default_rng(sweeps)
@@ -188,5 +190,37 @@ def test_fixed_sweep_count(seglib, oracle, sweeps, variant):
         z_o, _ = oracle.seg_time_step(p, u, prev, gs, zeros(shape, 8), np.ones(shape))
         assert z_g == z_o == sweeps
         assert np.array_equal(lib.arr("u"), u)
+    finally:
+        lib.deallocate()
+
+
+@pytest.mark.parametrize("variant,dims", [(11, (17, 11, 7, 3)), (19, (17, 11, 7, 3)),
+                                          (10, (8, 6, 5, 4)), (15, (30, 9, 6, 5))])
+def test_tolerance_driven_variants(seglib, oracle, variant, dims):
+    """Tolerance-driven steps (the fused residual and its per-block partials)
+    with several k planes per block: same sweep count and bits as the oracle."""
+    nx, ny, nz, nt = dims
+    shape = (nt + 2, nz + 2, ny + 2, nx + 2)
+    rng = np.random.default_rng(variant)
+    u0 = _random_volume(rng, shape)
+    gs = [_random_volume(rng, shape) + 0.01 for _ in range(8)]
+    kw = dict(h=1.0, tau=0.5, eps2=1e-6, omega_c=1.8, maxNoGSIteration=5000)
+    p = params(shape, gsTol=1e-9, **kw)
+    lib = seglib
+    lib.allocate(nx, ny, nz, nt)
+    lib.set_sweep_variant(variant)
+    try:
+        lib.set(gsTol=1e-9, **kw)
+        for f, gg in zip(FACES, gs):
+            lib.arr("g_" + f)[...] = gg
+        lib.host_wrote("g_e")
+        lib.arr("Mmask")[...] = 1.0
+        lib.arr("u")[...] = u0
+        u, prev = u0.copy(), np.zeros(shape)
+        for st in range(2):
+            z_g, _ = lib.seg_time_step()
+            z_o, _ = oracle.seg_time_step(p, u, prev, gs, zeros(shape, 8), np.ones(shape))
+            assert z_g == z_o
+            assert np.array_equal(lib.arr("u"), u)
     finally:
         lib.deallocate()
