@@ -535,6 +535,10 @@ API void seg4dSetRefinement(int max_rounds, double inner_floor) {
     if (inner_floor > 0) g_eng->inner_floor2_ = inner_floor * inner_floor;
 }
 
+API void seg4dSetInnerFraction(double frac) {
+    if (g_eng && frac > 0.0 && frac < 1.0) g_eng->inner_frac_ = frac;
+}
+
 API int seg4dLastRounds(void) { return g_last.rounds; }
 
 API int seg4dSetWarmStart(int mode) {

@@ -155,7 +155,10 @@ public:
     bool hist_ok_ = false;
     Net* net_ = nullptr;           // not owned
     long g_version_ = 0;           // bumped whenever the g arrays change
-    double inner_frac_ = 0.25;     // inner stop: R32 <= max(frac*tol, floor2*R)
+    // inner stop: R32 <= max(frac*tol, floor2*R).  frac 0.9 measured at
+    // 256x256x128x16: 825 FP32 sweeps for 30 steps against 910 at 0.25, one
+    // correction round in every step either way
+    double inner_frac_ = 0.9;
     double inner_floor2_ = 1e-10;  // (1e-5)^2: what an FP32 solve can resolve
     Profile prof;
     std::string err;

@@ -203,10 +203,16 @@ int seg4dSetSweepTile(int runs, int planes);
  * solve.  Returns the previous mode, -1 before _allocateMem_. */
 int seg4dSetPrecision(int mode);
 /* FP32-refined controls: at most max_rounds corrections per solve (default
- * 8); each FP32 solve stops once its residual is below max(tol/4,
- * inner_floor^2 * R) of the current FP64 residual R (default 1e-5).
- * Non-positive values keep the current setting. */
+ * 8); each FP32 solve stops once its residual is below max(frac * tol,
+ * inner_floor^2 * R) of the current FP64 residual R (inner_floor default
+ * 1e-5, frac: seg4dSetInnerFraction).  Non-positive values keep the current
+ * setting. */
 void seg4dSetRefinement(int max_rounds, double inner_floor);
+/* The frac above (0 < frac < 1): fraction of the outer bound at which an
+ * FP32 correction solve stops.  A larger fraction saves FP32 sweeps and
+ * risks a second correction round (the FP64 stopping test is unchanged).
+ * Default 0.9. */
+void seg4dSetInnerFraction(double frac);
 /* Correction rounds of the last FP32-refined solve (0 in FP64 mode);
  * seg4dLastSweeps then counts FP32 sweeps. */
 int seg4dLastRounds(void);
