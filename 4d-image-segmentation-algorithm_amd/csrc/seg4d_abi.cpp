@@ -548,7 +548,7 @@ API int seg4dSetWarmStart(int mode) {
         return -1;
     }
     const int old = g_eng->warm_;
-    g_eng->warm_ = (mode == 1 || mode == 2) ? mode : 0;
+    g_eng->warm_ = (mode >= 1 && mode <= 4) ? mode : 0;
     return old;
 }
 

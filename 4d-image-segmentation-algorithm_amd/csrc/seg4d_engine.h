@@ -150,9 +150,15 @@ public:
     int max_rounds_ = 8;           // FP32-refined: correction rounds per solve
     // SOR start: 0 = u^n (the reference); 1 = 2 u^n - u^(n-1) when the
     // previous call was a completed time step on the device (hist_ok_);
-    // 2 = the prediction of net_ (learned warm start).
+    // 2 = the prediction of net_ (learned warm start); 3 = 3 u^n - 3 u^(n-1)
+    // + u^(n-2) when two completed steps precede (hist2_ok_), else as 1;
+    // 4 = 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) after three, else as 3.
     int warm_ = 0;
     bool hist_ok_ = false;
+    bool hist2_ok_ = false, hist3_ok_ = false;
+    bool shifted2_ = false, shifted3_ = false;  // this step's setup filled prev2_ / prev3_
+    double* prev2_ = nullptr;      // u^(n-2) for the quadratic and cubic starts
+    double* prev3_ = nullptr;      // u^(n-3) for the cubic start
     Net* net_ = nullptr;           // not owned
     long g_version_ = 0;           // bumped whenever the g arrays change
     // inner stop: R32 <= max(frac*tol, floor2*R).  frac 0.9 measured at

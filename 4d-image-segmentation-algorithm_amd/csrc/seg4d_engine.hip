@@ -135,6 +135,10 @@ void Engine::release() {
         op32_[c] = dr_[c] = nullptr;
     }
     if (db_) hipFree(db_);
+    if (prev2_) hipFree(prev2_);
+    if (prev3_) hipFree(prev3_);
+    prev2_ = prev3_ = nullptr;
+    hist_ok_ = hist2_ok_ = hist3_ok_ = false;
     if (balls_buf_) hipFree(balls_buf_);
     balls_buf_ = nullptr;
     balls_cap_ = 0;
@@ -165,7 +169,7 @@ static Geo make_geo(int I, int J, int K, int L, int H) {
 
 int Engine::upload(ArrayId a, const double* host) {
     if (!full_[a]) return check(hipErrorInvalidValue, "upload: array not allocated");
-    if (a == A_U || a == A_PREV) hist_ok_ = false;
+    if (a == A_U || a == A_PREV) hist_ok_ = hist2_ok_ = hist3_ok_ = false;
     if (a >= A_G0 && a <= A_G7) g_version_++;
     if (check(hipMemcpyAsync(full_[a], host, (size_t)n_ * sizeof(double),
                              hipMemcpyHostToDevice, st_), "upload"))
@@ -196,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_f64(const double* __restrict__ 
 int Engine::copy(ArrayId src, ArrayId dst) {
     if (!full_[src] || !full_[dst]) return check(hipErrorInvalidValue, "copy: array not allocated");
     if (src == dst) return 0;
-    if (dst == A_U || dst == A_PREV) hist_ok_ = false;
+    if (dst == A_U || dst == A_PREV) hist_ok_ = hist2_ok_ = hist3_ok_ = false;
     if (dst >= A_G0 && dst <= A_G7) g_version_++;
     const long nb = (n_ + kBlock - 1) / kBlock;
     k_copy_f64<<<(unsigned)(nb < 65536 ? nb : 65536), kBlock, 0, st_>>>(full_[src], full_[dst], n_);
@@ -234,8 +238,30 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
         if (!prevf) return -1;
     }
     // the extrapolated and the learned start need prev_full = u^(n-1) from a
-    // completed step; without it the solve starts from u^n
-    int warm = (warm_ == 1 && hist_ok_ && write_prev) ? 1 : 0;
+    // completed step; without it the solve starts from u^n.  The quadratic
+    // and cubic starts need one / two more levels (prev2_, prev3_); with
+    // fewer, the highest order available is used.
+    int warm = 0;
+    if ((warm_ == 1 || warm_ >= 3) && hist_ok_ && write_prev) warm = 1;
+    if (warm_ >= 3 && hist_ok_ && hist2_ok_ && write_prev) warm = 3;
+    if (warm_ == 4 && hist_ok_ && hist2_ok_ && hist3_ok_ && write_prev) warm = 4;
+    double* prev2 = nullptr;
+    double* prev3 = nullptr;
+    shifted2_ = shifted3_ = false;
+    if (warm_ >= 3 && write_prev) {
+        if (!prev2_ && check(hipMalloc((void**)&prev2_, (size_t)n_ * sizeof(double)),
+                             "hipMalloc prev2"))
+            return -1;
+        prev2 = prev2_;
+        shifted2_ = hist_ok_;  // prev2 receives a valid u^(n-1)
+        if (warm_ == 4) {
+            if (!prev3_ && check(hipMalloc((void**)&prev3_, (size_t)n_ * sizeof(double)),
+                                 "hipMalloc prev3"))
+                return -1;
+            prev3 = prev3_;
+            shifted3_ = hist_ok_ && hist2_ok_;  // prev3 receives a valid u^(n-2)
+        }
+    }
     const double* startf = nullptr;
     if (warm_ == 2 && hist_ok_ && write_prev && net_ && net_->ready()) {
         double* st = scratch(A_TMP);
@@ -262,11 +288,11 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
         (void)hipEventRecord(e0, st_);
     }
     if (with_g) {
-        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, part_, warm);
-        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, part_, warm);
+        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, prev2, prev3, part_, warm);
+        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, prev2, prev3, part_, warm);
     } else {
-        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, part_, warm);
-        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, part_, warm);
+        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, prev2, prev3, part_, warm);
+        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, prev2, prev3, part_, warm);
     }
     if (launch_check("k_setup_step")) return -1;
     if (prof.on) {
@@ -625,7 +651,7 @@ int Engine::put_balls(const Balls& b, BallList* out) {
 
 int Engine::rescale(double* d, const Balls& balls, bool apply_mask) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
-    hist_ok_ = false;  // u changed outside a time step (seg_step sets it again)
+    hist_ok_ = hist2_ok_ = hist3_ok_ = false;  // u changed outside a time step
     if (balls.total > 0) {
         BallList B;
         if (put_balls(balls, &B)) return -1;
@@ -653,7 +679,7 @@ static int max_interior_dev(Engine& E, const double* d) {
 
 int Engine::threshold(const double* in, double* thr, const Balls& balls, double alpha,
                       double beta) {
-    hist_ok_ = false;
+    hist_ok_ = hist2_ok_ = hist3_ok_ = false;
     if (max_interior_dev(*this, in)) return -1;
     if (balls.total <= 0) return 0;
     BallList B;
@@ -664,7 +690,7 @@ int Engine::threshold(const double* in, double* thr, const Balls& balls, double 
 }
 
 int Engine::seed(double* d, double* mask, const Balls& balls) {
-    hist_ok_ = false;
+    hist_ok_ = hist2_ok_ = hist3_ok_ = false;
     if (balls.total <= 0) return 0;
     BallList B;
     if (put_balls(balls, &B)) return -1;
@@ -744,7 +770,9 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
         hipEventDestroy(e1);
     }
     if (check(hipStreamSynchronize(st_), "seg_step")) return -1;
-    hist_ok_ = true;  // prev_full = u^n and u = u^(n+1)
+    hist3_ok_ = shifted3_;  // prev3 = u^(n-2)
+    hist2_ok_ = shifted2_;  // prev2 = u^(n-1)
+    hist_ok_ = true;        // prev_full = u^n and u = u^(n+1)
     return 0;
 }
 
@@ -759,6 +787,8 @@ int Engine::mcf_step(const StepParams& P, double t, SolveStats* st) {
     if (solve(P, st)) return -1;
     if (unpack(nullptr)) return -1;
     if (check(hipStreamSynchronize(st_), "mcf_step")) return -1;
+    hist3_ok_ = shifted3_;
+    hist2_ok_ = shifted2_;
     hist_ok_ = true;
     return 0;
 }
@@ -814,7 +844,7 @@ int Engine::presmooth(double* d, double sigma, double h, double omega, double gs
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     const double c = sigma / (h * h);
     const double h4 = h * h * h * h;
-    hist_ok_ = false;  // prev is overwritten with the image
+    hist_ok_ = hist2_ok_ = hist3_ok_ = false;  // prev is overwritten with the image
     if (check(hipMemcpyAsync(full_[A_PREV], d, (size_t)n_ * sizeof(double),
                              hipMemcpyDeviceToDevice, st_), "presmooth copy"))
         return -1;

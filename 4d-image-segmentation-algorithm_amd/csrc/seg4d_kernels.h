@@ -191,12 +191,17 @@ __device__ __forceinline__ bool map_plane_block(const PlaneMap& M, int b, int& b
 
 // warm: 0 = SOR starts from u^n (the reference); 1 = it starts from the
 // linear extrapolation 2 u^n - u^(n-1) (prev_full holds u^(n-1) on entry);
-// 2 = from start_full (interior points; the learned start).
+// 2 = from start_full (interior points; the learned start); 3 = from the
+// quadratic extrapolation 3 u^n - 3 u^(n-1) + u^(n-2) (prev2 holds u^(n-2));
+// 4 = from the cubic one 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) (prev3
+// holds u^(n-3)).  Non-null history arrays shift by one level: prev3
+// receives u^(n-2), prev2 u^(n-1), prev_full u^n.
 template <bool WITH_G, bool POW2>
 __global__ __launch_bounds__(kBlock) void k_setup_step(
     Geo G, PlaneMap PM, Scale S, double eps2, double h4, const double* __restrict__ u,
     CPtr8 g, SplitBufs sb, double* __restrict__ prev_full,
-    const double* __restrict__ start_full, double* __restrict__ partials, int warm) {
+    const double* __restrict__ start_full, double* __restrict__ prev2,
+    double* __restrict__ prev3, double* __restrict__ partials, int warm) {
     int bx, k, l;
     if (!map_plane_block(PM, blockIdx.x, bx, k, l)) return;
     const int t = bx * kBlock + threadIdx.x;
@@ -208,8 +213,15 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
         const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
         const double uc = u[p];
         double start = uc;
-        if (warm == 1 && is_interior(G, i, j, k, l)) start = 2.0 * uc - prev_full[p];
-        if (warm == 2 && is_interior(G, i, j, k, l)) start = start_full[p];
+        const bool in = is_interior(G, i, j, k, l);
+        const double pf = (warm == 1 || warm >= 3 || prev2) ? prev_full[p] : 0.0;
+        const double p2 = prev2 ? prev2[p] : 0.0;
+        if (warm == 1 && in) start = 2.0 * uc - pf;
+        if (warm == 2 && in) start = start_full[p];
+        if (warm == 3 && in) start = (3.0 * uc - 3.0 * pf) + p2;
+        if (warm == 4 && in) start = ((4.0 * uc - 6.0 * pf) + 4.0 * p2) - prev3[p];
+        if (prev3) prev3[p] = p2;
+        if (prev2) prev2[p] = pf;
         prev_full[p] = uc;
         if (c == 0) {
             sb.ured[0][pp] = start;

@@ -23,6 +23,11 @@ Reported:
                  plus a fixed number of SOR iterations of step 1, extrapolated
                  with the GPU run's mean sweeps per step.
 
+The same 30 steps are then run in the FP32-refined form (FP32 SOR inside FP64
+defect correction, same FP64 stopping test) from u^n, from the linear and the
+cubic extrapolation of the previous solutions, and from the learned network's
+prediction, each reported with its time, sweeps, max|u - u_FP64| and mask IoU.
+
 Usage: python bench.py [--steps 30] [--warmup 1] [--no-cpu] [--cpu-iters 10]
 """
 from __future__ import annotations
@@ -208,8 +213,10 @@ def main():
 
     def run_refined(warm):
         """The same time steps in the FP32-refined form from the same u0;
-        warm = 1 starts every SOR solve from 2 u^n - u^(n-1), warm = 2 from
-        the learned network's prediction (both from the second step on)."""
+        warm = 1 starts every SOR solve from 2 u^n - u^(n-1), 4 from the cubic
+        extrapolation 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) (lower orders
+        while the history is shorter), 2 from the learned network's
+        prediction (from the second step on)."""
         u64 = inner.copy()
         lib.set_precision(1)
         lib.set_warm_start(warm)
@@ -245,7 +252,8 @@ def main():
         union = np.logical_or(mask64, maskm).sum()
         r32 = mprof["red1_f32_ms"] / max(mprof["red1_f32_launches"], 1)
         res = {
-            "warm_start": {0: "u^n", 1: "2u^n - u^(n-1)", 2: "learned"}[warm],
+            "warm_start": {0: "u^n", 1: "2u^n - u^(n-1)", 2: "learned",
+                           4: "4u^n - 6u^(n-1) + 4u^(n-2) - u^(n-3)"}[warm],
             "time_to_tolerance_s": t3 - t2,
             "ms_per_step": (t3 - t2) / args.steps * 1e3,
             "speedup_vs_fp64": total / (t3 - t2),
@@ -266,10 +274,11 @@ def main():
             f"IoU {res['mask_iou_vs_fp64_final']:.5f}")
         return res
 
-    mixed = mixed_warm = learned = None
+    mixed = mixed_warm = mixed_cubic = learned = None
     if not args.no_mixed:
         mixed = run_refined(0)
         mixed_warm = run_refined(1)
+        mixed_cubic = run_refined(4)
         if os.path.exists(net_path):
             lib.net_load(net_path)
             learned = run_refined(2)
@@ -334,6 +343,7 @@ def main():
         "mask_fraction": mask_frac,
         "fp32_refined": mixed,
         "fp32_refined_extrapolated_start": mixed_warm,
+        "fp32_refined_cubic_start": mixed_cubic,
         "fp32_refined_learned_start": learned,
         "cpu_baseline": cpu,
     }

@@ -220,7 +220,11 @@ int seg4dLastRounds(void);
  * linear extrapolation 2 u^n - u^(n-1), used when the previous call was a
  * completed _segTimeStep_ whose state stayed on the device (resident mode).
  * 2: the prediction of the learned network (seg4dNet*), under the same
- * condition as 1 (otherwise the solve starts from u^n).  In every mode
+ * condition as 1 (otherwise the solve starts from u^n).  3: the quadratic
+ * extrapolation 3 u^n - 3 u^(n-1) + u^(n-2) after two such steps (after
+ * one, as 1; one more padded FP64 array on the device).  4: the cubic one
+ * 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) after three (else as 3; two more
+ * arrays).  In every mode
  * the system solved, its right-hand side and the stopping test are
  * unchanged; only the number of sweeps (and the last bits of u) differ.
  * Returns the previous mode, -1 before _allocateMem_ or (mode 2) without a

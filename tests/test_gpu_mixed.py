@@ -73,10 +73,11 @@ def test_fp32_refined_vs_oracle(seglib, oracle, dims):
         lib.deallocate()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_extrapolated_start_vs_oracle(seglib, oracle, precision):
-    """SOR started from 2 u^n - u^(n-1) (resident mode): same stopping test,
-    trajectory within the stated error of the FP64 oracle, fewer sweeps."""
+@pytest.mark.parametrize("precision,warm", [(0, 1), (1, 1), (0, 3), (1, 3), (0, 4), (1, 4)])
+def test_extrapolated_start_vs_oracle(seglib, oracle, precision, warm):
+    """SOR started from 2 u^n - u^(n-1) (warm 1) or 3 u^n - 3 u^(n-1) +
+    u^(n-2) (warm 3) in resident mode: same stopping test, trajectory within
+    the stated error of the FP64 oracle, fewer sweeps."""
     lib = seglib
     dims = (40, 36, 20, 6)
     shape, img, u0 = _volume(lib, dims, 7)
@@ -94,14 +95,14 @@ def test_extrapolated_start_vs_oracle(seglib, oracle, precision):
         lib.upload("u")
         lib.upload("Mmask")
         lib.set_precision(precision)
-        lib.set_warm_start(1)
+        lib.set_warm_start(warm)
         u, prev, coef = u0.copy(), np.zeros(shape), zeros(shape, 8)
-        cold, warm = [], []
+        cold, zs = [], []
         for st in range(6):
             p = params(shape, gsTol=REL * REL * oracle.sumsq_interior(params(shape, **KW), u), **KW)
             cold.append(oracle.seg_time_step(p, u, prev, g, coef, np.ones(shape))[0])
             z, R = lib.seg_time_step()
-            warm.append(z)
+            zs.append(z)
             assert R <= lib.lib.seg4dLastTolerance()
         lib.download("u")
         ug = lib.arr("u")
@@ -109,9 +110,10 @@ def test_extrapolated_start_vs_oracle(seglib, oracle, precision):
         iou = oracle.iou(params(shape), oracle.mask(params(shape), u),
                          oracle.mask(params(shape), np.ascontiguousarray(ug)))
         assert iou >= 0.999
-        print(f"  precision {precision}: oracle sweeps {cold}, extrapolated start {warm}", flush=True)
+        print(f"  precision {precision}, warm {warm}: oracle sweeps {cold}, extrapolated start {zs}",
+              flush=True)
         if precision == 0:
-            assert sum(warm[2:]) < sum(cold[2:])
+            assert sum(zs[2:]) < sum(cold[2:])
     finally:
         lib.set_warm_start(0)
         lib.set_precision(0)
