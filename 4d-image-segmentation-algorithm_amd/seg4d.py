@@ -151,6 +151,8 @@ class SegLib(_Lib):
         L.seg4dProfileRead.argtypes = [POINTER(c_double), c_int]
         L.seg4dSetSweepVariant.restype = c_int
         L.seg4dSetSweepVariant.argtypes = [c_int]
+        L.seg4dSetSweepTile.restype = c_int
+        L.seg4dSetSweepTile.argtypes = [c_int, c_int]
         L.seg4dSetPrecision.restype = c_int
         L.seg4dSetPrecision.argtypes = [c_int]
         L.seg4dSetRefinement.restype = None
