@@ -689,6 +689,20 @@ API int seg4dSetSweepVariant(int v) {
     return old;
 }
 
+API int seg4dSetSweepChunk(int lsteps) {
+    if (!g_eng) return -1;
+    const int old = g_eng->sweep_lc_;
+    g_eng->sweep_lc_ = lsteps > 0 ? lsteps : 0;
+    return old;
+}
+
+API int seg4dSetPipelineDepth(int depth) {
+    if (!g_eng) return -1;
+    const int old = g_eng->pipe_depth_;
+    g_eng->pipe_depth_ = (depth > 0 && depth <= 16) ? depth : 0;
+    return old;
+}
+
 API int seg4dSetSweepTile(int runs, int planes) {
     if (!g_eng) return -1;
     if (runs < 1 || planes < 1) return -1;

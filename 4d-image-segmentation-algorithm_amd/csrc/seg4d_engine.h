@@ -144,6 +144,9 @@ public:
     // fastest at 256x256x128x16 (8 x 32: 1.401 ms, 16 x 16: 1.352 ms, 16 x 8:
     // 1.347 ms per red pass)
     int sweep_tb_ = 16, sweep_tk_ = 8;
+    int sweep_lc_ = 0;     // l steps per block of a colour pass; 0 = automatic
+    int pipe_depth_ = 0;   // iterations enqueued ahead of the stopping test; 0 = automatic
+    std::vector<hipEvent_t> evq_;  // one event per queued iteration
     // 0 = FP64 (the reference's arithmetic); 1 = FP32-refined: FP32 SOR on
     // the correction equation inside FP64 defect correction.
     int precision_ = 0;

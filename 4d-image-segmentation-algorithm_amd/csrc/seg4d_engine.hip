@@ -92,7 +92,9 @@ int Engine::init(int I, int J, int K, int L, bool seg, bool mcf) {
     if (alloc(&ub_, np_)) return -1;
     // partial-sum capacity: the largest grid of any reducing kernel
     const long g_full = (long)cdiv((long)I * J, kBlock) * K * L;
-    const long g_sweep = (long)cdiv((long)H_ * (J - 2), kBlock) * (K - 2);
+    // (a SOR pass has at most cdiv(points per row plane, kBlock / 4) runs
+    // per k plane and one frame per block)
+    const long g_sweep = (long)cdiv((long)H_ * (J - 2), kBlock / 4) * (K - 2) * (L - 2);
     long np = g_full > g_sweep ? g_full : g_sweep;
     if (np < 4096) np = 4096;
     npart_ = np;
@@ -149,6 +151,8 @@ void Engine::release() {
     ctl2_ = hctl2_ = nullptr;
     ctl_ = hctl_ = nullptr;
     for (auto e : pev_) hipEventDestroy(e);
+    for (auto e : evq_) hipEventDestroy(e);
+    evq_.clear();
     pev_.clear();
     if (ev_) hipEventDestroy(ev_);
     ev_ = nullptr;
@@ -314,7 +318,21 @@ SweepMap Engine::sweep_map() const {
     SweepMap M;
     M.kp = (sweep_variant_ & 16) ? 4 : (sweep_variant_ & 8) ? 2 : 1;
     M.gx = (int)cdiv((long)H_ * (J_ - 2), kBlock / M.kp);
-    M.gk = (int)cdiv((long)(K_ - 2), M.kp);
+    M.gk0 = (int)cdiv((long)(K_ - 2), M.kp);
+    // l chunks: a volume with fewer than ~4096 blocks of whole l-marches
+    // splits its frames so that the launch fills the GPU (even chunk lengths
+    // keep the two-step load phases whole)
+    const int nl = L_ - 2;
+    int lc = sweep_lc_;
+    if (lc <= 0) {
+        const long b0 = (long)M.gx * M.gk0;
+        const long want = b0 >= 4096 ? 1 : cdiv(4096, b0);
+        lc = (int)cdiv(nl, want);
+        lc = lc < 2 ? 2 : lc + (lc & 1);
+    }
+    if (lc > nl) lc = nl;
+    M.lc = lc;
+    M.gk = M.gk0 * (int)cdiv(nl, lc);
     M.tiled = (sweep_variant_ & 2) ? 1 : 0;
     M.tb = sweep_tb_;
     M.tk = sweep_tk_;
@@ -345,13 +363,18 @@ int sor_run(Engine& E, const SorJob<T>& J, int* zout) {
     const Geo G = make_geo(E.I_, E.J_, E.K_, E.L_, E.H_);
     const SweepMap M = E.sweep_map();
     const int nb = M.gx * M.gk;
+    if ((long)nb > E.npart_) {
+        std::fprintf(stderr, "seg4d: SOR grid of %d blocks exceeds the partial buffer\n", nb);
+        return -1;
+    }
     const dim3 grid(M.tiled ? (unsigned)(((M.ntiles + 7) / 8) * 8 * M.tb * M.tk) : (unsigned)nb);
     double* part_r = E.part_;
     double* part_b = E.part_ + E.npart_;
     hipStream_t st = E.st_;
     int pe = 0;
     std::vector<int> kinds;
-    auto ev_pair = [&](int kind) -> hipEvent_t* {
+    std::vector<int> iters;  // iteration index of each timed pass
+    auto ev_pair = [&](int kind, int iter) -> hipEvent_t* {
         while ((int)E.pev_.size() < pe + 2) {
             hipEvent_t e;
             if (hipEventCreate(&e) != hipSuccess) return nullptr;
@@ -360,6 +383,7 @@ int sor_run(Engine& E, const SorJob<T>& J, int* zout) {
         hipEvent_t* r = &E.pev_[pe];
         pe += 2;
         kinds.push_back(kind);
+        iters.push_back(iter);
         return r;
     };
     const bool u2 = (E.sweep_variant_ & 1) != 0;
@@ -383,43 +407,67 @@ int sor_run(Engine& E, const SorJob<T>& J, int* zout) {
 #undef S4D_PASS
     };
     auto red = [&](int z, bool resid) -> int {
-        hipEvent_t* e = E.prof.on ? ev_pair(J.kind0 + (resid ? 1 : 0)) : nullptr;
+        hipEvent_t* e = E.prof.on ? ev_pair(J.kind0 + (resid ? 1 : 0), z) : nullptr;
         if (e) (void)hipEventRecord(e[0], st);
         pass(resid ? 1 : 0, 0, J.red[(z - 1) & 1], J.red[z & 1], J.black, J.opnd[0],
-             resid ? part_r : nullptr, nullptr);
+             resid ? part_r : nullptr, J.ctl);
         if (e) (void)hipEventRecord(e[1], st);
         return E.launch_check("k_sor_sweep red");
     };
     auto black = [&](int z) -> int {
-        hipEvent_t* e = E.prof.on ? ev_pair(J.kind0 + 2) : nullptr;
+        hipEvent_t* e = E.prof.on ? ev_pair(J.kind0 + 2, z) : nullptr;
         if (e) (void)hipEventRecord(e[0], st);
         pass(2, 1, J.black, J.black, J.red[z & 1], J.opnd[1], part_b, J.ctl);
         if (e) (void)hipEventRecord(e[1], st);
         return E.launch_check("k_sor_sweep black");
     };
     if (red(1, false) || black(1)) return -1;
-    int z = 1;
+    // Iterations are enqueued up to `depth` ahead of the stopping decision the
+    // host has seen; after a stop every queued pass and reduce is a no-op (they
+    // read the device flag), so a deeper queue only hides the host round trip.
+    const int nred_pts = (int)std::min<long>(E.prof.nred > 0 ? E.prof.nred : 1, 1L << 30);
+    const int depth = E.pipe_depth_ > 0 ? E.pipe_depth_ : (nred_pts < (4 << 20) ? 4 : 1);
+    while ((int)E.evq_.size() < depth) {
+        hipEvent_t e;
+        if (E.check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event")) return -1;
+        E.evq_.push_back(e);
+    }
+    int z = 1;       // next iteration whose residual gets reduced
+    int queued = 0;  // reduced iterations the host has not looked at yet
     for (;;) {
-        if (red(z + 1, true)) return -1;
-        k_reduce_resid<<<1, 1024, 0, st>>>(part_r, nb, part_b, nb, J.ctl, nullptr, z, J.cap);
-        if (E.launch_check("k_reduce_resid")) return -1;
-        if (E.check(hipMemcpyAsync(J.hctl, J.ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, st),
-                    "ctl copy"))
-            return -1;
-        if (E.check(hipEventRecord(E.ev_, st), "event")) return -1;
-        if (black(z + 1)) return -1;  // becomes a no-op if iteration z stopped
+        while (queued < depth) {
+            if (red(z + 1, true)) return -1;
+            k_reduce_resid<<<1, 1024, 0, st>>>(part_r, nb, part_b, nb, J.ctl, nullptr, z, J.cap,
+                                               1);
+            if (E.launch_check("k_reduce_resid")) return -1;
+            if (E.check(hipMemcpyAsync(J.hctl, J.ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, st),
+                        "ctl copy"))
+                return -1;
+            if (E.check(hipEventRecord(E.evq_[z % depth], st), "event")) return -1;
+            if (black(z + 1)) return -1;  // a no-op once iteration z stopped
+            z++;
+            queued++;
+        }
+        const int zo = z - queued;  // oldest iteration not looked at
         hipError_t q;
-        while ((q = hipEventQuery(E.ev_)) == hipErrorNotReady) {
+        while ((q = hipEventQuery(E.evq_[zo % depth])) == hipErrorNotReady) {
         }
         if (E.check(q, "solve wait")) return -1;
+        queued--;
+        // the host copy holds the state after reduce(zo) or a later one
         if (J.hctl->stop) break;
-        z++;
     }
+    if (E.check(hipStreamSynchronize(st), "solve sync")) return -1;
+    z = J.hctl->zfinal;
     if (E.prof.on) {
-        if (E.check(hipStreamSynchronize(st), "prof sync")) return -1;
         for (int q = 0; q < pe; q += 2) {
             float ms = 0.f;
             (void)hipEventElapsedTime(&ms, E.pev_[q], E.pev_[q + 1]);
+            // passes enqueued after the stop returned at once: not counted
+            // (red(z') yields the residual of z' - 1, black(z') is part of z')
+            const int kd = kinds[q / 2] - J.kind0, it = iters[q / 2];
+            const bool real = (kd == 2) ? it <= z : it <= z + 1;
+            if (!real) continue;
             E.prof.sweep_ms[kinds[q / 2]] += ms;
             E.prof.sweep_launches[kinds[q / 2]]++;
         }
@@ -505,7 +553,7 @@ int Engine::defect64(const StepParams& P) {
     if (prof.on) (void)hipEventRecord(e1, st_);
     // z = 1, cap = 1: the reduction always records R and raises stop; the
     // outer decision is taken on the host from R and tol
-    k_reduce_resid<<<1, 1024, 0, st_>>>(part_r, nb, part_b, nb, ctl_, nullptr, 1, 1);
+    k_reduce_resid<<<1, 1024, 0, st_>>>(part_r, nb, part_b, nb, ctl_, nullptr, 1, 1, 0);
     if (launch_check("k_reduce_resid")) return -1;
     if (check(hipMemcpyAsync(hctl_, ctl_, sizeof(SorCtl), hipMemcpyDeviceToHost, st_), "ctl copy"))
         return -1;

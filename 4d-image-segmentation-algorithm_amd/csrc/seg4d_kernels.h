@@ -276,9 +276,11 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
 //          k+-1 planes are co-resident on one XCD (one L2) and march l in step.
 struct SweepMap {
     int gx, gk;      // block grid: runs of kBlock / kp points per row plane,
-                     // groups of kp consecutive interior k planes
+                     // (groups of kp consecutive interior k planes) x (l chunks)
     int kp;          // k planes per block (1, 2, 4): the block's planes share
                      // their other-colour neighbour planes through L1/L2
+    int gk0, lc;     // k groups; l steps per block (interior frames split into
+                     // gk / gk0 chunks of lc, so that small volumes fill the GPU)
     int tiled;
     int tb, tk;      // tile shape; tb * tk blocks per tile
     int ntb, ntiles;
@@ -419,11 +421,14 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(
     int bx, kb;
     if (!map_block(M, blockIdx.x, gridDim.x, bx, kb)) return;
     const int lb = (kb - 1) * M.gx + bx;
-    // the block's kp planes: threads [sub * per, (sub + 1) * per) take plane
-    // (kb - 1) kp + sub + 1, run bx of `per` points
+    // block index kb - 1 = l chunk * gk0 + k group; the block's kp planes:
+    // threads [sub * per, (sub + 1) * per) take plane kgrp kp + sub + 1, run bx
+    // of `per` points, frames l0 .. l1
+    const int kgrp = (kb - 1) % M.gk0, lch = (kb - 1) / M.gk0;
+    const int l0 = 1 + lch * M.lc;
     const int per = kBlock / M.kp;
     const int sub = threadIdx.x / per;
-    const int kk = (kb - 1) * M.kp + sub + 1;
+    const int kk = kgrp * M.kp + sub + 1;
     const bool kok = kk <= G.K - 2;
     const int k = kok ? kk : G.K - 2;  // indices stay in range for idle threads
     const int t = bx * per + (threadIdx.x - sub * per);
@@ -440,10 +445,10 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(
     float* def0 = (RESID == 3) ? defect_ob + ob_off + 8 * 64 : nullptr;
     // window w[q] = other colour at column m, row l-1+q
     T w[U + 2];
-    w[0] = rowok ? y[base] : T(0);
-    w[1] = rowok ? y[base + G.pw] : T(0);
-    const int lmax = G.L - 2;
-    int l = 1;
+    w[0] = rowok ? y[base + (l0 - 1) * G.pw] : T(0);
+    w[1] = rowok ? y[base + l0 * G.pw] : T(0);
+    const int lmax = min(l0 + M.lc - 1, G.L - 2);
+    int l = l0;
     for (; l + U - 1 <= lmax; l += U) {
         PointIn<T> P[U];
 #pragma unroll
@@ -507,7 +512,10 @@ __global__ void k_inner_ctl(const SorCtl* __restrict__ outer, SorCtl* inner, dou
 // test `while (R > gsTol && z < maxNoGSIteration)` (segmentation4D.c:309).
 __global__ __launch_bounds__(1024) void k_reduce_resid(
     const double* __restrict__ pa, int na, const double* __restrict__ pb, int nbp,
-    SorCtl* ctl, double* __restrict__ hist, int z, int cap) {
+    SorCtl* ctl, double* __restrict__ hist, int z, int cap, int queued) {
+    // a reduce enqueued ahead of the host's decision (queued != 0) keeps a
+    // stopped state: the passes after the stop left stale partials
+    if (queued && ctl->stop) return;
     double s = 0.0;
     for (int q = threadIdx.x; q < na; q += 1024) s += pa[q];
     double s2 = 0.0;

@@ -153,6 +153,9 @@ class SegLib(_Lib):
         L.seg4dSetSweepVariant.argtypes = [c_int]
         L.seg4dSetSweepTile.restype = c_int
         L.seg4dSetSweepTile.argtypes = [c_int, c_int]
+        for fn in ("seg4dSetSweepChunk", "seg4dSetPipelineDepth"):
+            getattr(L, fn).restype = c_int
+            getattr(L, fn).argtypes = [c_int]
         L.seg4dSetPrecision.restype = c_int
         L.seg4dSetPrecision.argtypes = [c_int]
         L.seg4dSetRefinement.restype = None

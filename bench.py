@@ -54,15 +54,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def load_traffic():
+def load_traffic(dims):
     """HBM bytes per launch of k_sor_sweep<1> measured by rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), as committed in
-    profiles/sor_sweep_pmc.md; None if that summary is absent."""
+    profiles/sor_sweep_pmc.md; None if that summary is absent or was measured
+    on another volume than dims = (nx, ny, nz, nt)."""
     path = os.path.join(ROOT, "profiles", "sor_sweep_pmc.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read())
-    return float(m.group(1)) if m else None
+    text = open(path).read()
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", text)
+    v = re.search(r"traffic_volume\s*=\s*(\d+)x(\d+)x(\d+)x(\d+)", text)
+    if not m or not v or tuple(int(x) for x in v.groups()) != tuple(dims):
+        return None
+    return float(m.group(1))
 
 
 def params(lib):
@@ -292,7 +297,7 @@ def main():
     sweep_ms = prof["red0_ms"] + prof["red1_ms"] + prof["black_ms"]
     it = prof["iterations"]
     sweep_bytes = BYTES_PER_UPDATE * ((it + args.steps) * nred + it * nblack)
-    traffic = load_traffic()
+    traffic = load_traffic((nx, ny, nz, nt))
     value = prof["voxel_updates"] / total
 
     cpu = None

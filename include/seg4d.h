@@ -194,6 +194,16 @@ int seg4dSetSweepVariant(int v);
 /* Tile of the tiled launch order in blocks: `runs` runs x `planes` k-plane
  * groups per XCD residency round (default 16 x 8).  0 on success. */
 int seg4dSetSweepTile(int runs, int planes);
+/* Frames per block of a SOR colour pass (tuning knob, identical results):
+ * 0 (default) = all frames unless the volume is too small to fill the GPU,
+ * then chunks of an even length; n > 0 = chunks of n frames.  Returns the
+ * previous value. */
+int seg4dSetSweepChunk(int lsteps);
+/* SOR iterations enqueued ahead of the stopping test the host has seen
+ * (1..16; 0 = automatic: 4 for volumes under 4M points per colour, else 1).
+ * Passes enqueued after the stop are no-ops; results are identical.
+ * Returns the previous value. */
+int seg4dSetPipelineDepth(int depth);
 /* Arithmetic of the SOR solve.  0 (default): FP64, the reference's
  * arithmetic and bits.  1: FP32-refined -- FP64 defect correction in which
  * the defect b - A u of the FP64 system is recomputed in FP64 on the GPU and

@@ -162,12 +162,17 @@ def test_seg_random_vs_oracle(seglib, oracle, dims):
         lib.deallocate()
 
 
-@pytest.mark.parametrize("sweeps,variant", [(1, 0), (2, 0), (7, 0), (7, 1), (7, 2), (7, 3),
-                                            (7, 7), (7, 11), (7, 15), (7, 19), (7, 23), (7, 9)])
-def test_fixed_sweep_count(seglib, oracle, sweeps, variant):
+@pytest.mark.parametrize("sweeps,variant,chunk,depth",
+                         [(1, 0, 0, 0), (2, 0, 0, 0), (7, 0, 0, 0), (7, 1, 0, 0), (7, 2, 0, 0),
+                          (7, 3, 0, 0), (7, 7, 0, 0), (7, 11, 0, 0), (7, 15, 0, 0), (7, 19, 0, 0),
+                          (7, 23, 0, 0), (7, 9, 0, 0), (7, 3, 100, 1), (7, 3, 2, 4), (7, 3, 3, 2),
+                          (7, 2, 1, 16), (5, 11, 2, 3)])
+def test_fixed_sweep_count(seglib, oracle, sweeps, variant, chunk, depth):
     """gsTol < 0 forces exactly maxNoGSIteration sweeps (sweep cap path);
     every SOR pass kernel variant gives the same bits (bits 8 / 16: two /
-    four k planes per block; nz = 9 leaves the last block partly idle)."""
+    four k planes per block; nz = 9 leaves the last block partly idle), so
+    do frame chunks (0: automatic, 100: one chunk) and solve pipelines of
+    any depth (0: automatic)."""
     nx, ny, nz, nt = 12, 10, 9, 4
     shape = (nt + 2, nz + 2, ny + 2, nx + 2)
     rng = np.random.default_rng(sweeps)
@@ -178,6 +183,8 @@ def test_fixed_sweep_count(seglib, oracle, sweeps, variant):
     lib = seglib
     lib.allocate(nx, ny, nz, nt)
     lib.set_sweep_variant(variant)
+    lib.lib.seg4dSetSweepChunk(chunk)
+    lib.lib.seg4dSetPipelineDepth(depth)
     try:
         lib.set(gsTol=-1.0, **kw)
         for f, gg in zip(FACES, gs):
@@ -194,9 +201,12 @@ def test_fixed_sweep_count(seglib, oracle, sweeps, variant):
         lib.deallocate()
 
 
-@pytest.mark.parametrize("variant,dims", [(11, (17, 11, 7, 3)), (19, (17, 11, 7, 3)),
-                                          (10, (8, 6, 5, 4)), (15, (30, 9, 6, 5))])
-def test_tolerance_driven_variants(seglib, oracle, variant, dims):
+@pytest.mark.parametrize("variant,dims,chunk,depth",
+                         [(11, (17, 11, 7, 3), 0, 0), (19, (17, 11, 7, 3), 0, 0),
+                          (10, (8, 6, 5, 4), 0, 0), (15, (30, 9, 6, 5), 0, 0),
+                          (3, (31, 10, 7, 6), 100, 1), (3, (31, 10, 7, 6), 2, 5),
+                          (3, (31, 10, 7, 6), 4, 2)])
+def test_tolerance_driven_variants(seglib, oracle, variant, dims, chunk, depth):
     """Tolerance-driven steps (the fused residual and its per-block partials)
     with several k planes per block: same sweep count and bits as the oracle."""
     nx, ny, nz, nt = dims
@@ -209,6 +219,8 @@ def test_tolerance_driven_variants(seglib, oracle, variant, dims):
     lib = seglib
     lib.allocate(nx, ny, nz, nt)
     lib.set_sweep_variant(variant)
+    lib.lib.seg4dSetSweepChunk(chunk)
+    lib.lib.seg4dSetPipelineDepth(depth)
     try:
         lib.set(gsTol=1e-9, **kw)
         for f, gg in zip(FACES, gs):
