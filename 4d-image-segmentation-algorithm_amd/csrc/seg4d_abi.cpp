@@ -539,6 +539,13 @@ API void seg4dSetInnerFraction(double frac) {
     if (g_eng && frac > 0.0 && frac < 1.0) g_eng->inner_frac_ = frac;
 }
 
+API double seg4dSetInnerOmega(double omega) {
+    if (!g_eng) return -1.0;
+    const double old = g_eng->inner_omega_;
+    if (omega == 0.0 || (omega > 0.0 && omega < 2.0)) g_eng->inner_omega_ = omega;
+    return old;
+}
+
 API int seg4dLastRounds(void) { return g_last.rounds; }
 
 API int seg4dSetWarmStart(int mode) {

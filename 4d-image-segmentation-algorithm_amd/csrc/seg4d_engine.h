@@ -168,6 +168,7 @@ public:
     // 256x256x128x16: 825 FP32 sweeps for 30 steps against 910 at 0.25, one
     // correction round in every step either way
     double inner_frac_ = 0.9;
+    double inner_omega_ = 1.2;     // FP32 correction solve's SOR factor (0: omega_c)
     double inner_floor2_ = 1e-10;  // (1e-5)^2: what an FP32 solve can resolve
     Profile prof;
     std::string err;

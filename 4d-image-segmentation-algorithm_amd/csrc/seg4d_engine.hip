@@ -578,7 +578,7 @@ int Engine::solve_mixed(const StepParams& P, SolveStats* out) {
     const long nint = (long)(I_ - 2) * (J_ - 2) * (K_ - 2) * (L_ - 2);
     SorJob<float> J;
     J.c = (float)(P.tau / (P.h * P.h));
-    J.om = (float)P.omega;
+    J.om = (float)(inner_omega_ > 0.0 ? inner_omega_ : P.omega);
     J.h4 = P.h * P.h * P.h * P.h;
     J.opnd[0] = op32_[0];
     J.opnd[1] = op32_[1];

@@ -153,6 +153,8 @@ class SegLib(_Lib):
         L.seg4dSetSweepVariant.argtypes = [c_int]
         L.seg4dSetSweepTile.restype = c_int
         L.seg4dSetSweepTile.argtypes = [c_int, c_int]
+        L.seg4dSetInnerOmega.restype = c_double
+        L.seg4dSetInnerOmega.argtypes = [c_double]
         for fn in ("seg4dSetSweepChunk", "seg4dSetPipelineDepth"):
             getattr(L, fn).restype = c_int
             getattr(L, fn).argtypes = [c_int]

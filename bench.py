@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--net-out", default=None)
     ap.add_argument("--train-iters", type=int, default=2500)
     ap.add_argument("--train-runs", type=int, default=8)
+    ap.add_argument("--inner-omega", type=float, default=None,
+                    help="SOR factor of the FP32 correction solves (default: omega_c)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one device copy of u (padded volume, "
                          "FP64) as the known-size reference for rocprofv3 --pmc byte counts")
@@ -225,6 +227,8 @@ def main():
         u64 = inner.copy()
         lib.set_precision(1)
         lib.set_warm_start(warm)
+        om_keep = lib.lib.seg4dSetInnerOmega(
+            args.inner_omega if args.inner_omega is not None else -1.0)
         lib.arr("u")[...] = u0
         lib.upload("u")
         lib.set(numberOfTimeStep=1)
@@ -249,6 +253,7 @@ def main():
         pms1, pn1 = lib.net_predict_stats()
         lib.set_precision(0)
         lib.set_warm_start(0)
+        inner_om = lib.lib.seg4dSetInnerOmega(om_keep)
         lib.download("u")
         um = lib.arr("u")[1:-1, 1:-1, 1:-1, 1:-1]
         thm = 0.5 * (um.max() + um.min())
@@ -259,6 +264,7 @@ def main():
         res = {
             "warm_start": {0: "u^n", 1: "2u^n - u^(n-1)", 2: "learned",
                            4: "4u^n - 6u^(n-1) + 4u^(n-2) - u^(n-3)"}[warm],
+            "inner_omega": inner_om if inner_om > 0 else "omega_c",
             "time_to_tolerance_s": t3 - t2,
             "ms_per_step": (t3 - t2) / args.steps * 1e3,
             "speedup_vs_fp64": total / (t3 - t2),

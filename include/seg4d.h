@@ -223,6 +223,15 @@ void seg4dSetRefinement(int max_rounds, double inner_floor);
  * risks a second correction round (the FP64 stopping test is unchanged).
  * Default 0.9. */
 void seg4dSetInnerFraction(double frac);
+/* Relaxation factor of the FP32 correction solves (0 < omega < 2; 0 uses
+ * omega_c, as the reference's solve does).  The correction solve only has to
+ * reduce the defect: any convergent factor gives a result that meets the
+ * same FP64 stopping test.  Default 1.2: omega_c = 1.8 over-relaxes this
+ * system (the error of an SOR sweep at 1.8 contracts by about 0.8), and 1.2
+ * took the fewest sweeps on the 64x64x32x8 and 256x256x128x16 workloads.
+ * Other values are ignored, so a negative argument queries the setting.
+ * Returns the previous setting. */
+double seg4dSetInnerOmega(double omega);
 /* Correction rounds of the last FP32-refined solve (0 in FP64 mode);
  * seg4dLastSweeps then counts FP32 sweeps. */
 int seg4dLastRounds(void);

@@ -29,12 +29,17 @@ def _volume(lib, dims, seed):
     return shape, img, u0
 
 
-@pytest.mark.parametrize("dims", [(40, 36, 20, 6), (33, 17, 12, 5)])
-def test_fp32_refined_vs_oracle(seglib, oracle, dims):
-    print(f"[gpu] fp32-refined {dims}", flush=True)
+@pytest.mark.parametrize("dims,inner_omega", [((40, 36, 20, 6), -1.0), ((33, 17, 12, 5), -1.0),
+                                              ((40, 36, 20, 6), 0.0), ((33, 17, 12, 5), 1.5)])
+def test_fp32_refined_vs_oracle(seglib, oracle, dims, inner_omega):
+    """inner_omega: SOR factor of the correction solves (-1 keeps the
+    default 1.2, 0 takes omega_c = 1.8)."""
+    print(f"[gpu] fp32-refined {dims} inner omega {inner_omega}", flush=True)
     lib = seglib
     shape, img, u0 = _volume(lib, dims, 3)
     lib.allocate(*dims)
+    om_keep = lib.lib.seg4dSetInnerOmega(inner_omega)
+    assert om_keep == 1.2
     try:
         lib.set(gsTol=0.0, **KW)
         lib.set_relative_tolerance(REL)
@@ -68,6 +73,7 @@ def test_fp32_refined_vs_oracle(seglib, oracle, dims):
             print(f"  step {st + 1}: fp32 sweeps {z}, rounds {rounds}, max|du| {err:.2e}, IoU {iou}",
                   flush=True)
     finally:
+        lib.lib.seg4dSetInnerOmega(om_keep)
         lib.set_precision(0)
         lib.set_relative_tolerance(0.0)
         lib.deallocate()
