@@ -232,7 +232,6 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
     SplitBufs sb;
     for (int c = 0; c < 2; c++) {
         sb.opnd[c] = op_[c];
-        sb.opnd32[c] = precision_ == 1 ? op32_[c] : nullptr;
         sb.ured[c] = ur_[c];
     }
     sb.ublack = ub_;
@@ -249,21 +248,28 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
     if ((warm_ == 1 || warm_ >= 3) && hist_ok_ && write_prev) warm = 1;
     if (warm_ >= 3 && hist_ok_ && hist2_ok_ && write_prev) warm = 3;
     if (warm_ == 4 && hist_ok_ && hist2_ok_ && hist3_ok_ && write_prev) warm = 4;
-    double* prev2 = nullptr;
-    double* prev3 = nullptr;
+    // History ring: prev_full (u^(n-1)), prev2_ (u^(n-2)), prev3_ (u^(n-3)).
+    // u^n is written over the oldest level the mode keeps, and the pointers
+    // rotate after the launch (prev_full = u^n, prev2_ = u^(n-1), ...), so no
+    // level is copied.
+    const double* h2 = nullptr;
+    const double* h3 = nullptr;
+    double* pout = prevf;
     shifted2_ = shifted3_ = false;
     if (warm_ >= 3 && write_prev) {
         if (!prev2_ && check(hipMalloc((void**)&prev2_, (size_t)n_ * sizeof(double)),
                              "hipMalloc prev2"))
             return -1;
-        prev2 = prev2_;
-        shifted2_ = hist_ok_;  // prev2 receives a valid u^(n-1)
+        h2 = prev2_;
+        pout = prev2_;
+        shifted2_ = hist_ok_;  // prev2 becomes a valid u^(n-1)
         if (warm_ == 4) {
             if (!prev3_ && check(hipMalloc((void**)&prev3_, (size_t)n_ * sizeof(double)),
                                  "hipMalloc prev3"))
                 return -1;
-            prev3 = prev3_;
-            shifted3_ = hist_ok_ && hist2_ok_;  // prev3 receives a valid u^(n-2)
+            h3 = prev3_;
+            pout = prev3_;
+            shifted3_ = hist_ok_ && hist2_ok_;  // prev3 becomes a valid u^(n-2)
         }
     }
     const double* startf = nullptr;
@@ -292,13 +298,23 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
         (void)hipEventRecord(e0, st_);
     }
     if (with_g) {
-        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, prev2, prev3, part_, warm);
-        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, prev2, prev3, part_, warm);
+        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm);
+        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm);
     } else {
-        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, prev2, prev3, part_, warm);
-        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, startf, prev2, prev3, part_, warm);
+        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm);
+        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm);
     }
     if (launch_check("k_setup_step")) return -1;
+    if (pout == prev3_ && prev3_) {
+        double* o1 = full_[A_PREV];
+        double* o2 = prev2_;
+        full_[A_PREV] = prev3_;
+        prev2_ = o1;
+        prev3_ = o2;
+    } else if (pout == prev2_ && prev2_) {
+        prev2_ = full_[A_PREV];
+        full_[A_PREV] = pout;
+    }
     if (prof.on) {
         hipEventRecord(e1, st_);
         hipEventSynchronize(e1);
@@ -532,7 +548,8 @@ int Engine::ensure_mixed() {
 // FP64 residual of the reference system for the current colour-layout state
 // (red in ur_[0], black in ub_); writes the FP32 defect into the FP32 operand
 // blocks and the residual into ctl_->R (host copy in hctl_).
-int Engine::defect64(const StepParams& P) {
+// copy32: also write the FP32 coefficient copies (first defect of a step).
+int Engine::defect64(const StepParams& P, bool copy32) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     const SweepMap M = sweep_map();
     const int nb = M.gx * M.gk;
@@ -547,8 +564,13 @@ int Engine::defect64(const StepParams& P) {
         (void)hipEventCreate(&e1);
         (void)hipEventRecord(e0, st_);
     }
-    k_sor_sweep<double, 3, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
-    k_sor_sweep<double, 3, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);
+    if (copy32) {
+        k_sor_sweep<double, 4, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
+        k_sor_sweep<double, 4, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);
+    } else {
+        k_sor_sweep<double, 3, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
+        k_sor_sweep<double, 3, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);
+    }
     if (launch_check("k_sor_sweep defect")) return -1;
     if (prof.on) (void)hipEventRecord(e1, st_);
     // z = 1, cap = 1: the reduction always records R and raises stop; the
@@ -591,7 +613,7 @@ int Engine::solve_mixed(const StepParams& P, SolveStats* out) {
     J.kind0 = 3;
     long sweeps = 0;
     int rounds = 0;
-    if (defect64(P)) return -1;
+    if (defect64(P, true)) return -1;
     while (hctl_->R > hctl_->tol && rounds < max_rounds_) {
         // d = 0 (ghosts stay 0: homogeneous boundary of the correction)
         for (float* d : {dr_[0], dr_[1], db_})
@@ -608,7 +630,7 @@ int Engine::solve_mixed(const StepParams& P, SolveStats* out) {
         k_correct<<<g, kBlock, 0, st_>>>(ur_[0], dr_[z & 1], np_);
         k_correct<<<g, kBlock, 0, st_>>>(ub_, db_, np_);
         if (launch_check("k_correct")) return -1;
-        if (defect64(P)) return -1;
+        if (defect64(P, false)) return -1;
     }
     red_final_ = 0;
     if (out) {

@@ -160,7 +160,6 @@ __device__ __forceinline__ bool is_interior(const Geo& G, int i, int j, int k, i
 // ----------------------------------------------------------------------------
 struct SplitBufs {
     double* opnd[2];     // [colour] operand blocks (coefficients + prev)
-    float* opnd32[2];    // [colour] FP32 coefficient copies (or null)
     double* ured[2];     // red ping-pong buffers
     double* ublack;
 };
@@ -199,9 +198,9 @@ __device__ __forceinline__ bool map_plane_block(const PlaneMap& M, int b, int& b
 template <bool WITH_G, bool POW2>
 __global__ __launch_bounds__(kBlock) void k_setup_step(
     Geo G, PlaneMap PM, Scale S, double eps2, double h4, const double* __restrict__ u,
-    CPtr8 g, SplitBufs sb, double* __restrict__ prev_full,
-    const double* __restrict__ start_full, double* __restrict__ prev2,
-    double* __restrict__ prev3, double* __restrict__ partials, int warm) {
+    CPtr8 g, SplitBufs sb, const double* hist1, const double* hist2, const double* hist3,
+    double* prev_out, const double* __restrict__ start_full, double* __restrict__ partials,
+    int warm) {
     int bx, k, l;
     if (!map_plane_block(PM, blockIdx.x, bx, k, l)) return;
     const int t = bx * kBlock + threadIdx.x;
@@ -214,25 +213,27 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
         const double uc = u[p];
         double start = uc;
         const bool in = is_interior(G, i, j, k, l);
-        const double pf = (warm == 1 || warm >= 3 || prev2) ? prev_full[p] : 0.0;
-        const double p2 = prev2 ? prev2[p] : 0.0;
+        // history u^(n-1), u^(n-2), u^(n-3) as the start needs it; u^n goes to
+        // prev_out, which may be one of them (read first, same point)
+        const double pf = (warm == 1 || warm >= 3) ? hist1[p] : 0.0;
+        const double p2 = (warm >= 3) ? hist2[p] : 0.0;
+        const double p3 = (warm == 4) ? hist3[p] : 0.0;
         if (warm == 1 && in) start = 2.0 * uc - pf;
         if (warm == 2 && in) start = start_full[p];
         if (warm == 3 && in) start = (3.0 * uc - 3.0 * pf) + p2;
-        if (warm == 4 && in) start = ((4.0 * uc - 6.0 * pf) + 4.0 * p2) - prev3[p];
-        if (prev3) prev3[p] = p2;
-        if (prev2) prev2[p] = pf;
-        prev_full[p] = uc;
+        if (warm == 4 && in) start = ((4.0 * uc - 6.0 * pf) + 4.0 * p2) - p3;
+        prev_out[p] = uc;
         if (c == 0) {
             sb.ured[0][pp] = start;
-            sb.ured[1][pp] = start;
+            // the second red buffer's interior is written by the first red
+            // pass before anything reads it: only its ghosts are set here
+            if (!in) sb.ured[1][pp] = start;
         } else {
             sb.ublack[pp] = start;
         }
         if (is_interior(G, i, j, k, l)) {
             const long oi = opnd_index(G, k, l, (j - 1) * G.H + (i >> 1));
             double* ob = (c ? sb.opnd[1] : sb.opnd[0]) + oi;
-            float* ob32 = sb.opnd32[0] ? (c ? sb.opnd32[1] : sb.opnd32[0]) + oi : nullptr;
             ob[8 * 64] = uc;
             double a[8];
             all_face_sq<POW2>(u, p, G, S, a);
@@ -248,7 +249,6 @@ __global__ __launch_bounds__(kBlock) void k_setup_step(
                 if constexpr (WITH_G) cf = vc * g.p[f][p] * (1.0 / N[f]);
                 else cf = vc * (1.0 / N[f]);
                 ob[f * 64] = cf;
-                if (ob32) ob32[f * 64] = (float)cf;
             }
             acc = uc * uc * h4;
         }
@@ -351,7 +351,9 @@ __device__ __forceinline__ void load_point(const Geo& G, const T* __restrict__ o
 // One point update, reference operand order (segmentation4D.c:242-256); the
 // residual term r = u (1 + c sum) - c sum coef u_nb - prev (segmentation4D.c:
 // 290-307) before (RESID 1) or after (RESID 2) the update.  RESID 3 does not
-// update: it writes -r (the defect, FP32) to *defect and accumulates r^2 h^4.
+// update: it writes -r (the defect, FP32) to *defect and accumulates r^2 h^4;
+// RESID 4 also writes the coefficients, rounded to FP32, 8..1 components
+// before *defect.
 template <typename T, int RESID>
 __device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart, T yfw, T c,
                                              T om, double h4, T* xout, float* defect,
@@ -377,12 +379,16 @@ __device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart
     dg = dg + P.c[6];
     dg = dg + P.c[7];
     const T D = 1 + c * dg;
-    if constexpr (RESID == 1 || RESID == 3) {
+    if constexpr (RESID == 1 || RESID >= 3) {
         const T r = P.uo * D - c * num - P.pv;
         const double rd = (double)r;
         acc += rd * rd * h4;
-        if constexpr (RESID == 3) {
+        if constexpr (RESID >= 3) {
             *defect = (float)(-r);
+            if constexpr (RESID == 4) {  // FP32 copies of the 8 coefficients
+#pragma unroll
+                for (int f = 0; f < 8; f++) defect[(f - 8) * 64] = (float)P.c[f];
+            }
             return;
         }
     }
@@ -409,7 +415,8 @@ __device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart
 // 2: residual AFTER the update (black pass of iteration z: its red
 // neighbours are final for z).  segmentation4D.c:287-308.  3: residual only,
 // the defect -r written as FP32 into component 8 of the operand blocks
-// `defect_ob` (the right-hand side of the FP32 correction equation).
+// `defect_ob` (the right-hand side of the FP32 correction equation); 4: as 3,
+// and the FP32 copies of the 8 coefficients into components 0..7.
 // ----------------------------------------------------------------------------
 template <typename T, int RESID, int U, bool NTO = false>
 __global__ __launch_bounds__(kBlock) void k_sor_sweep(
@@ -442,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(
     const long ob_off = opnd_index(G, k, 1, t);
     const T* ob0 = opnd + ob_off;
     const long ostride = (long)(G.K - 2) * G.RP * kOpnd;
-    float* def0 = (RESID == 3) ? defect_ob + ob_off + 8 * 64 : nullptr;
+    float* def0 = (RESID >= 3) ? defect_ob + ob_off + 8 * 64 : nullptr;
     // window w[q] = other colour at column m, row l-1+q
     T w[U + 2];
     w[0] = rowok ? y[base + (l0 - 1) * G.pw] : T(0);
