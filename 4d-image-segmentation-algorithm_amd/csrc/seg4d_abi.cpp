@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "seg4d_batch.h"
 #include "seg4d_engine.h"
 #include "seg4d_net.h"
 
@@ -40,6 +41,7 @@ namespace {
 
 s4d::Engine* g_eng = nullptr;
 s4d::Net* g_net = nullptr;   // learned warm start (survives re-allocation)
+s4d::Batch* g_batch = nullptr;  // batched form (independent of the globals' arrays)
 int g_resident = 0;
 int g_verbose = 1;
 double g_rel = 0.0;
@@ -730,5 +732,123 @@ API int seg4dDeviceInfo(char* buf, int len) {
         return -1;
     std::snprintf(buf, (size_t)len, "%s|%d CUs|%s|%.1f GB", prop.name, prop.multiProcessorCount,
                   prop.gcnArchName, prop.totalGlobalMem / 1e9);
+    return 0;
+}
+
+// ---- single-volume launch-order autotuner ----------------------------------
+
+API int seg4dAutotune(int reps, int* runs, int* planes, float* best_ms) {
+    if (!need_engine("seg4dAutotune")) return -1;
+    float ms = 0.f;
+    if (g_eng->autotune(reps > 0 ? reps : 3, &ms)) {
+        fail("seg4dAutotune");
+        return -1;
+    }
+    if (runs) *runs = (g_eng->sweep_variant_ & 2) ? g_eng->sweep_tb_ : 0;
+    if (planes) *planes = (g_eng->sweep_variant_ & 2) ? g_eng->sweep_tk_ : 0;
+    if (best_ms) *best_ms = ms;
+    return 0;
+}
+
+// ---- batched form -------------------------------------------------------------
+
+static bool need_batch(const char* fn) {
+    if (g_batch && g_batch->size() > 0) return true;
+    s4d::set_error(std::string(fn) + ": no batch (seg4dBatchCreate first)");
+    return false;
+}
+
+static int batch_fail(const char* fn) {
+    std::string msg = fn;
+    if (g_batch && !g_batch->err.empty()) msg += " (" + g_batch->err + ")";
+    s4d::set_error(msg);
+    std::fprintf(stderr, "seg4d: %s\n", msg.c_str());
+    return -1;
+}
+
+API int seg4dBatchCreate(int nvol, const int* dims) {
+    delete g_batch;
+    g_batch = new s4d::Batch();
+    if (g_batch->init(nvol, dims)) {
+        const int rc = batch_fail("seg4dBatchCreate");
+        delete g_batch;
+        g_batch = nullptr;
+        return rc;
+    }
+    return 0;
+}
+
+API void seg4dBatchDestroy(void) {
+    delete g_batch;
+    g_batch = nullptr;
+}
+
+API int seg4dBatchSetVolume(int v, const double* img, const double* u0, const double* mask) {
+    if (!need_batch("seg4dBatchSetVolume")) return -1;
+    s4d::EdgeParams ep;
+    ep.sigma = sigma;
+    ep.h = h;
+    ep.omega = omega_sigma;
+    ep.gsTol = gsTol;
+    ep.rel2 = g_rel * g_rel;
+    ep.cap = maxNoGSIteration_sigma;
+    ep.K = K;
+    ep.theta = theta_img;
+    if (g_batch->set_volume(v, img, u0, mask, ep)) return batch_fail("seg4dBatchSetVolume");
+    return 0;
+}
+
+API int seg4dBatchStep(void) {
+    if (!need_batch("seg4dBatchStep")) return -1;
+    if (g_batch->step(step_params())) return batch_fail("seg4dBatchStep");
+    return 0;
+}
+
+API int seg4dBatchGetU(int v, double* out) {
+    if (!need_batch("seg4dBatchGetU")) return -1;
+    if (g_batch->download_u(v, out)) return batch_fail("seg4dBatchGetU");
+    return 0;
+}
+
+API int seg4dBatchStats(int v, int* sweeps, int* rounds, double* R, double* tol) {
+    if (!need_batch("seg4dBatchStats") || v < 0 || v >= g_batch->size()) return -1;
+    const s4d::SolveStats& st = g_batch->stats[v];
+    if (sweeps) *sweeps = st.z;
+    if (rounds) *rounds = st.rounds;
+    if (R) *R = st.R;
+    if (tol) *tol = st.tol;
+    return 0;
+}
+
+API double seg4dBatchLastStepMs(void) { return g_batch ? g_batch->last_step_ms : 0.0; }
+
+API int seg4dBatchConfigure(int precision, int warm, int graph, int depth) {
+    if (!need_batch("seg4dBatchConfigure")) return -1;
+    if (precision < 0 || precision > 1 || !(warm == 0 || warm == 1 || warm == 3 || warm == 4) ||
+        graph < 0 || graph > 1 || depth < 1 || depth > 16) {
+        s4d::set_error("seg4dBatchConfigure: precision 0/1, warm 0/1/3/4, graph 0/1, depth 1..16");
+        return -1;
+    }
+    g_batch->precision = precision;
+    g_batch->warm = warm;
+    g_batch->graph = graph;
+    g_batch->depth = depth;
+    return 0;
+}
+
+API int seg4dBatchAutotune(int reps, float* best_ms) {
+    if (!need_batch("seg4dBatchAutotune")) return -1;
+    std::vector<float> ms;
+    if (g_batch->autotune(reps > 0 ? reps : 3, &ms)) return batch_fail("seg4dBatchAutotune");
+    if (best_ms)
+        for (int v = 0; v < g_batch->size(); v++) best_ms[v] = ms[v];
+    return 0;
+}
+
+API int seg4dBatchTile(int v, int* runs, int* planes) {
+    if (!need_batch("seg4dBatchTile") || v < 0 || v >= g_batch->size()) return -1;
+    s4d::Engine& E = g_batch->vol(v);
+    if (runs) *runs = (E.sweep_variant_ & 2) ? E.sweep_tb_ : 0;
+    if (planes) *planes = (E.sweep_variant_ & 2) ? E.sweep_tk_ : 0;
     return 0;
 }

@@ -1,0 +1,391 @@
+// seg4d_batch.hip -- the batched form (see seg4d_batch.h).
+#include "seg4d_batch.h"
+
+#include <algorithm>
+#include <cstring>
+
+#include "seg4d_batch_kernels.h"
+
+namespace s4d {
+
+namespace {
+inline int cdivi(long a, long b) { return (int)((a + b - 1) / b); }
+}  // namespace
+
+Batch::Batch() = default;
+Batch::~Batch() { release(); }
+
+int Batch::check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return 0;
+    err = std::string(what) + ": " + hipGetErrorString(e);
+    set_error(err);
+    return -1;
+}
+
+void Batch::release() {
+    if (st_) (void)hipStreamSynchronize(st_);
+    if (g64_) (void)hipGraphExecDestroy(g64_);
+    if (g32_) (void)hipGraphExecDestroy(g32_);
+    g64_ = g32_ = nullptr;
+    g64_cap_ = g32_cap_ = -1;
+    E_.clear();  // engines release their buffers (the stream is not theirs)
+    if (ctl_) (void)hipFree(ctl_);
+    if (ctl2_) (void)hipFree(ctl2_);
+    if (hctl_) (void)hipHostFree(hctl_);
+    if (hctl2_) (void)hipHostFree(hctl2_);
+    if (dj64_) (void)hipFree(dj64_);
+    if (dj32_) (void)hipFree(dj32_);
+    ctl_ = ctl2_ = hctl_ = hctl2_ = nullptr;
+    dj64_ = nullptr;
+    dj32_ = nullptr;
+    j64_.clear();
+    j32_.clear();
+    stats.clear();
+    for (auto e : evq_) (void)hipEventDestroy(e);
+    evq_.clear();
+    jobs_ok_ = false;
+    if (st_) (void)hipStreamDestroy(st_);
+    st_ = nullptr;
+}
+
+int Batch::init(int n, const int* dims) {
+    release();
+    if (n < 1 || !dims) {
+        err = "batch: need at least one volume";
+        set_error(err);
+        return -1;
+    }
+    if (check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate")) return -1;
+    for (int v = 0; v < n; v++) {
+        const int* d = dims + 4 * v;
+        E_.emplace_back(new Engine());
+        Engine& E = *E_.back();
+        if (E.init(d[0] + 2, d[1] + 2, d[2] + 2, d[3] + 2, true, false) || E.adopt_stream(st_)) {
+            err = "batch volume " + std::to_string(v) + ": " + E.err;
+            set_error(err);
+            return -1;
+        }
+    }
+    const size_t cb = (size_t)n * sizeof(SorCtl);
+    if (check(hipMalloc((void**)&ctl_, cb), "hipMalloc ctl") ||
+        check(hipMalloc((void**)&ctl2_, cb), "hipMalloc ctl") ||
+        check(hipHostMalloc((void**)&hctl_, cb, hipHostMallocDefault), "hipHostMalloc") ||
+        check(hipHostMalloc((void**)&hctl2_, cb, hipHostMallocDefault), "hipHostMalloc") ||
+        check(hipMalloc((void**)&dj64_, (size_t)n * sizeof(BatchJob<double>)), "hipMalloc jobs") ||
+        check(hipMalloc((void**)&dj32_, (size_t)n * sizeof(BatchJob<float>)), "hipMalloc jobs"))
+        return -1;
+    std::memset(hctl_, 0, cb);
+    std::memset(hctl2_, 0, cb);
+    if (check(hipMemsetAsync(ctl_, 0, cb, st_), "memset ctl") ||
+        check(hipMemsetAsync(ctl2_, 0, cb, st_), "memset ctl"))
+        return -1;
+    stats.assign(n, SolveStats());
+    return check(hipStreamSynchronize(st_), "batch init");
+}
+
+int Batch::set_volume(int v, const double* img, const double* u0, const double* mask,
+                      const EdgeParams& ep) {
+    if (v < 0 || v >= size() || !img || !u0 || !mask) {
+        err = "batch: bad volume index or null array";
+        set_error(err);
+        return -1;
+    }
+    Engine& E = *E_[v];
+    if (E.upload(A_I0, img) || E.upload(A_U, u0) || E.upload(A_MASK, mask)) return -1;
+    int z = 0;
+    double R = 0.0;
+    // the edge indicator of the image alone: rho = 0, so the second
+    // image's term vanishes exactly and the image serves as both arguments
+    if (E.presmooth(E.full(A_I0), ep.sigma, ep.h, ep.omega, ep.gsTol, ep.rel2, ep.cap, &z, &R) ||
+        E.gfun(E.full(A_I0), E.full(A_I0), ep.h, ep.K, ep.theta, 0.0))
+        return -1;
+    return E.sync();
+}
+
+int Batch::build_jobs(const StepParams& P) {
+    const int n = size();
+    const double c = P.tau / (P.h * P.h);
+    const double h4 = P.h * P.h * P.h * P.h;
+    const double om32 = inner_omega > 0.0 ? inner_omega : P.omega;
+    if (jobs_ok_ && c == jobs_c_ && P.omega == jobs_om_ && om32 == jobs_om32_ && h4 == jobs_h4_ &&
+        precision == jobs_prec_)
+        return 0;
+    j64_.assign(n, BatchJob<double>());
+    j32_.assign(n, BatchJob<float>());
+    long b0 = 0, cb0 = 0;
+    for (int v = 0; v < n; v++) {
+        Engine& E = *E_[v];
+        if (precision == 1 && E.ensure_mixed()) return -1;
+        const SweepMap M = E.sweep_map();
+        const int nbk = M.tiled ? ((M.ntiles + 7) / 8) * 8 * M.tb * M.tk : M.gx * M.gk;
+        const int nbp = M.gx * M.gk;
+        if ((long)nbp > E.npart_) {
+            err = "batch: SOR grid exceeds a volume's partial buffer";
+            set_error(err);
+            return -1;
+        }
+        const int cnb = std::min(cdivi(E.np_, kBlock), 2048);
+        BatchJob<double>& A = j64_[v];
+        BatchJob<float>& B = j32_[v];
+        A.G = B.G = E.geometry();
+        A.M = B.M = M;
+        A.block0 = B.block0 = (int)b0;
+        A.nblocks = B.nblocks = nbk;
+        A.cblock0 = B.cblock0 = (int)cb0;
+        A.cnblocks = B.cnblocks = cnb;
+        b0 += ((nbk + 7) / 8) * 8;  // volumes start on XCD 0
+        cb0 += cnb;
+        A.c = c;
+        A.om = P.omega;
+        B.c = (float)c;
+        B.om = (float)om32;
+        A.h4 = B.h4 = h4;
+        A.opnd[0] = E.op_[0];
+        A.opnd[1] = E.op_[1];
+        A.red[0] = E.ur_[0];
+        A.red[1] = E.ur_[1];
+        A.black = E.ub_;
+        A.part_r = B.part_r = E.part_;
+        A.part_b = B.part_b = E.part_ + E.npart_;
+        A.nb = B.nb = nbp;
+        A.np = B.np = E.np_;
+        A.ctl = ctl_ + v;
+        B.ctl = ctl2_ + v;
+        A.defect_ob[0] = E.op32_[0];
+        A.defect_ob[1] = E.op32_[1];
+        B.defect_ob[0] = B.defect_ob[1] = nullptr;
+        B.opnd[0] = E.op32_[0];
+        B.opnd[1] = E.op32_[1];
+        B.red[0] = E.dr_[0];
+        B.red[1] = E.dr_[1];
+        B.black = E.db_;
+    }
+    if (b0 >= (1L << 31) || cb0 >= (1L << 31)) {
+        err = "batch: grid too large";
+        set_error(err);
+        return -1;
+    }
+    grid_ = (int)b0;
+    cgrid_ = (int)cb0;
+    if (check(hipMemcpyAsync(dj64_, j64_.data(), (size_t)n * sizeof(BatchJob<double>),
+                             hipMemcpyHostToDevice, st_), "jobs") ||
+        check(hipMemcpyAsync(dj32_, j32_.data(), (size_t)n * sizeof(BatchJob<float>),
+                             hipMemcpyHostToDevice, st_), "jobs") ||
+        check(hipStreamSynchronize(st_), "jobs"))
+        return -1;
+    // captured graphs hold the old grid sizes
+    if (g64_) (void)hipGraphExecDestroy(g64_);
+    if (g32_) (void)hipGraphExecDestroy(g32_);
+    g64_ = g32_ = nullptr;
+    g64_cap_ = g32_cap_ = -1;
+    jobs_ok_ = true;
+    jobs_c_ = c;
+    jobs_om_ = P.omega;
+    jobs_om32_ = om32;
+    jobs_h4_ = h4;
+    jobs_prec_ = precision;
+    return 0;
+}
+
+// Batched red-black SOR of every volume of `hj` to its stopping test; the
+// loop structure is the single-volume solve's (red(z+1) yields the red
+// residual terms of z, reduce(z), black(z+1) a no-op once z stopped), with
+// one launch per pass for all volumes and the decision per volume.
+template <typename T>
+int Batch::sor_run(std::vector<BatchJob<T>>& hj, BatchJob<T>* dj, SorCtl* dctl, SorCtl* hctl,
+                   int cap, hipGraphExec_t* gexec, int* gcap) {
+    (void)hj;
+    const int n = size();
+    const dim3 grid((unsigned)grid_);
+    int rc = 0;
+    auto red = [&](int zr, int zw, bool resid) {
+        if (resid) k_sor_batch<T, 1, 2><<<grid, kBlock, 0, st_>>>(dj, n, 0, zr, zw);
+        else k_sor_batch<T, 0, 2><<<grid, kBlock, 0, st_>>>(dj, n, 0, zr, zw);
+    };
+    auto black = [&](int zr) { k_sor_batch<T, 2, 2><<<grid, kBlock, 0, st_>>>(dj, n, 1, zr, 0); };
+    // iteration z + 1 and the stopping test of iteration z
+    auto iter = [&](int z) {
+        red(z & 1, (z + 1) & 1, true);
+        k_reduce_batch<T><<<n, 1024, 0, st_>>>(dj, cap);
+        rc |= check(hipMemcpyAsync(hctl, dctl, (size_t)n * sizeof(SorCtl), hipMemcpyDeviceToHost,
+                                   st_), "ctl copy");
+        black((z + 1) & 1);
+    };
+    if (graph && (*gexec == nullptr || *gcap != cap)) {
+        if (*gexec) (void)hipGraphExecDestroy(*gexec);
+        *gexec = nullptr;
+        // two iterations from an odd z: every replay starts at an odd
+        // iteration, so the ping-pong parities of the capture hold
+        hipGraph_t g = nullptr;
+        if (check(hipStreamSynchronize(st_), "capture sync") ||
+            check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture"))
+            return -1;
+        iter(1);
+        iter(2);
+        const hipError_t ec = hipStreamEndCapture(st_, &g);
+        if (check(ec, "end capture") || rc) return -1;
+        if (check(hipGetLastError(), "captured launches")) return -1;
+        const hipError_t ei = hipGraphInstantiateWithFlags(gexec, g, 0);
+        (void)hipGraphDestroy(g);
+        if (check(ei, "graph instantiate")) return -1;
+        *gcap = cap;
+    }
+    red(0, 1, false);
+    black(1);
+    if (check(hipGetLastError(), "batched pass")) return -1;
+    const int per = graph ? 2 : 1;  // iterations per queued unit
+    const int dq = depth > 0 ? depth : 1;
+    while ((int)evq_.size() < dq) {
+        hipEvent_t e;
+        if (check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event")) return -1;
+        evq_.push_back(e);
+    }
+    int z = 1;
+    long unit = 0;
+    int queued = 0;
+    for (;;) {
+        while (queued < dq) {
+            if (graph) rc |= check(hipGraphLaunch(*gexec, st_), "graph launch");
+            else iter(z);
+            rc |= check(hipGetLastError(), "batched iteration");
+            rc |= check(hipEventRecord(evq_[unit % dq], st_), "event");
+            if (rc) return -1;
+            z += per;
+            unit++;
+            queued++;
+        }
+        hipError_t q;
+        while ((q = hipEventQuery(evq_[(unit - queued) % dq])) == hipErrorNotReady) {
+        }
+        if (check(q, "batch solve wait")) return -1;
+        queued--;
+        bool all = true;
+        for (int v = 0; v < n && all; v++) all = hctl[v].stop != 0;
+        if (all) break;
+    }
+    return check(hipStreamSynchronize(st_), "batch solve sync");
+}
+
+int Batch::step(const StepParams& P) {
+    const int n = size();
+    if (n == 0) {
+        err = "batch: not initialised";
+        set_error(err);
+        return -1;
+    }
+    for (auto& e : E_) {
+        e->precision_ = precision;
+        e->warm_ = warm;
+    }
+    if (build_jobs(P)) return -1;
+    const int cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
+    hipEvent_t e0, e1;
+    if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return -1;
+    int rc = check(hipEventRecord(e0, st_), "event");
+    for (int v = 0; v < n && !rc; v++)
+        if (E_[v]->setup(P, true, true, ctl_ + v)) {
+            err = E_[v]->err;
+            rc = -1;
+        }
+    last_iterations = 0;
+    if (!rc && precision == 0) {
+        rc = sor_run<double>(j64_, dj64_, ctl_, hctl_, cap, &g64_, &g64_cap_);
+        for (int v = 0; v < n && !rc; v++) {
+            stats[v].z = hctl_[v].zfinal;
+            stats[v].R = hctl_[v].R;
+            stats[v].tol = hctl_[v].tol;
+            stats[v].sumsq = hctl_[v].sumsq;
+            stats[v].rounds = 0;
+            E_[v]->red_final_ = hctl_[v].zfinal & 1;
+            last_iterations += hctl_[v].zfinal;
+        }
+    } else if (!rc) {
+        const dim3 grid((unsigned)grid_), cgrid((unsigned)cgrid_);
+        std::vector<int> sweeps(n, 0);
+        auto defect = [&](bool copy32) {
+            if (copy32) {
+                k_defect_batch<4><<<grid, kBlock, 0, st_>>>(dj64_, n, 0);
+                k_defect_batch<4><<<grid, kBlock, 0, st_>>>(dj64_, n, 1);
+            } else {
+                k_defect_batch<3><<<grid, kBlock, 0, st_>>>(dj64_, n, 0);
+                k_defect_batch<3><<<grid, kBlock, 0, st_>>>(dj64_, n, 1);
+            }
+            k_defect_decide<<<n, 1024, 0, st_>>>(dj64_, max_rounds);
+            int r = check(hipGetLastError(), "batched defect");
+            r |= check(hipMemcpyAsync(hctl_, ctl_, (size_t)n * sizeof(SorCtl),
+                                      hipMemcpyDeviceToHost, st_), "ctl copy");
+            return r | check(hipStreamSynchronize(st_), "defect sync");
+        };
+        rc = defect(true);
+        for (;;) {
+            if (rc) break;
+            std::vector<char> active(n);
+            bool any = false;
+            for (int v = 0; v < n; v++) {
+                active[v] = hctl_[v].stop == 0;
+                any = any || active[v];
+            }
+            if (!any) break;
+            k_zero_batch<<<cgrid, kBlock, 0, st_>>>(dj64_, dj32_, n);
+            k_inner_ctl_batch<<<(n + 63) / 64, 64, 0, st_>>>(dj64_, dj32_, n, inner_frac,
+                                                            inner_floor2);
+            rc = check(hipGetLastError(), "batched round set-up");
+            if (!rc) rc = sor_run<float>(j32_, dj32_, ctl2_, hctl2_, cap, &g32_, &g32_cap_);
+            if (rc) break;
+            for (int v = 0; v < n; v++)
+                if (active[v]) sweeps[v] += hctl2_[v].zfinal;
+            k_correct_batch<<<cgrid, kBlock, 0, st_>>>(dj64_, dj32_, n);
+            rc = check(hipGetLastError(), "batched correction");
+            if (!rc) rc = defect(false);
+        }
+        for (int v = 0; v < n && !rc; v++) {
+            stats[v].z = sweeps[v];
+            stats[v].R = hctl_[v].R;
+            stats[v].tol = hctl_[v].tol;
+            stats[v].sumsq = hctl_[v].sumsq;
+            stats[v].rounds = hctl_[v].z;
+            E_[v]->red_final_ = 0;
+            last_iterations += sweeps[v];
+        }
+    }
+    for (int v = 0; v < n && !rc; v++)
+        if (E_[v]->unpack(E_[v]->full(A_MASK))) {
+            err = E_[v]->err;
+            rc = -1;
+        }
+    if (!rc) rc = check(hipEventRecord(e1, st_), "event");
+    if (!rc) rc = check(hipEventSynchronize(e1), "event");
+    float ms = 0.f;
+    if (!rc) rc = check(hipEventElapsedTime(&ms, e0, e1), "event");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return -1;
+    last_step_ms = ms;
+    for (auto& e : E_) e->step_done();
+    return 0;
+}
+
+int Batch::autotune(int reps, std::vector<float>* best_ms) {
+    if (best_ms) best_ms->assign(size(), 0.f);
+    for (int v = 0; v < size(); v++) {
+        float ms = 0.f;
+        if (E_[v]->autotune(reps, &ms)) {
+            err = E_[v]->err;
+            return -1;
+        }
+        if (best_ms) (*best_ms)[v] = ms;
+    }
+    jobs_ok_ = false;
+    return 0;
+}
+
+int Batch::download_u(int v, double* host) {
+    if (v < 0 || v >= size() || !host) {
+        err = "batch: bad volume index or null array";
+        set_error(err);
+        return -1;
+    }
+    return E_[v]->download(A_U, host);
+}
+
+}  // namespace s4d

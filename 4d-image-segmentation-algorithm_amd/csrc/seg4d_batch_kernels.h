@@ -1,0 +1,180 @@
+// seg4d_batch_kernels.h -- kernels of the batched form: many volumes of
+// different sizes in one launch per colour pass.
+//
+// Each volume keeps its own buffers, geometry, launch order and control
+// block; a descriptor per volume (BatchJob) holds them in device memory.
+// Block b of a batched launch belongs to the volume whose block range
+// [block0, block0 + nblocks) contains it, and does exactly the work block
+// b - block0 does in that volume's own launch (same kernel body, same
+// partial index, same fixed-order reduction), so every volume's result is
+// bitwise that of its single-volume solve.  Volume ranges start at multiples
+// of 8, so a block's XCD (b mod 8) is the one the volume's own launch order
+// assumes.
+#pragma once
+#include "seg4d_kernels.h"
+
+namespace s4d {
+
+template <typename T>
+struct BatchJob {
+    Geo G;
+    SweepMap M;
+    int block0, nblocks;    // SOR launch: this volume's blocks
+    int cblock0, cnblocks;  // element-wise launches: this volume's blocks
+    T c, om;
+    double h4;
+    const T* opnd[2];       // operand blocks per colour
+    T* red[2];              // red ping-pong
+    T* black;
+    double* part_r;         // block partials per colour
+    double* part_b;
+    int nb;                 // partials per colour (gx * gk)
+    long np;                // colour-array elements
+    SorCtl* ctl;            // control block of this job's solve
+    float* defect_ob[2];    // FP64 jobs: FP32 operand blocks (defect, copies)
+};
+
+template <typename T>
+__device__ __forceinline__ int batch_volume(const BatchJob<T>* __restrict__ J, int nvol, int b) {
+    int v = 0;
+    while (v + 1 < nvol && b >= J[v + 1].block0) v++;
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ int batch_cvolume(const BatchJob<T>* __restrict__ J, int nvol, int b) {
+    int v = 0;
+    while (v + 1 < nvol && b >= J[v + 1].cblock0) v++;
+    return v;
+}
+
+// One SOR colour pass of every running volume.  colour 0: red, reads
+// red[zr], writes red[zw]; colour 1: black in place, its red neighbours in
+// red[zr].  Volumes whose control block says stop return at once.
+template <typename T, int RESID, int U>
+static __global__ __launch_bounds__(kBlock) void k_sor_batch(const BatchJob<T>* __restrict__ J,
+                                                      int nvol, int colour, int zr, int zw) {
+    const int v = batch_volume(J, nvol, blockIdx.x);
+    const BatchJob<T>& B = J[v];
+    if (B.ctl->stop) return;
+    const int lb = blockIdx.x - B.block0;
+    if (lb >= B.nblocks) return;
+    int bx, kb;
+    if (!map_block(B.M, lb, B.nblocks, bx, kb)) return;
+    if (colour == 0)
+        sor_block<T, RESID, U, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[zr], B.red[zw],
+                                      B.black, B.opnd[0], B.part_r, nullptr);
+    else
+        sor_block<T, RESID, U, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, B.black,
+                                      B.red[zr], B.opnd[1], B.part_b, nullptr);
+}
+
+// FP64 defect pass (RESID 3, or 4 with the FP32 coefficient copies) of every
+// volume whose FP64 control block has not stopped; the state is red[0] and
+// black (the FP32-refined form keeps no red ping-pong).
+template <int RESID>
+static __global__ __launch_bounds__(kBlock) void k_defect_batch(const BatchJob<double>* __restrict__ J,
+                                                         int nvol, int colour) {
+    const int v = batch_volume(J, nvol, blockIdx.x);
+    const BatchJob<double>& B = J[v];
+    if (B.ctl->stop) return;
+    const int lb = blockIdx.x - B.block0;
+    if (lb >= B.nblocks) return;
+    int bx, kb;
+    if (!map_block(B.M, lb, B.nblocks, bx, kb)) return;
+    if (colour == 0)
+        sor_block<double, RESID, 1, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[0], nullptr,
+                                           B.black, B.opnd[0], B.part_r, B.defect_ob[0]);
+    else
+        sor_block<double, RESID, 1, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, nullptr,
+                                           B.red[0], B.opnd[1], B.part_b, B.defect_ob[1]);
+}
+
+// Residual reduction and stopping test of every running volume (one block
+// per volume); the iteration index is the control block's own counter.
+template <typename T>
+static __global__ __launch_bounds__(1024) void k_reduce_batch(const BatchJob<T>* __restrict__ J, int cap) {
+    const BatchJob<T>& B = J[blockIdx.x];
+    SorCtl* ctl = B.ctl;
+    if (ctl->stop) return;
+    const double R = resid_sum(B.part_r, B.nb, B.part_b, B.nb);
+    if (threadIdx.x == 0) {
+        const int z = ctl->z + 1;
+        ctl->R = R;
+        ctl->z = z;
+        if (!((R > ctl->tol) && (z < cap))) {
+            ctl->stop = 1;
+            ctl->zfinal = z;
+        }
+    }
+}
+
+// FP64 residual of the FP32-refined form and its decision: a volume stops
+// once R <= tol (or R is not a number) or after max_rounds corrections; the
+// control block's z counts the corrections.
+static __global__ __launch_bounds__(1024) void k_defect_decide(const BatchJob<double>* __restrict__ J,
+                                                        int max_rounds) {
+    const BatchJob<double>& B = J[blockIdx.x];
+    SorCtl* ctl = B.ctl;
+    if (ctl->stop) return;
+    const double R = resid_sum(B.part_r, B.nb, B.part_b, B.nb);
+    if (threadIdx.x == 0) {
+        ctl->R = R;
+        if (!(R > ctl->tol) || ctl->z >= max_rounds) {
+            ctl->stop = 1;
+            ctl->zfinal = ctl->z;
+        }
+    }
+}
+
+// Control blocks of the correction solves: bound max(frac tol, floor2 R) of
+// the volume's FP64 system; volumes that stopped skip the round.
+static __global__ void k_inner_ctl_batch(const BatchJob<double>* __restrict__ J64,
+                                  const BatchJob<float>* __restrict__ J32, int nvol, double frac,
+                                  double floor2) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvol) return;
+    SorCtl* outer = J64[v].ctl;
+    SorCtl* inner = J32[v].ctl;
+    const double a = frac * outer->tol, b2 = floor2 * outer->R;
+    inner->tol = a > b2 ? a : b2;
+    inner->R = 0.0;
+    inner->z = 0;
+    inner->zfinal = 0;
+    inner->stop = outer->stop;
+    if (!outer->stop) outer->z = outer->z + 1;
+}
+
+// d = 0 (all three correction arrays, ghosts included) for running volumes.
+static __global__ __launch_bounds__(kBlock) void k_zero_batch(const BatchJob<double>* __restrict__ J64,
+                                                       const BatchJob<float>* __restrict__ J32,
+                                                       int nvol) {
+    const int v = batch_cvolume(J32, nvol, blockIdx.x);
+    if (J64[v].ctl->stop) return;
+    const BatchJob<float>& B = J32[v];
+    const long stride = (long)B.cnblocks * kBlock;
+    for (long q = (long)(blockIdx.x - B.cblock0) * kBlock + threadIdx.x; q < B.np; q += stride) {
+        B.red[0][q] = 0.0f;
+        B.red[1][q] = 0.0f;
+        B.black[q] = 0.0f;
+    }
+}
+
+// u += d for running volumes; the accepted red correction is the buffer the
+// last counted red pass of the correction solve wrote.
+static __global__ __launch_bounds__(kBlock) void k_correct_batch(const BatchJob<double>* __restrict__ J64,
+                                                          const BatchJob<float>* __restrict__ J32,
+                                                          int nvol) {
+    const int v = batch_cvolume(J32, nvol, blockIdx.x);
+    if (J64[v].ctl->stop) return;
+    const BatchJob<float>& D = J32[v];
+    const BatchJob<double>& U = J64[v];
+    const float* dr = D.red[D.ctl->zfinal & 1];
+    const long stride = (long)D.cnblocks * kBlock;
+    for (long q = (long)(blockIdx.x - D.cblock0) * kBlock + threadIdx.x; q < D.np; q += stride) {
+        U.red[0][q] = U.red[0][q] + (double)dr[q];
+        U.black[q] = U.black[q] + (double)D.black[q];
+    }
+}
+
+}  // namespace s4d

@@ -95,7 +95,7 @@ public:
     // --- time step pieces -------------------------------------------------
     // Coefficients from u (and g if with_g) into the colour layout, u packed
     // into the colour arrays, and (if write_prev) prev = u.
-    int setup(const StepParams& P, bool with_g, bool write_prev);
+    int setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl_target = nullptr);
     // Red-black SOR from the packed state to the stopping test.
     int solve(const StepParams& P, SolveStats* st);
     // Colour layout -> u (interior), with the Mmask pass fused if mask.
@@ -135,6 +135,17 @@ public:
     int sync();
 
     hipStream_t stream() const { return st_; }
+    // Runs this engine's work on stream s (not owned) instead of its own.
+    int adopt_stream(hipStream_t s);
+    // Completed time step: the history levels the setup filled become valid.
+    void step_done();
+    struct Geo geometry() const;
+    // Times the red pass (FP64, no residual) over tile shapes of the tiled
+    // order and the linear order, `reps` launches each, and keeps the
+    // fastest; *best_ms receives its time.  Results are unchanged by the
+    // choice; the pass writes only the second red buffer, whose interior
+    // every solve rewrites first.
+    int autotune(int reps, float* best_ms);
     // SOR pass kernel variant (tuning knob, identical results), bit flags:
     // 1 = two l-steps per load phase, 2 = tiled XCD-resident launch order,
     // 4 = nontemporal operand-block loads, 8 = two k planes per block,
@@ -187,6 +198,7 @@ public:
     int I_ = 0, J_ = 0, K_ = 0, L_ = 0, H_ = 0;
     long n_ = 0, np_ = 0;        // full / colour-plane element counts
     hipStream_t st_ = nullptr;
+    bool own_st_ = true;
     double* full_[A_COUNT] = {};
     double* op_[2] = {};         // operand blocks (8 coefficients + prev) per colour
     long nop_ = 0;               // elements of one operand-block array

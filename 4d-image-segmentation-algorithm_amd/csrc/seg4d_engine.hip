@@ -156,9 +156,24 @@ void Engine::release() {
     pev_.clear();
     if (ev_) hipEventDestroy(ev_);
     ev_ = nullptr;
-    if (st_) hipStreamDestroy(st_);
+    if (st_ && own_st_) hipStreamDestroy(st_);
     st_ = nullptr;
+    own_st_ = true;
     ready_ = false;
+}
+
+int Engine::adopt_stream(hipStream_t s) {
+    if (check(hipStreamSynchronize(st_), "adopt_stream sync")) return -1;
+    if (own_st_ && st_) (void)hipStreamDestroy(st_);
+    st_ = s;
+    own_st_ = false;
+    return 0;
+}
+
+void Engine::step_done() {
+    hist3_ok_ = shifted3_;  // prev3 = u^(n-2)
+    hist2_ok_ = shifted2_;  // prev2 = u^(n-1)
+    hist_ok_ = true;        // prev_full = u^n and u = u^(n+1)
 }
 
 static Geo make_geo(int I, int J, int K, int L, int H) {
@@ -220,7 +235,9 @@ int Engine::download_coef(int f, double* host) {
     return download(A_TMP, host);
 }
 
-int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
+Geo Engine::geometry() const { return make_geo(I_, J_, K_, L_, H_); }
+
+int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl_target) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     const Scale S = make_scale(P.h);
     const bool p2 = is_pow2(P.h);
@@ -326,8 +343,56 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev) {
         hipEventDestroy(e1);
     }
     const int np = PM.gx * PM.gk * PM.gl;  // one partial per (run, k, l)
-    k_init_ctl<<<1, 1024, 0, st_>>>(part_, np, ctl_, P.gsTol, P.rel2);
+    k_init_ctl<<<1, 1024, 0, st_>>>(part_, np, ctl_target ? ctl_target : ctl_, P.gsTol, P.rel2);
     return launch_check("k_init_ctl");
+}
+
+int Engine::autotune(int reps, float* best_ms) {
+    static const int tiles[][2] = {{16, 8}, {8, 8}, {8, 16}, {8, 32}, {16, 4}, {16, 16},
+                                   {32, 4}, {32, 8}, {4, 32}, {64, 4}};
+    const int ntile = (int)(sizeof(tiles) / sizeof(tiles[0]));
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    const int keep_v = sweep_variant_, keep_tb = sweep_tb_, keep_tk = sweep_tk_;
+    hipEvent_t e0, e1;
+    if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return -1;
+    float best = 1e30f;
+    int bv = keep_v, btb = keep_tb, btk = keep_tk;
+    int rc = 0;
+    for (int q = 0; q <= ntile && rc == 0; q++) {
+        // q < ntile: tiled order with tiles[q]; q == ntile: linear order
+        sweep_variant_ = q < ntile ? 3 : 1;
+        if (q < ntile) {
+            sweep_tb_ = tiles[q][0];
+            sweep_tk_ = tiles[q][1];
+        }
+        const SweepMap M = sweep_map();
+        const dim3 grid(M.tiled ? (unsigned)(((M.ntiles + 7) / 8) * 8 * M.tb * M.tk)
+                                : (unsigned)(M.gx * M.gk));
+        float cand = 1e30f;
+        for (int r = 0; r <= reps && rc == 0; r++) {
+            rc |= check(hipEventRecord(e0, st_), "event");
+            k_sor_sweep<double, 0, 2><<<grid, kBlock, 0, st_>>>(G, M, 0.5, 1.0, 1.0, 0, ur_[0], ur_[1], ub_, op_[0], nullptr, nullptr, nullptr);
+            rc |= launch_check("autotune pass");
+            rc |= check(hipEventRecord(e1, st_), "event");
+            rc |= check(hipEventSynchronize(e1), "event");
+            float ms = 0.f;
+            rc |= check(hipEventElapsedTime(&ms, e0, e1), "event");
+            if (r > 0 && ms < cand) cand = ms;  // the first launch warms the caches
+        }
+        if (cand < best) {
+            best = cand;
+            bv = sweep_variant_;
+            btb = sweep_tb_;
+            btk = sweep_tk_;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    sweep_variant_ = rc ? keep_v : bv;
+    sweep_tb_ = rc ? keep_tb : btb;
+    sweep_tk_ = rc ? keep_tk : btk;
+    if (best_ms) *best_ms = best;
+    return rc ? -1 : 0;
 }
 
 SweepMap Engine::sweep_map() const {
@@ -840,9 +905,7 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
         hipEventDestroy(e1);
     }
     if (check(hipStreamSynchronize(st_), "seg_step")) return -1;
-    hist3_ok_ = shifted3_;  // prev3 = u^(n-2)
-    hist2_ok_ = shifted2_;  // prev2 = u^(n-1)
-    hist_ok_ = true;        // prev_full = u^n and u = u^(n+1)
+    step_done();
     return 0;
 }
 
@@ -857,9 +920,7 @@ int Engine::mcf_step(const StepParams& P, double t, SolveStats* st) {
     if (solve(P, st)) return -1;
     if (unpack(nullptr)) return -1;
     if (check(hipStreamSynchronize(st_), "mcf_step")) return -1;
-    hist3_ok_ = shifted3_;
-    hist2_ok_ = shifted2_;
-    hist_ok_ = true;
+    step_done();
     return 0;
 }
 

@@ -196,7 +196,7 @@ __device__ __forceinline__ bool map_plane_block(const PlaneMap& M, int b, int& b
 // holds u^(n-3)).  Non-null history arrays shift by one level: prev3
 // receives u^(n-2), prev2 u^(n-1), prev_full u^n.
 template <bool WITH_G, bool POW2>
-__global__ __launch_bounds__(kBlock) void k_setup_step(
+static __global__ __launch_bounds__(kBlock) void k_setup_step(
     Geo G, PlaneMap PM, Scale S, double eps2, double h4, const double* __restrict__ u,
     CPtr8 g, SplitBufs sb, const double* hist1, const double* hist2, const double* hist3,
     double* prev_out, const double* __restrict__ start_full, double* __restrict__ partials,
@@ -418,15 +418,14 @@ __device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart
 // `defect_ob` (the right-hand side of the FP32 correction equation); 4: as 3,
 // and the FP32 copies of the 8 coefficients into components 0..7.
 // ----------------------------------------------------------------------------
-template <typename T, int RESID, int U, bool NTO = false>
-__global__ __launch_bounds__(kBlock) void k_sor_sweep(
-    Geo G, SweepMap M, T c, T om, double h4, int colour,
+// The work of one block (run bx, k group / l chunk kb), shared by the
+// single-volume and the batched kernels; every thread of the block calls it.
+template <typename T, int RESID, int U, bool NTO>
+__device__ __forceinline__ void sor_block(
+    const Geo& G, const SweepMap& M, int bx, int kb, T c, T om, double h4, int colour,
     const T* xin, T* xout, const T* __restrict__ y,
     const T* __restrict__ opnd, double* __restrict__ partials,
-    const SorCtl* __restrict__ ctl, float* __restrict__ defect_ob) {
-    if (ctl != nullptr && ctl->stop) return;
-    int bx, kb;
-    if (!map_block(M, blockIdx.x, gridDim.x, bx, kb)) return;
+    float* __restrict__ defect_ob) {
     const int lb = (kb - 1) * M.gx + bx;
     // block index kb - 1 = l chunk * gk0 + k group; the block's kp planes:
     // threads [sub * per, (sub + 1) * per) take plane kgrp kp + sub + 1, run bx
@@ -494,8 +493,21 @@ __global__ __launch_bounds__(kBlock) void k_sor_sweep(
     }
 }
 
+template <typename T, int RESID, int U, bool NTO = false>
+static __global__ __launch_bounds__(kBlock) void k_sor_sweep(
+    Geo G, SweepMap M, T c, T om, double h4, int colour,
+    const T* xin, T* xout, const T* __restrict__ y,
+    const T* __restrict__ opnd, double* __restrict__ partials,
+    const SorCtl* __restrict__ ctl, float* __restrict__ defect_ob) {
+    if (ctl != nullptr && ctl->stop) return;
+    int bx, kb;
+    if (!map_block(M, blockIdx.x, gridDim.x, bx, kb)) return;
+    sor_block<T, RESID, U, NTO>(G, M, bx, kb, c, om, h4, colour, xin, xout, y, opnd, partials,
+                                defect_ob);
+}
+
 // u += d over a colour array (the defect-correction update; ghosts of d are 0).
-__global__ __launch_bounds__(kBlock) void k_correct(double* __restrict__ u,
+static __global__ __launch_bounds__(kBlock) void k_correct(double* __restrict__ u,
                                                    const float* __restrict__ d, long n) {
     for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < n; q += (long)gridDim.x * kBlock)
         u[q] = u[q] + (double)d[q];
@@ -503,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void k_correct(double* __restrict__ u,
 
 // Sets the control block of one FP32 correction solve: stop when its residual
 // is below max(floor2 * R, frac * tol) of the outer FP64 system.
-__global__ void k_inner_ctl(const SorCtl* __restrict__ outer, SorCtl* inner, double frac,
+static __global__ void k_inner_ctl(const SorCtl* __restrict__ outer, SorCtl* inner, double frac,
                             double floor2) {
     if (threadIdx.x == 0) {
         const double a = frac * outer->tol, b2 = floor2 * outer->R;
@@ -515,14 +527,10 @@ __global__ void k_inner_ctl(const SorCtl* __restrict__ outer, SorCtl* inner, dou
     }
 }
 
-// Final residual reduction of one SOR iteration and the reference's stopping
-// test `while (R > gsTol && z < maxNoGSIteration)` (segmentation4D.c:309).
-__global__ __launch_bounds__(1024) void k_reduce_resid(
-    const double* __restrict__ pa, int na, const double* __restrict__ pb, int nbp,
-    SorCtl* ctl, double* __restrict__ hist, int z, int cap, int queued) {
-    // a reduce enqueued ahead of the host's decision (queued != 0) keeps a
-    // stopped state: the passes after the stop left stale partials
-    if (queued && ctl->stop) return;
+// Fixed-order sum of the red and black block partials of one iteration by a
+// block of 1024 threads; the result is valid in thread 0.
+__device__ __forceinline__ double resid_sum(const double* __restrict__ pa, int na,
+                                            const double* __restrict__ pb, int nbp) {
     double s = 0.0;
     for (int q = threadIdx.x; q < na; q += 1024) s += pa[q];
     double s2 = 0.0;
@@ -532,9 +540,21 @@ __global__ __launch_bounds__(1024) void k_reduce_resid(
     __shared__ double sm[16];
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
     __syncthreads();
+    double R = sm[0];
+    for (int q = 1; q < 16; q++) R += sm[q];
+    return R;
+}
+
+// Final residual reduction of one SOR iteration and the reference's stopping
+// test `while (R > gsTol && z < maxNoGSIteration)` (segmentation4D.c:309).
+static __global__ __launch_bounds__(1024) void k_reduce_resid(
+    const double* __restrict__ pa, int na, const double* __restrict__ pb, int nbp,
+    SorCtl* ctl, double* __restrict__ hist, int z, int cap, int queued) {
+    // a reduce enqueued ahead of the host's decision (queued != 0) keeps a
+    // stopped state: the passes after the stop left stale partials
+    if (queued && ctl->stop) return;
+    const double R = resid_sum(pa, na, pb, nbp);
     if (threadIdx.x == 0) {
-        double R = sm[0];
-        for (int q = 1; q < 16; q++) R += sm[q];
         ctl->R = R;
         ctl->z = z;
         if (hist) hist[z - 1] = R;
@@ -547,7 +567,7 @@ __global__ __launch_bounds__(1024) void k_reduce_resid(
 }
 
 // Sets the per-solve tolerance: absolute gsTol, or rel^2 * sum(prev^2 h^4).
-__global__ __launch_bounds__(1024) void k_init_ctl(const double* __restrict__ partials,
+static __global__ __launch_bounds__(1024) void k_init_ctl(const double* __restrict__ partials,
                                                   int np, SorCtl* ctl, double gsTol,
                                                   double rel2) {
     double s = 0.0;
@@ -569,7 +589,7 @@ __global__ __launch_bounds__(1024) void k_init_ctl(const double* __restrict__ pa
 }
 
 // Generic fixed-order sum of `np` partials into *out.
-__global__ __launch_bounds__(1024) void k_reduce_sum(const double* __restrict__ partials,
+static __global__ __launch_bounds__(1024) void k_reduce_sum(const double* __restrict__ partials,
                                                     int np, double* out) {
     double s = 0.0;
     for (int q = threadIdx.x; q < np; q += 1024) s += partials[q];
@@ -589,7 +609,7 @@ __global__ __launch_bounds__(1024) void k_reduce_sum(const double* __restrict__ 
 // the reference's Mmask pass (segmentation4D.c:1093-1101) fused in when
 // there are no balls to rescale first.
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_unpack(Geo G, const double* __restrict__ ur,
+static __global__ __launch_bounds__(kBlock) void k_unpack(Geo G, const double* __restrict__ ur,
                                                   const double* __restrict__ ub,
                                                   double* __restrict__ u,
                                                   const double* __restrict__ mask) {
@@ -612,7 +632,7 @@ __global__ __launch_bounds__(kBlock) void k_unpack(Geo G, const double* __restri
 // Thread per (packed column m, row j) of one (k,l) row plane.
 // ----------------------------------------------------------------------------
 template <bool CONST_COEF>
-__global__ __launch_bounds__(kBlock) void k_full_pass(Geo G, double c, double om,
+static __global__ __launch_bounds__(kBlock) void k_full_pass(Geo G, double c, double om,
                                                      int colour, double* u,
                                                      const double* __restrict__ prev,
                                                      CPtr8 cf) {
@@ -665,7 +685,7 @@ __global__ __launch_bounds__(kBlock) void k_full_pass(Geo G, double c, double om
 // Residual partials of the full-layout system; grid x over the interior
 // (j,i) plane, y = k-1, z = l-1.
 template <bool CONST_COEF>
-__global__ __launch_bounds__(kBlock) void k_full_resid(Geo G, double c, double h4,
+static __global__ __launch_bounds__(kBlock) void k_full_resid(Geo G, double c, double h4,
                                                       const double* __restrict__ u,
                                                       const double* __restrict__ prev,
                                                       CPtr8 cf,
@@ -724,7 +744,7 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
-__global__ __launch_bounds__(kBlock) void k_reflect(Geo G, double* u) {
+static __global__ __launch_bounds__(kBlock) void k_reflect(Geo G, double* u) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     const int k = blockIdx.y, l = blockIdx.z;
     if (t >= G.I * G.J) return;
@@ -741,7 +761,7 @@ __global__ __launch_bounds__(kBlock) void k_reflect(Geo G, double* u) {
 // Grid x over interior (j,i), y = k-1, z = l-1.
 // ----------------------------------------------------------------------------
 template <bool POW2>
-__global__ __launch_bounds__(kBlock) void k_gfun(Geo G, Scale S, double K_, double theta,
+static __global__ __launch_bounds__(kBlock) void k_gfun(Geo G, Scale S, double K_, double theta,
                                                 double rho, const double* __restrict__ a,
                                                 const double* __restrict__ b, Ptr8 g) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -762,7 +782,7 @@ __global__ __launch_bounds__(kBlock) void k_gfun(Geo G, Scale S, double K_, doub
 // ----------------------------------------------------------------------------
 // Partial sums of (a-b)^2 h^4 (segmentation4D.c:194-213); grid-stride with a
 // fixed grid so that the summation order is deterministic.
-__global__ __launch_bounds__(kBlock) void k_sqdiff(const double* __restrict__ a,
+static __global__ __launch_bounds__(kBlock) void k_sqdiff(const double* __restrict__ a,
                                                   const double* __restrict__ b, long n,
                                                   double h4, double* __restrict__ partials) {
     double acc = 0.0;
@@ -781,7 +801,7 @@ __global__ __launch_bounds__(kBlock) void k_sqdiff(const double* __restrict__ a,
 // eight sequential face passes leave on every ghost the value of the last
 // pass that touches it (priority w > z > y > x, high side written after low
 // side), and that pass uses the fixed coordinate -(h/2)-1.25 or (h/2)+1.25.
-__global__ __launch_bounds__(kBlock) void k_mcf_exact(Geo G, double h, double tt,
+static __global__ __launch_bounds__(kBlock) void k_mcf_exact(Geo G, double h, double tt,
                                                      double* __restrict__ out,
                                                      int ghosts_only) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -808,7 +828,7 @@ __global__ __launch_bounds__(kBlock) void k_mcf_exact(Geo G, double h, double tt
 
 // Re-pack the ghost shell of the full array into the colour arrays (both red
 // buffers).  Used after the MCF boundary fill.
-__global__ __launch_bounds__(kBlock) void k_pack_ghosts(Geo G, const double* __restrict__ u,
+static __global__ __launch_bounds__(kBlock) void k_pack_ghosts(Geo G, const double* __restrict__ u,
                                                        double* ur0, double* ur1, double* ub) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     const int k = blockIdx.y, l = blockIdx.z;
@@ -829,7 +849,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_ghosts(Geo G, const double* __r
 // Colour layout -> full layout for one coefficient face (interior only),
 // used when the host asks for the coefficient arrays.
 // Component f (0..7 coefficient, 8 prev) of the operand blocks -> full layout.
-__global__ __launch_bounds__(kBlock) void k_unpack_coef(Geo G, const double* __restrict__ orr,
+static __global__ __launch_bounds__(kBlock) void k_unpack_coef(Geo G, const double* __restrict__ orr,
                                                        const double* __restrict__ obl, int f,
                                                        double* __restrict__ out) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -934,7 +954,7 @@ __device__ void ball_minmax_block(const Geo& G, const double* a, int l, const Ba
 
 // Per-ball rescale to [0, 1] (segmentation4D.c:1020-1092): ball 0 of a frame
 // overwrites, later balls keep the larger value.
-__global__ __launch_bounds__(kBlock) void k_rescale_balls(Geo G, double* u, BallList B) {
+static __global__ __launch_bounds__(kBlock) void k_rescale_balls(Geo G, double* u, BallList B) {
     const int l = blockIdx.x + 1;
     for (int c = 0; c < B.count[l]; c++) {
         const int qc = B.offset[l] + c;
@@ -961,7 +981,7 @@ __global__ __launch_bounds__(kBlock) void k_rescale_balls(Geo G, double* u, Ball
 // Local thresholding (segmentation4D.c:971-1018): inside each ball, points of
 // `in` brighter than alpha*min + beta*max of that ball are set to the
 // image's interior maximum *gmax in `thr` (which may alias `in`).
-__global__ __launch_bounds__(kBlock) void k_threshold_balls(Geo G, const double* in,
+static __global__ __launch_bounds__(kBlock) void k_threshold_balls(Geo G, const double* in,
                                                            double* thr, BallList B,
                                                            double alpha, double beta,
                                                            const double* gmax_p) {
@@ -987,7 +1007,7 @@ __global__ __launch_bounds__(kBlock) void k_threshold_balls(Geo G, const double*
 // Seed of the segmentation function (segmentation4D.c:1106-1148): within the
 // ball box and distance d <= 12, u = max(u, |1/(d + 1/2) - 1/(12 + 1/2)|)
 // and Mmask = 1.
-__global__ __launch_bounds__(kBlock) void k_seed_balls(Geo G, double* u, double* mask,
+static __global__ __launch_bounds__(kBlock) void k_seed_balls(Geo G, double* u, double* mask,
                                                       BallList B) {
     const int l = blockIdx.x + 1;
     const double v = 0.5, Rs = 12.;
@@ -1011,7 +1031,7 @@ __global__ __launch_bounds__(kBlock) void k_seed_balls(Geo G, double* u, double*
 }
 
 // One ball's min and max (segmentation4D.c:897-969) into out[0], out[1].
-__global__ __launch_bounds__(kBlock) void k_ball_minmax(Geo G, const double* a, int l,
+static __global__ __launch_bounds__(kBlock) void k_ball_minmax(Geo G, const double* a, int l,
                                                        double x0, double y0, double z0,
                                                        double R, double* out) {
     const BallBox bb = ball_box(G, x0, y0, z0, R);
@@ -1025,7 +1045,7 @@ __global__ __launch_bounds__(kBlock) void k_ball_minmax(Geo G, const double* a, 
 
 // Interior maximum from 0 (segmentation4D.c:876-895), per-block partials;
 // the order of a max does not change its value.
-__global__ __launch_bounds__(kBlock) void k_max_interior(Geo G, const double* __restrict__ a,
+static __global__ __launch_bounds__(kBlock) void k_max_interior(Geo G, const double* __restrict__ a,
                                                         double* __restrict__ partials) {
     __shared__ double sm[kBlock / 64];
     const int t = blockIdx.x * kBlock + threadIdx.x;
@@ -1048,7 +1068,7 @@ __global__ __launch_bounds__(kBlock) void k_max_interior(Geo G, const double* __
     }
 }
 
-__global__ __launch_bounds__(1024) void k_reduce_max(const double* __restrict__ partials,
+static __global__ __launch_bounds__(1024) void k_reduce_max(const double* __restrict__ partials,
                                                     int np, double* out) {
     __shared__ double sm[16];
     double m = 0;
@@ -1065,7 +1085,7 @@ __global__ __launch_bounds__(1024) void k_reduce_max(const double* __restrict__ 
 }
 
 // Mmask pass alone (segmentation4D.c:1093-1101).
-__global__ __launch_bounds__(kBlock) void k_apply_mask(Geo G, double* __restrict__ u,
+static __global__ __launch_bounds__(kBlock) void k_apply_mask(Geo G, double* __restrict__ u,
                                                       const double* __restrict__ mask) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     const int k = blockIdx.y + 1, l = blockIdx.z + 1;

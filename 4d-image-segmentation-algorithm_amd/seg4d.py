@@ -194,6 +194,27 @@ class SegLib(_Lib):
         L.seg4dNetPredicts.restype = ctypes.c_long
         L.seg4dDeviceInfo.restype = c_int
         L.seg4dDeviceInfo.argtypes = [c_char_p, c_int]
+        L.seg4dAutotune.restype = c_int
+        L.seg4dAutotune.argtypes = [c_int, POINTER(c_int), POINTER(c_int),
+                                    POINTER(ctypes.c_float)]
+        L.seg4dBatchCreate.restype = c_int
+        L.seg4dBatchCreate.argtypes = [c_int, POINTER(c_int)]
+        L.seg4dBatchDestroy.restype = None
+        L.seg4dBatchSetVolume.restype = c_int
+        L.seg4dBatchSetVolume.argtypes = [c_int] + [POINTER(c_double)] * 3
+        L.seg4dBatchStep.restype = c_int
+        L.seg4dBatchGetU.restype = c_int
+        L.seg4dBatchGetU.argtypes = [c_int, POINTER(c_double)]
+        L.seg4dBatchStats.restype = c_int
+        L.seg4dBatchStats.argtypes = [c_int, POINTER(c_int), POINTER(c_int), POINTER(c_double),
+                                      POINTER(c_double)]
+        L.seg4dBatchLastStepMs.restype = c_double
+        L.seg4dBatchConfigure.restype = c_int
+        L.seg4dBatchConfigure.argtypes = [c_int] * 4
+        L.seg4dBatchAutotune.restype = c_int
+        L.seg4dBatchAutotune.argtypes = [c_int, POINTER(ctypes.c_float)]
+        L.seg4dBatchTile.restype = c_int
+        L.seg4dBatchTile.argtypes = [c_int, POINTER(c_int), POINTER(c_int)]
         L.seg4d_gen_preset.restype = c_int
         L.seg4d_gen_preset.argtypes = [c_int, c_uint64, POINTER(GenSpec)]
         L.seg4d_generate.restype = c_int
@@ -415,6 +436,75 @@ class SegLib(_Lib):
 
     def set_sweep_variant(self, v):
         return self.lib.seg4dSetSweepVariant(int(v))
+
+    def autotune(self, reps=3):
+        """Tile of the fastest SOR launch order on the current volume:
+        (runs, planes, ms); (0, 0, ms) is the linear order."""
+        tb, tk, ms = c_int(), c_int(), ctypes.c_float()
+        if self.lib.seg4dAutotune(int(reps), ctypes.byref(tb), ctypes.byref(tk),
+                                  ctypes.byref(ms)) != 0:
+            raise RuntimeError(f"seg4dAutotune failed: {self.last_error()}")
+        return tb.value, tk.value, ms.value
+
+    # -- batched form -------------------------------------------------------------
+    def batch_create(self, dims):
+        """dims: list of interior (nx, ny, nz, nt)."""
+        flat = (c_int * (4 * len(dims)))(*[int(x) for d in dims for x in d])
+        if self.lib.seg4dBatchCreate(len(dims), flat) != 0:
+            raise RuntimeError(f"seg4dBatchCreate failed: {self.last_error()}")
+        self._batch_dims = [tuple(int(x) for x in d) for d in dims]
+
+    def batch_destroy(self):
+        self.lib.seg4dBatchDestroy()
+        self._batch_dims = []
+
+    def batch_shape(self, v):
+        nx, ny, nz, nt = self._batch_dims[v]
+        return (nt + 2, nz + 2, ny + 2, nx + 2)
+
+    def batch_set_volume(self, v, img, u0, mask):
+        shp = self.batch_shape(v)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (img, u0, mask)]
+        for a in arrs:
+            if a.shape != shp:
+                raise ValueError(f"volume {v}: array shape {a.shape}, expected {shp}")
+        if self.lib.seg4dBatchSetVolume(v, *[a.ctypes.data_as(POINTER(c_double))
+                                              for a in arrs]) != 0:
+            raise RuntimeError(f"seg4dBatchSetVolume failed: {self.last_error()}")
+
+    def batch_configure(self, precision=0, warm=0, graph=0, depth=2):
+        if self.lib.seg4dBatchConfigure(int(precision), int(warm), int(graph), int(depth)) != 0:
+            raise RuntimeError(f"seg4dBatchConfigure failed: {self.last_error()}")
+
+    def batch_step(self):
+        if self.lib.seg4dBatchStep() != 0:
+            raise RuntimeError(f"seg4dBatchStep failed: {self.last_error()}")
+        return self.lib.seg4dBatchLastStepMs()
+
+    def batch_u(self, v):
+        out = np.zeros(self.batch_shape(v))
+        if self.lib.seg4dBatchGetU(v, out.ctypes.data_as(POINTER(c_double))) != 0:
+            raise RuntimeError(f"seg4dBatchGetU failed: {self.last_error()}")
+        return out
+
+    def batch_stats(self, v):
+        z, r, R, tol = c_int(), c_int(), c_double(), c_double()
+        if self.lib.seg4dBatchStats(v, ctypes.byref(z), ctypes.byref(r), ctypes.byref(R),
+                                    ctypes.byref(tol)) != 0:
+            raise RuntimeError(f"seg4dBatchStats failed: {self.last_error()}")
+        return z.value, r.value, R.value, tol.value
+
+    def batch_autotune(self, reps=3):
+        n = len(self._batch_dims)
+        ms = (ctypes.c_float * n)()
+        if self.lib.seg4dBatchAutotune(int(reps), ms) != 0:
+            raise RuntimeError(f"seg4dBatchAutotune failed: {self.last_error()}")
+        tiles = []
+        for v in range(n):
+            tb, tk = c_int(), c_int()
+            self.lib.seg4dBatchTile(v, ctypes.byref(tb), ctypes.byref(tk))
+            tiles.append((tb.value, tk.value))
+        return tiles, list(ms)
 
     def synchronize(self):
         self.check(self.lib.seg4dSynchronize(), "seg4dSynchronize")

@@ -134,6 +134,12 @@ def main():
     ap.add_argument("--train-runs", type=int, default=8)
     ap.add_argument("--inner-omega", type=float, default=None,
                     help="SOR factor of the FP32 correction solves (default: omega_c)")
+    ap.add_argument("--batch", action="store_true",
+                    help="the batched workload instead: 32 volumes of uneven size")
+    ap.add_argument("--batch-volumes", type=int, default=32)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batched workload: skip the one-volume-at-a-time comparison")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one device copy of u (padded volume, "
                          "FP64) as the known-size reference for rocprofv3 --pmc byte counts")
@@ -148,6 +154,9 @@ def main():
 
     lib = seg4d.SegLib()
     lib.set_verbose(False)
+    if args.batch:
+        run_batch_bench(args, lib)
+        return
     net_path = args.net or warmstart.DEFAULT_WEIGHTS
     training = None
     if args.train_net and not args.no_mixed:
@@ -357,6 +366,133 @@ def main():
         "fp32_refined_cubic_start": mixed_cubic,
         "fp32_refined_learned_start": learned,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def run_batch_bench(args, lib):
+    """Batched workload: `--batch-volumes` volumes of uneven size (generator
+    preset 3: x, y in {48..192 step 16}, z in {32..128 step 16}, t in
+    {6..16}, 2-6 objects each), `--batch-steps` time steps, relative residual
+    1e-6, launch tiles from the autotuner.  Times the batch in FP64 (eager
+    and graph-replayed iterations) and FP32-refined (start u^n and cubic
+    start), and the same FP64 steps one volume at a time on the
+    single-volume path; prints one JSON line."""
+    nv, steps = args.batch_volumes, args.batch_steps
+    specs = [lib.gen_spec(3, 300 + v) for v in range(nv)]
+    dims = [(sp.nx, sp.ny, sp.nz, sp.nt) for sp in specs]
+    nint = [nx * ny * nz * nt for nx, ny, nz, nt in dims]
+    log(f"[batch] {nv} volumes, {sum(nint) / 1e6:.1f} M interior voxels, on {lib.device_info()}")
+    vols = []
+    for sp, d in zip(specs, dims):
+        shape = (d[3] + 2, d[2] + 2, d[1] + 2, d[0] + 2)
+        img, u0 = np.zeros(shape), np.zeros(shape)
+        lib.generate(sp, img, u0)
+        vols.append((img, u0))
+    params(lib)
+
+    def batch_run(precision, warm, graph, depth=2):
+        lib.batch_create(dims)
+        try:
+            lib.batch_configure(precision, warm, graph, depth)
+            t = time.perf_counter()
+            for v, (img, u0) in enumerate(vols):
+                lib.batch_set_volume(v, img, u0, np.ones_like(img))
+            edge_s = time.perf_counter() - t
+            tiles, _ = lib.batch_autotune(3)
+            lib.synchronize()
+            sweeps, dev_ms = [], 0.0
+            t0 = time.perf_counter()
+            for s in range(steps):
+                lib.set(numberOfTimeStep=s + 1)
+                dev_ms += lib.batch_step()
+                sweeps.append([lib.batch_stats(v)[0] for v in range(nv)])
+            t1 = time.perf_counter()
+            us = [lib.batch_u(v)[1:-1, 1:-1, 1:-1, 1:-1].copy() for v in range(nv)]
+        finally:
+            lib.batch_destroy()
+        total = t1 - t0
+        upd = sum(sum(nint[v] * sw[v] for v in range(nv)) for sw in sweeps)
+        tl = {}
+        for t_ in tiles:
+            tl[str(t_)] = tl.get(str(t_), 0) + 1
+        return {"time_s": total, "device_ms": dev_ms, "iterations": int(np.sum(sweeps)),
+                "voxel_updates_per_s": upd / total, "edge_indicator_s_pcie_inclusive": edge_s,
+                "tiles": tl}, us
+
+    log("[batch] FP64, eager iterations ...")
+    fp64, u64 = batch_run(0, 0, 0)
+    log(f"[batch] FP64: {fp64['time_s']:.3f} s, {fp64['voxel_updates_per_s']:.3e} voxel-updates/s")
+    log("[batch] FP64, graph-replayed iterations ...")
+    fp64g, _ = batch_run(0, 0, 1)
+    log(f"[batch] FP64 graph: {fp64g['time_s']:.3f} s")
+    ref = {}
+    for name, (prec, warm, graph) in (("fp32_refined", (1, 0, 1)),
+                                      ("fp32_refined_cubic_start", (1, 4, 1))):
+        log(f"[batch] {name} ...")
+        r, um = batch_run(prec, warm, graph)
+        ious, dus = [], []
+        for a, b in zip(u64, um):
+            ma = a >= 0.5 * (a.max() + a.min())
+            mb = b >= 0.5 * (b.max() + b.min())
+            un = np.logical_or(ma, mb).sum()
+            ious.append(float(np.logical_and(ma, mb).sum() / un) if un else 1.0)
+            dus.append(float(np.abs(a - b).max()))
+        r["speedup_vs_fp64_batch"] = fp64["time_s"] / r["time_s"]
+        r["min_mask_iou_vs_fp64"] = min(ious)
+        r["max_abs_du_vs_fp64"] = max(dus)
+        ref[name] = r
+        log(f"[batch] {name}: {r['time_s']:.3f} s ({r['speedup_vs_fp64_batch']:.2f}x), "
+            f"min IoU {min(ious):.5f}")
+    seq = None
+    if not args.no_sequential:
+        log("[batch] FP64 one volume at a time ...")
+        tot, upd, ok = 0.0, 0, True
+        for v, (img, u0) in enumerate(vols):
+            lib.allocate(*dims[v])
+            params(lib)
+            lib.arr("i0")[...] = img
+            lib.arr("u")[...] = 0.0
+            lib.arr("Mmask")[...] = 1.0
+            lib.gf("i0", "u")
+            lib.set_resident(True)
+            lib.arr("u")[...] = u0
+            lib.upload("u")
+            lib.upload("Mmask")
+            lib.autotune(3)
+            lib.synchronize()
+            t0 = time.perf_counter()
+            for s in range(steps):
+                lib.set(numberOfTimeStep=s + 1)
+                z, _ = lib.seg_time_step()
+                upd += nint[v] * z
+            lib.synchronize()
+            tot += time.perf_counter() - t0
+            lib.download("u")
+            ok = ok and bool(np.array_equal(lib.arr("u")[1:-1, 1:-1, 1:-1, 1:-1], u64[v]))
+            lib.set_resident(False)
+            lib.deallocate()
+        seq = {"time_s": tot, "voxel_updates_per_s": upd / tot,
+               "batch_speedup": tot / fp64["time_s"], "bitwise_equal_to_batch": ok}
+        log(f"[batch] sequential: {tot:.3f} s; batch {tot / fp64['time_s']:.2f}x; "
+            f"bitwise equal {ok}")
+    out = {
+        "metric": "SOR voxel-updates/s, batched volumes",
+        "value": fp64["voxel_updates_per_s"],
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "higher_is_better": True,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": (f"{nv} volumes of uneven size (x, y in 48..192, z in 32..128, "
+                                f"t in 6..16), {steps} time steps each, relative residual 1e-6"),
+                   "volumes": [list(d) for d in dims], "interior_voxels": int(sum(nint))},
+        "fp64_batch": fp64,
+        "fp64_batch_graph": fp64g,
+        "fp32_refined_batch": ref["fp32_refined"],
+        "fp32_refined_cubic_start_batch": ref["fp32_refined_cubic_start"],
+        "fp64_sequential": seq,
     }
     print(json.dumps(out))
 

@@ -287,6 +287,50 @@ int seg4dNetGrad(int I, int J, int K, int L, int P, const uint16_t *feat_bf16,
                  double *loss);
 double seg4dNetPredictMs(void);           /* total time of the predictions  */
 long seg4dNetPredicts(void);
+/* Launch-order autotuner of the current volume (no counterpart in the
+ * reference): times the FP64 red pass over tile shapes of the XCD-resident
+ * order and the linear order (reps launches each, default 3) on the current
+ * device state and keeps the fastest.  Results are unchanged by the choice.
+ * *runs / *planes receive the tile (0 for the linear order), *best_ms its
+ * time; any output may be NULL.  Returns 0, or -1 without a volume. */
+int seg4dAutotune(int reps, int *runs, int *planes, float *best_ms);
+
+/* ---- Batched form (no counterpart in the reference) ------------------------
+ * nvol volumes of different sizes stepped together: every SOR colour pass,
+ * FP64 defect, correction and stopping test of all volumes is one launch,
+ * with the stopping decisions per volume on the device.  Each volume's
+ * result is bitwise that of the same step on the single-volume library with
+ * the same settings.  The batch is independent of _allocateMem_'s arrays;
+ * its parameters are the reference's globals at each call (h, tau, eps2,
+ * omega_c, gsTol or the relative tolerance, maxNoGSIteration; for
+ * seg4dBatchSetVolume sigma, omega_sigma, maxNoGSIteration_sigma, K,
+ * theta_img).  One batch at a time; all calls return 0, or -1 with
+ * seg4dLastError set. */
+/* dims[4v .. 4v+3] = interior (nx, ny, nz, nt) of volume v. */
+int seg4dBatchCreate(int nvol, const int *dims);
+void seg4dBatchDestroy(void);
+/* Host arrays of volume v's padded size (x fastest): the image, whose edge
+ * indicator (presmoothing, then g with rho_thr = 0) becomes the volume's,
+ * the initial u and Mmask. */
+int seg4dBatchSetVolume(int v, const double *img, const double *u0, const double *mask);
+/* One time step of every volume (set-up, SOR solve, mask epilogue). */
+int seg4dBatchStep(void);
+int seg4dBatchGetU(int v, double *u);
+/* Last step of volume v: SOR iterations (FP32-refined: FP32 sweeps),
+ * correction rounds, FP64 residual and its bound. */
+int seg4dBatchStats(int v, int *sweeps, int *rounds, double *R, double *tol);
+double seg4dBatchLastStepMs(void);   /* device time of the last step */
+/* precision 0 (FP64) / 1 (FP32-refined, correction solves at the factor of
+ * seg4dSetInnerOmega's default 1.2); warm start 0, 1, 3 or 4 as
+ * seg4dSetWarmStart; graph 1 replays the solve iterations from a captured
+ * HIP graph (two iterations per replay); depth 1..16 iterations (replays)
+ * queued ahead of the stopping test.  Defaults 0, 0, 0, 2. */
+int seg4dBatchConfigure(int precision, int warm, int graph, int depth);
+/* Per-volume launch-order tiles (seg4dAutotune on each volume); best_ms[v]
+ * (may be NULL) receives each volume's red-pass time. */
+int seg4dBatchAutotune(int reps, float *best_ms);
+int seg4dBatchTile(int v, int *runs, int *planes);
+
 /* Blocks until all queued device work is done. */
 bool seg4dSynchronize(void);
 /* Device name / CU count / arch into buf; returns 0 on success. */
