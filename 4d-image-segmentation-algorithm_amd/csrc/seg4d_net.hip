@@ -462,6 +462,13 @@ int Net::ensure_acts(int I, int J, int K, int L, bool training) {
     return 0;
 }
 
+// Blocks of k_conv_s: 16-point x tiles x 4 x 4 row groups x interior frames.
+static unsigned spatial_blocks(const NGeo& G) {
+    const long n = (long)((G.I - 2 + 15) / 16) * ((G.J - 2 + 3) / 4) * ((G.K - 2 + 3) / 4) *
+                   ((G.L / G.P) * (G.P - 2));
+    return (unsigned)n;
+}
+
 static int conv_waves(long nrow, int* rows_per_wave) {
     const long waves = std::min<long>(nrow, 256L * 16);
     *rows_per_wave = (int)cdivl(nrow, waves);
@@ -473,17 +480,18 @@ int Net::forward(const NGeo& G, hipStream_t st, bool training) {
     int rpw = 1;
     const int waves = conv_waves(nrow, &rpw);
     const dim3 grid((unsigned)cdivl(waves, 4));
-    const Taps Ts = spatial_taps(G, 1), Tt = temporal_taps(G, 1);
+    const Taps Tt = temporal_taps(G, 1);
+    const dim3 sgrid(spatial_blocks(G));
     const bf16* X = feat_;
     for (int L = 0; L < kLayers; L++) {
         bf16* S = training ? S_[L] : S_[0];
         bf16* Y = training ? Y_[L] : Y_[L & 1];
         if (L == 0)
-            net::k_conv<8, kSpatialTaps, net::EPI_NONE><<<grid, 256, 0, st>>>(G, Ts, X, wb_ + off_ws[L],
-                                                                nullptr, nullptr, S, rpw);
+            net::k_conv_s<8, net::EPI_NONE><<<sgrid, 256, 0, st>>>(G, 1, X, wb_ + off_ws[L],
+                                                                  nullptr, nullptr, S);
         else
-            net::k_conv<16, kSpatialTaps, net::EPI_NONE><<<grid, 256, 0, st>>>(G, Ts, X, wb_ + off_ws[L],
-                                                                 nullptr, nullptr, S, rpw);
+            net::k_conv_s<16, net::EPI_NONE><<<sgrid, 256, 0, st>>>(G, 1, X, wb_ + off_ws[L],
+                                                                   nullptr, nullptr, S);
         net::k_conv<16, kTemporalTaps, net::EPI_BIAS_RELU><<<grid, 256, 0, st>>>(
             G, Tt, S, wb_ + off_wt[L], wm_ + off_b[L], nullptr, Y, rpw);
         X = Y;
@@ -501,7 +509,7 @@ int Net::backward(const NGeo& G, hipStream_t st, float inv_n) {
     const int waves = conv_waves(nrow, &rpw);
     const dim3 grid((unsigned)cdivl(waves, 4));
     const Taps Ts = spatial_taps(G, 1), Tt = temporal_taps(G, 1);
-    const Taps Tsb = spatial_taps(G, -1), Ttb = temporal_taps(G, -1);
+    const Taps Ttb = temporal_taps(G, -1);
     const int nbl = (int)cdivl(nv, 256);
     // loss and projection
     net::k_loss_project_bwd<<<nbl, 256, 0, st>>>(G, out_, target_, lmask_, Y_[kLayers - 1],
@@ -532,8 +540,8 @@ int Net::backward(const NGeo& G, hipStream_t st, float inv_n) {
             net::k_conv_dw<16, kSpatialTaps><<<nblk, 64, 0, st>>>(G, Ts, Y_[L - 1], dB_, partial_, rpb);
             k_reduce_cols<<<kSpatialTaps * 256, 256, 0, st>>>(
                 partial_, nblk, dw_stride, 16, kSpatialTaps * 256, 1.f, grad_ + off_ws[L]);
-            net::k_conv<16, kSpatialTaps, net::EPI_MASK><<<grid, 256, 0, st>>>(
-                G, Tsb, dB_, wtr_ + off_ws[L], nullptr, Y_[L - 1], dA_, rpw);
+            net::k_conv_s<16, net::EPI_MASK><<<dim3(spatial_blocks(G)), 256, 0, st>>>(
+                G, -1, dB_, wtr_ + off_ws[L], nullptr, Y_[L - 1], dA_);
         }
     }
     return check(hipGetLastError(), "net backward");

@@ -166,6 +166,126 @@ __global__ __launch_bounds__(256) void k_conv(NGeo G, Taps T, const bf16* __rest
 }
 
 // ----------------------------------------------------------------------------
+// 3x3x3 spatial convolution with the input patch staged in LDS.  A block of 4
+// waves computes a 16-point x-tile of 4 x 4 rows (j, k) of one frame: it loads
+// the (16 + 2) x (4 + 2) x (4 + 2) points around them (zero outside the grid)
+// into LDS once, so each input point is fetched from L2 once per block
+// instead of once per tap; wave w then computes rows (j0 + jj, k0 + w), jj =
+// 0..3, with four independent accumulators.  The MFMA computes C = W^T X^T:
+// A = weights (lane: out channel m, 8 consecutive k), B = inputs (8 channels
+// of one point), so a lane's four results are four consecutive channels of
+// one point, stored as one 8-byte write.  sign = +1: taps d(t) =
+// (t % 3 - 1, t / 3 % 3 - 1, t / 9 - 1) in (x, y, z); -1: taps -d(t) (the
+// backward-data pass, with per-tap transposed weights).
+// ----------------------------------------------------------------------------
+constexpr int kSpX = 18, kSpY = 6, kSpZ = 6;  // staged patch (points)
+
+template <int CIN, int EPI>
+__global__ __launch_bounds__(256) void k_conv_s(NGeo G, int sign, const bf16* __restrict__ X,
+                                                 const bf16* __restrict__ W,
+                                                 const float* __restrict__ bias,
+                                                 const bf16* __restrict__ mask,
+                                                 bf16* __restrict__ Y) {
+    constexpr int NT = 27;
+    constexpr int TPS = 32 / CIN;                // taps per MFMA K-step
+    constexpr int NS = (NT + TPS - 1) / TPS;     // K-steps
+    constexpr int C8 = CIN / 8;                  // 16-byte chunks per point
+    constexpr int NP = kSpZ * kSpY * kSpX;
+    __shared__ bf16x8 sp[NP * C8];
+    const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = ln >> 4, m = ln & 15;
+    // block -> (x tile, j group, k group, interior frame)
+    const int ntx = (G.I - 2 + 15) / 16, nj = (G.J - 2 + 3) / 4, nk = (G.K - 2 + 3) / 4;
+    long b = blockIdx.x;
+    const int tx = (int)(b % ntx);
+    b /= ntx;
+    const int tj = (int)(b % nj);
+    b /= nj;
+    const int tk = (int)(b % nk);
+    const int lq = (int)(b / nk);
+    const int nlp = G.P - 2;
+    const int l = (lq / nlp) * G.P + (lq % nlp) + 1;
+    const int i0 = 1 + 16 * tx, j0 = 1 + 4 * tj, k0 = 1 + 4 * tk;
+    for (int e = threadIdx.x; e < NP * C8; e += 256) {
+        const int c = e % C8;
+        int r = e / C8;
+        const int x = r % kSpX;
+        r /= kSpX;
+        const int y = r % kSpY, z = r / kSpY;
+        const int i = i0 - 1 + x, j = j0 - 1 + y, k = k0 - 1 + z;
+        bf16x8 v;
+        if (i <= G.I - 1 && j <= G.J - 1 && k <= G.K - 1) {
+            v = *reinterpret_cast<const bf16x8*>(X + nvox(G, i, j, k, l) * CIN + 8 * c);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = (bf16)0.0f;
+        }
+        sp[e] = v;
+    }
+    // A operands: lane holds W[tap][ci][co = m] for k = 8 g .. 8 g + 7
+    bf16x8 wreg[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        bf16x8 a;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int kk = s * 32 + 8 * g + q;
+            const int tap = kk / CIN, ci = kk % CIN;
+            a[q] = tap < NT ? W[(tap * CIN + ci) * kCh + m] : (bf16)0.0f;
+        }
+        wreg[s] = a;
+    }
+    // this lane's tap per K-step as an LDS point offset, and its chunk
+    const int tap_in_step = (8 * g) / CIN, chunk = ((8 * g) % CIN) / 8;
+    int toff[NS];
+    bool tok[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int t = s * TPS + tap_in_step;
+        const int dx = sign * (t % 3 - 1), dy = sign * (t / 3 % 3 - 1), dz = sign * (t / 9 - 1);
+        tok[s] = t < NT;
+        toff[s] = (dz * kSpY + dy) * kSpX + dx;
+    }
+    __syncthreads();
+    f32x4 acc[4];
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) acc[jj] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // patch position of this lane's output point in row jj: (m+1, jj+1, wv+1)
+    const int base = ((wv + 1) * kSpY + 1) * kSpX + (m + 1);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            bf16x8 bv;
+            if (tok[s]) {
+                bv = sp[(base + jj * kSpX + toff[s]) * C8 + chunk];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; q++) bv[q] = (bf16)0.0f;
+            }
+            acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wreg[s], bv, acc[jj], 0, 0, 0);
+        }
+    }
+    const int i = i0 + m, k = k0 + wv;
+    if (i > G.I - 2 || k > G.K - 2) return;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+        const int j = j0 + jj;
+        if (j > G.J - 2) continue;
+        const long dst = nvox(G, i, j, k, l) * kCh + 4 * g;
+        bf16 o[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            float v = acc[jj][r];
+            if (EPI == EPI_BIAS_RELU) v = fmaxf(v + bias[4 * g + r], 0.0f);
+            if (EPI == EPI_MASK) v = ((float)mask[dst + r] > 0.0f) ? v : 0.0f;
+            o[r] = (bf16)v;
+        }
+        *reinterpret_cast<uint2*>(Y + dst) = *reinterpret_cast<const uint2*>(o);
+    }
+}
+
+// ----------------------------------------------------------------------------
 // Weight gradient dW[tap][ci][co] = sum_v X[v + d(tap)][ci] dY[v][co] over the
 // interior points of a range of rows, as per-block partials (FP32, summed by
 // k_reduce_rows).  K of the MFMA is 32 x points of one row; X and dY are
