@@ -302,13 +302,15 @@ int Batch::step(const StepParams& P) {
     } else if (!rc) {
         const dim3 grid((unsigned)grid_), cgrid((unsigned)cgrid_);
         std::vector<int> sweeps(n, 0);
-        auto defect = [&](bool copy32) {
-            if (copy32) {
+        // first defect of the step (with the FP32 coefficient copies), or
+        // the defect after a correction (u += d fused)
+        auto defect = [&](bool first) {
+            if (first) {
                 k_defect_batch<4><<<grid, kBlock, 0, st_>>>(dj64_, n, 0);
                 k_defect_batch<4><<<grid, kBlock, 0, st_>>>(dj64_, n, 1);
             } else {
-                k_defect_batch<3><<<grid, kBlock, 0, st_>>>(dj64_, n, 0);
-                k_defect_batch<3><<<grid, kBlock, 0, st_>>>(dj64_, n, 1);
+                k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, n, 0);
+                k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, n, 1);
             }
             k_defect_decide<<<n, 1024, 0, st_>>>(dj64_, max_rounds);
             int r = check(hipGetLastError(), "batched defect");
@@ -334,9 +336,7 @@ int Batch::step(const StepParams& P) {
             if (rc) break;
             for (int v = 0; v < n; v++)
                 if (active[v]) sweeps[v] += hctl2_[v].zfinal;
-            k_correct_batch<<<cgrid, kBlock, 0, st_>>>(dj64_, dj32_, n);
-            rc = check(hipGetLastError(), "batched correction");
-            if (!rc) rc = defect(false);
+            rc = defect(false);
         }
         for (int v = 0; v < n && !rc; v++) {
             stats[v].z = sweeps[v];

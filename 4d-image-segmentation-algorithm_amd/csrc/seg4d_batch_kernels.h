@@ -90,6 +90,31 @@ static __global__ __launch_bounds__(kBlock) void k_defect_batch(const BatchJob<d
                                            B.red[0], B.opnd[1], B.part_b, B.defect_ob[1]);
 }
 
+// FP64 defect of the corrected state u + d of every running volume (the
+// correction update fused into the defect, as in the single-volume form):
+// red stores u_r + d_r and reads black neighbours u_b + d_b; black stores
+// u_b + d_b with its red neighbours already corrected.
+static __global__ __launch_bounds__(kBlock) void k_defect_corr_batch(
+    const BatchJob<double>* __restrict__ J64, const BatchJob<float>* __restrict__ J32, int nvol,
+    int colour) {
+    const int v = batch_volume(J64, nvol, blockIdx.x);
+    const BatchJob<double>& B = J64[v];
+    if (B.ctl->stop) return;
+    const int lb = blockIdx.x - B.block0;
+    if (lb >= B.nblocks) return;
+    int bx, kb;
+    if (!map_block(B.M, lb, B.nblocks, bx, kb)) return;
+    const BatchJob<float>& D = J32[v];
+    if (colour == 0)
+        sor_block<double, 5, 1, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[0], B.red[0],
+                                       B.black, B.opnd[0], B.part_r, B.defect_ob[0],
+                                       D.red[D.ctl->zfinal & 1], D.black);
+    else
+        sor_block<double, 5, 1, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, B.black,
+                                       B.red[0], B.opnd[1], B.part_b, B.defect_ob[1], D.black,
+                                       nullptr);
+}
+
 // Residual reduction and stopping test of every running volume (one block
 // per volume); the iteration index is the control block's own counter.
 template <typename T>
@@ -157,23 +182,6 @@ static __global__ __launch_bounds__(kBlock) void k_zero_batch(const BatchJob<dou
         B.red[0][q] = 0.0f;
         B.red[1][q] = 0.0f;
         B.black[q] = 0.0f;
-    }
-}
-
-// u += d for running volumes; the accepted red correction is the buffer the
-// last counted red pass of the correction solve wrote.
-static __global__ __launch_bounds__(kBlock) void k_correct_batch(const BatchJob<double>* __restrict__ J64,
-                                                          const BatchJob<float>* __restrict__ J32,
-                                                          int nvol) {
-    const int v = batch_cvolume(J32, nvol, blockIdx.x);
-    if (J64[v].ctl->stop) return;
-    const BatchJob<float>& D = J32[v];
-    const BatchJob<double>& U = J64[v];
-    const float* dr = D.red[D.ctl->zfinal & 1];
-    const long stride = (long)D.cnblocks * kBlock;
-    for (long q = (long)(blockIdx.x - D.cblock0) * kBlock + threadIdx.x; q < D.np; q += stride) {
-        U.red[0][q] = U.red[0][q] + (double)dr[q];
-        U.black[q] = U.black[q] + (double)D.black[q];
     }
 }
 

@@ -191,7 +191,7 @@ public:
     int reduce_to_host(int np, double* out);
     struct SweepMap sweep_map() const;
     int ensure_mixed();
-    int defect64(const StepParams& P, bool copy32);
+    int defect64(const StepParams& P, bool copy32, const float* dred = nullptr);
     int solve_mixed(const StepParams& P, SolveStats* st);
 
     bool ready_ = false;

@@ -342,8 +342,14 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
         hipEventDestroy(e0);
         hipEventDestroy(e1);
     }
-    const int np = PM.gx * PM.gk * PM.gl;  // one partial per (run, k, l)
-    k_init_ctl<<<1, 1024, 0, st_>>>(part_, np, ctl_target ? ctl_target : ctl_, P.gsTol, P.rel2);
+    // one partial per (run, k, l), summed in two fixed-order levels
+    const long np = (long)PM.gx * PM.gk * PM.gl;
+    const int chunk = 4096;
+    const int nb2 = (int)cdiv(np, chunk);
+    if (nb2 > npart_) return check(hipErrorInvalidValue, "setup partials");
+    k_sum_chunks<<<nb2, 1024, 0, st_>>>(part_, np, chunk, part_ + npart_);
+    k_init_ctl<<<1, 1024, 0, st_>>>(part_ + npart_, nb2, ctl_target ? ctl_target : ctl_, P.gsTol,
+                                    P.rel2);
     return launch_check("k_init_ctl");
 }
 
@@ -613,8 +619,10 @@ int Engine::ensure_mixed() {
 // FP64 residual of the reference system for the current colour-layout state
 // (red in ur_[0], black in ub_); writes the FP32 defect into the FP32 operand
 // blocks and the residual into ctl_->R (host copy in hctl_).
-// copy32: also write the FP32 coefficient copies (first defect of a step).
-int Engine::defect64(const StepParams& P, bool copy32) {
+// copy32: also write the FP32 coefficient copies (first defect of a step);
+// dred != null: first apply the correction (red: dred, black: db_), i.e.
+// u += d fused into the defect passes.
+int Engine::defect64(const StepParams& P, bool copy32, const float* dred) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     const SweepMap M = sweep_map();
     const int nb = M.gx * M.gk;
@@ -632,6 +640,11 @@ int Engine::defect64(const StepParams& P, bool copy32) {
     if (copy32) {
         k_sor_sweep<double, 4, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
         k_sor_sweep<double, 4, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);
+    } else if (dred) {
+        // red: own u_r + d_r (stored), black neighbours u_b + d_b; black: own
+        // u_b + d_b (stored), red neighbours already corrected
+        k_sor_sweep<double, 5, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], ur_[0], ub_, op_[0], part_r, nullptr, op32_[0], dred, db_);
+        k_sor_sweep<double, 5, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[0], op_[1], part_b, nullptr, op32_[1], db_, nullptr);
     } else {
         k_sor_sweep<double, 3, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
         k_sor_sweep<double, 3, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);
@@ -689,13 +702,9 @@ int Engine::solve_mixed(const StepParams& P, SolveStats* out) {
         if (sor_run<float>(*this, J, &z)) return -1;
         sweeps += z;
         rounds++;
-        const long nblk = (np_ + kBlock - 1) / kBlock;
-        const unsigned g = (unsigned)(nblk < 8192 ? nblk : 8192);
-        // the accepted red correction is the buffer written by red(z)
-        k_correct<<<g, kBlock, 0, st_>>>(ur_[0], dr_[z & 1], np_);
-        k_correct<<<g, kBlock, 0, st_>>>(ub_, db_, np_);
-        if (launch_check("k_correct")) return -1;
-        if (defect64(P, false)) return -1;
+        // u += d (the accepted red correction is the buffer written by red(z))
+        // fused into the next FP64 defect
+        if (defect64(P, false, dr_[z & 1])) return -1;
     }
     red_final_ = 0;
     if (out) {

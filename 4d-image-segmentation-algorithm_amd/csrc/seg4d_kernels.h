@@ -322,19 +322,30 @@ struct PointIn {
 // NTO: the operand-block loads carry the nontemporal hint (the stream is read
 // once per pass; the hint keeps it from displacing the other colour's planes,
 // which neighbouring blocks re-read, from L2).
-template <typename T, bool NTO = false>
+// A value of the other colour; CORR: plus its FP32 correction dy when given
+// (the corrected state u + d of the FP32-refined form, formed as u[q] + d[q]
+// exactly like the separate correction update).
+template <typename T, bool CORR>
+__device__ __forceinline__ T other(const T* __restrict__ y, const float* __restrict__ dy, int q) {
+    if constexpr (CORR) return dy ? y[q] + (T)dy[q] : y[q];
+    else return y[q];
+}
+
+template <typename T, bool NTO = false, bool CORR = false>
 __device__ __forceinline__ void load_point(const Geo& G, const T* __restrict__ ob,
                                            const T* __restrict__ y, const T* xin, int pp,
-                                           int sft, bool act, PointIn<T>& P) {
+                                           int sft, bool act, PointIn<T>& P,
+                                           const float* __restrict__ dx = nullptr,
+                                           const float* __restrict__ dy = nullptr) {
     P.pp = pp;
     P.sft = sft;
     P.act = act;
     if (act) {
-        P.yo = y[pp + (sft ? 1 : -1)];
-        P.yS = y[pp - G.py];
-        P.yN = y[pp + G.py];
-        P.yB = y[pp - G.pz];
-        P.yT = y[pp + G.pz];
+        P.yo = other<T, CORR>(y, dy, pp + (sft ? 1 : -1));
+        P.yS = other<T, CORR>(y, dy, pp - G.py);
+        P.yN = other<T, CORR>(y, dy, pp + G.py);
+        P.yB = other<T, CORR>(y, dy, pp - G.pz);
+        P.yT = other<T, CORR>(y, dy, pp + G.pz);
         if constexpr (NTO) {
 #pragma unroll
             for (int f = 0; f < 8; f++) P.c[f] = __builtin_nontemporal_load(ob + f * 64);
@@ -344,7 +355,8 @@ __device__ __forceinline__ void load_point(const Geo& G, const T* __restrict__ o
             for (int f = 0; f < 8; f++) P.c[f] = ob[f * 64];
             P.pv = ob[8 * 64];
         }
-        P.uo = xin[pp];
+        if constexpr (CORR) P.uo = xin[pp] + (T)dx[pp];
+        else P.uo = xin[pp];
     }
 }
 
@@ -353,7 +365,8 @@ __device__ __forceinline__ void load_point(const Geo& G, const T* __restrict__ o
 // 290-307) before (RESID 1) or after (RESID 2) the update.  RESID 3 does not
 // update: it writes -r (the defect, FP32) to *defect and accumulates r^2 h^4;
 // RESID 4 also writes the coefficients, rounded to FP32, 8..1 components
-// before *defect.
+// before *defect; RESID 5 is RESID 3 on the corrected state u + d, whose own
+// value it stores (the defect-correction update fused into the next defect).
 template <typename T, int RESID>
 __device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart, T yfw, T c,
                                              T om, double h4, T* xout, float* defect,
@@ -389,6 +402,7 @@ __device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart
 #pragma unroll
                 for (int f = 0; f < 8; f++) defect[(f - 8) * 64] = (float)P.c[f];
             }
+            if constexpr (RESID == 5) xout[P.pp] = P.uo;
             return;
         }
     }
@@ -425,7 +439,9 @@ __device__ __forceinline__ void sor_block(
     const Geo& G, const SweepMap& M, int bx, int kb, T c, T om, double h4, int colour,
     const T* xin, T* xout, const T* __restrict__ y,
     const T* __restrict__ opnd, double* __restrict__ partials,
-    float* __restrict__ defect_ob) {
+    float* __restrict__ defect_ob, const float* __restrict__ dx = nullptr,
+    const float* __restrict__ dy = nullptr) {
+    constexpr bool CORR = RESID == 5;  // dx: own correction, dy: the other colour's (or none)
     const int lb = (kb - 1) * M.gx + bx;
     // block index kb - 1 = l chunk * gk0 + k group; the block's kp planes:
     // threads [sub * per, (sub + 1) * per) take plane kgrp kp + sub + 1, run bx
@@ -451,8 +467,8 @@ __device__ __forceinline__ void sor_block(
     float* def0 = (RESID >= 3) ? defect_ob + ob_off + 8 * 64 : nullptr;
     // window w[q] = other colour at column m, row l-1+q
     T w[U + 2];
-    w[0] = rowok ? y[base + (l0 - 1) * G.pw] : T(0);
-    w[1] = rowok ? y[base + l0 * G.pw] : T(0);
+    w[0] = rowok ? other<T, CORR>(y, dy, base + (l0 - 1) * G.pw) : T(0);
+    w[1] = rowok ? other<T, CORR>(y, dy, base + l0 * G.pw) : T(0);
     const int lmax = min(l0 + M.lc - 1, G.L - 2);
     int l = l0;
     for (; l + U - 1 <= lmax; l += U) {
@@ -461,11 +477,11 @@ __device__ __forceinline__ void sor_block(
         for (int q = 0; q < U; q++) {
             const int lq = l + q;
             const int pp = base + lq * G.pw;
-            w[q + 2] = rowok ? y[pp + G.pw] : T(0);
+            w[q + 2] = rowok ? other<T, CORR>(y, dy, pp + G.pw) : T(0);
             const int sft = colour ^ ((j + k + lq) & 1);
             const int i = 2 * m + sft;
-            load_point<T, NTO>(G, ob0 + (lq - 1) * ostride, y, xin, pp, sft,
-                               rowok && i >= 1 && i <= G.I - 2, P[q]);
+            load_point<T, NTO, CORR>(G, ob0 + (lq - 1) * ostride, y, xin, pp, sft,
+                                     rowok && i >= 1 && i <= G.I - 2, P[q], dx, dy);
         }
 #pragma unroll
         for (int q = 0; q < U; q++)
@@ -477,11 +493,11 @@ __device__ __forceinline__ void sor_block(
     for (; l <= lmax; l++) {  // remainder, one step at a time
         PointIn<T> P;
         const int pp = base + l * G.pw;
-        w[2] = rowok ? y[pp + G.pw] : T(0);
+        w[2] = rowok ? other<T, CORR>(y, dy, pp + G.pw) : T(0);
         const int sft = colour ^ ((j + k + l) & 1);
         const int i = 2 * m + sft;
-        load_point<T, NTO>(G, ob0 + (l - 1) * ostride, y, xin, pp, sft,
-                           rowok && i >= 1 && i <= G.I - 2, P);
+        load_point<T, NTO, CORR>(G, ob0 + (l - 1) * ostride, y, xin, pp, sft,
+                                 rowok && i >= 1 && i <= G.I - 2, P, dx, dy);
         update_point<T, RESID>(P, w[0], w[1], w[2], c, om, h4, xout,
                                def0 ? def0 + (l - 1) * ostride : nullptr, acc);
         w[0] = w[1];
@@ -498,19 +514,13 @@ static __global__ __launch_bounds__(kBlock) void k_sor_sweep(
     Geo G, SweepMap M, T c, T om, double h4, int colour,
     const T* xin, T* xout, const T* __restrict__ y,
     const T* __restrict__ opnd, double* __restrict__ partials,
-    const SorCtl* __restrict__ ctl, float* __restrict__ defect_ob) {
+    const SorCtl* __restrict__ ctl, float* __restrict__ defect_ob,
+    const float* __restrict__ dx = nullptr, const float* __restrict__ dy = nullptr) {
     if (ctl != nullptr && ctl->stop) return;
     int bx, kb;
     if (!map_block(M, blockIdx.x, gridDim.x, bx, kb)) return;
     sor_block<T, RESID, U, NTO>(G, M, bx, kb, c, om, h4, colour, xin, xout, y, opnd, partials,
-                                defect_ob);
-}
-
-// u += d over a colour array (the defect-correction update; ghosts of d are 0).
-static __global__ __launch_bounds__(kBlock) void k_correct(double* __restrict__ u,
-                                                   const float* __restrict__ d, long n) {
-    for (long q = (long)blockIdx.x * kBlock + threadIdx.x; q < n; q += (long)gridDim.x * kBlock)
-        u[q] = u[q] + (double)d[q];
+                                defect_ob, dx, dy);
 }
 
 // Sets the control block of one FP32 correction solve: stop when its residual
@@ -563,6 +573,25 @@ static __global__ __launch_bounds__(1024) void k_reduce_resid(
             ctl->stop = 1;
             ctl->zfinal = z;
         }
+    }
+}
+
+// First level of a two-level fixed-order sum: block b of 1024 threads sums
+// in[b * chunk .. min(n, (b + 1) * chunk)) into out[b].
+static __global__ __launch_bounds__(1024) void k_sum_chunks(const double* __restrict__ in, long n,
+                                                           int chunk, double* __restrict__ out) {
+    const long q0 = (long)blockIdx.x * chunk;
+    const long q1 = q0 + chunk < n ? q0 + chunk : n;
+    double s = 0.0;
+    for (long q = q0 + threadIdx.x; q < q1; q += 1024) s += in[q];
+    s = wave_sum(s);
+    __shared__ double sm[16];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double S = sm[0];
+        for (int q = 1; q < 16; q++) S += sm[q];
+        out[blockIdx.x] = S;
     }
 }
 
