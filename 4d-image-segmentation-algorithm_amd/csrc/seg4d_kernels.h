@@ -134,6 +134,75 @@ __device__ __forceinline__ double face_sq(const double* __restrict__ u, int p,
     return acc;
 }
 
+// The 33-point stencil of u around p (centre, 8 faces, 24 edges) loaded into
+// registers before any arithmetic, so that all its loads are in flight
+// together: s[dw+1][dz+1][dy+1][dx+1] (only points with at most two nonzero
+// offsets are loaded).  The face sums are then formed exactly as face_sq does.
+struct Stencil33 {
+    double s[3][3][3][3];
+};
+
+__device__ __forceinline__ void load_stencil33(const double* __restrict__ u, int p,
+                                               const Geo& G, Stencil33& T) {
+    const int st[4] = {1, G.sy, G.sz, G.sw};
+#pragma unroll
+    for (int w = 0; w < 3; w++)
+#pragma unroll
+        for (int z = 0; z < 3; z++)
+#pragma unroll
+            for (int y = 0; y < 3; y++)
+#pragma unroll
+                for (int x = 0; x < 3; x++) {
+                    const int nz = (x != 1) + (y != 1) + (z != 1) + (w != 1);
+                    if (nz <= 2)
+                        T.s[w][z][y][x] = u[p + (x - 1) * st[0] + (y - 1) * st[1] +
+                                            (z - 1) * st[2] + (w - 1) * st[3]];
+                }
+}
+
+template <int F, bool POW2>
+__device__ __forceinline__ double face_sq_reg(const Stencil33& T, const Scale& S) {
+    constexpr int A = F >> 1;
+    constexpr int sa = (F & 1) ? -1 : 1;
+    // value at offset o (o[axis] in {-1, 0, 1})
+    auto at = [&](int o0, int o1, int o2, int o3) -> double {
+        return T.s[o3 + 1][o2 + 1][o1 + 1][o0 + 1];
+    };
+    auto off = [&](int axis, int v, int q) -> int { return axis == q ? v : 0; };
+    double acc = 0.0;
+#pragma unroll
+    for (int B = 0; B < 4; B++) {
+        double c;
+        if (B == A) {
+            c = scl<POW2>(at(off(A, sa, 0), off(A, sa, 1), off(A, sa, 2), off(A, sa, 3)) -
+                              at(0, 0, 0, 0),
+                          S.h, S.inv_h);
+        } else {
+            const double pApB = at(off(A, sa, 0) + off(B, 1, 0), off(A, sa, 1) + off(B, 1, 1),
+                                   off(A, sa, 2) + off(B, 1, 2), off(A, sa, 3) + off(B, 1, 3));
+            const double pB = at(off(B, 1, 0), off(B, 1, 1), off(B, 1, 2), off(B, 1, 3));
+            const double pAmB = at(off(A, sa, 0) - off(B, 1, 0), off(A, sa, 1) - off(B, 1, 1),
+                                   off(A, sa, 2) - off(B, 1, 2), off(A, sa, 3) - off(B, 1, 3));
+            const double mB = at(-off(B, 1, 0), -off(B, 1, 1), -off(B, 1, 2), -off(B, 1, 3));
+            c = scl<POW2>((pApB + pB) - (pAmB + mB), S.q, S.inv_q);
+        }
+        acc = (B == 0) ? c * c : acc + c * c;
+    }
+    return acc;
+}
+
+template <bool POW2>
+__device__ __forceinline__ void all_face_sq_reg(const Stencil33& T, const Scale& S, double a[8]) {
+    a[0] = face_sq_reg<0, POW2>(T, S);
+    a[1] = face_sq_reg<1, POW2>(T, S);
+    a[2] = face_sq_reg<2, POW2>(T, S);
+    a[3] = face_sq_reg<3, POW2>(T, S);
+    a[4] = face_sq_reg<4, POW2>(T, S);
+    a[5] = face_sq_reg<5, POW2>(T, S);
+    a[6] = face_sq_reg<6, POW2>(T, S);
+    a[7] = face_sq_reg<7, POW2>(T, S);
+}
+
 template <bool POW2>
 __device__ __forceinline__ void all_face_sq(const double* __restrict__ u, int p,
                                             const Geo& G, const Scale& S,
@@ -235,8 +304,15 @@ static __global__ __launch_bounds__(kBlock) void k_setup_step(
             const long oi = opnd_index(G, k, l, (j - 1) * G.H + (i >> 1));
             double* ob = (c ? sb.opnd[1] : sb.opnd[0]) + oi;
             ob[8 * 64] = uc;
+            Stencil33 T;
+            load_stencil33(u, p, G, T);
+            double gl[8];
+            if constexpr (WITH_G) {
+#pragma unroll
+                for (int f = 0; f < 8; f++) gl[f] = g.p[f][p];
+            }
             double a[8];
-            all_face_sq<POW2>(u, p, G, S, a);
+            all_face_sq_reg<POW2>(T, S, a);
             double N[8];
 #pragma unroll
             for (int f = 0; f < 8; f++) N[f] = sqrt(a[f] + eps2);
@@ -246,7 +322,7 @@ static __global__ __launch_bounds__(kBlock) void k_setup_step(
 #pragma unroll
             for (int f = 0; f < 8; f++) {
                 double cf;
-                if constexpr (WITH_G) cf = vc * g.p[f][p] * (1.0 / N[f]);
+                if constexpr (WITH_G) cf = vc * gl[f] * (1.0 / N[f]);
                 else cf = vc * (1.0 / N[f]);
                 ob[f * 64] = cf;
             }
