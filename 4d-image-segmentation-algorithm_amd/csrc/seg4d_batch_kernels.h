@@ -83,10 +83,10 @@ static __global__ __launch_bounds__(kBlock) void k_defect_batch(const BatchJob<d
     int bx, kb;
     if (!map_block(B.M, lb, B.nblocks, bx, kb)) return;
     if (colour == 0)
-        sor_block<double, RESID, 1, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[0], nullptr,
+        sor_block<double, RESID, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[0], nullptr,
                                            B.black, B.opnd[0], B.part_r, B.defect_ob[0]);
     else
-        sor_block<double, RESID, 1, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, nullptr,
+        sor_block<double, RESID, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, nullptr,
                                            B.red[0], B.opnd[1], B.part_b, B.defect_ob[1]);
 }
 
@@ -106,11 +106,11 @@ static __global__ __launch_bounds__(kBlock) void k_defect_corr_batch(
     if (!map_block(B.M, lb, B.nblocks, bx, kb)) return;
     const BatchJob<float>& D = J32[v];
     if (colour == 0)
-        sor_block<double, 5, 1, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[0], B.red[0],
+        sor_block<double, 5, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[0], B.red[0],
                                        B.black, B.opnd[0], B.part_r, B.defect_ob[0],
                                        D.red[D.ctl->zfinal & 1], D.black);
     else
-        sor_block<double, 5, 1, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, B.black,
+        sor_block<double, 5, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, B.black,
                                        B.red[0], B.opnd[1], B.part_b, B.defect_ob[1], D.black,
                                        nullptr);
 }

@@ -638,16 +638,16 @@ int Engine::defect64(const StepParams& P, bool copy32, const float* dred) {
         (void)hipEventRecord(e0, st_);
     }
     if (copy32) {
-        k_sor_sweep<double, 4, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
-        k_sor_sweep<double, 4, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);
+        k_sor_sweep<double, 4, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
+        k_sor_sweep<double, 4, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);
     } else if (dred) {
         // red: own u_r + d_r (stored), black neighbours u_b + d_b; black: own
         // u_b + d_b (stored), red neighbours already corrected
-        k_sor_sweep<double, 5, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], ur_[0], ub_, op_[0], part_r, nullptr, op32_[0], dred, db_);
-        k_sor_sweep<double, 5, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[0], op_[1], part_b, nullptr, op32_[1], db_, nullptr);
+        k_sor_sweep<double, 5, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], ur_[0], ub_, op_[0], part_r, nullptr, op32_[0], dred, db_);
+        k_sor_sweep<double, 5, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[0], op_[1], part_b, nullptr, op32_[1], db_, nullptr);
     } else {
-        k_sor_sweep<double, 3, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
-        k_sor_sweep<double, 3, 1><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);
+        k_sor_sweep<double, 3, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
+        k_sor_sweep<double, 3, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);
     }
     if (launch_check("k_sor_sweep defect")) return -1;
     if (prof.on) (void)hipEventRecord(e1, st_);
