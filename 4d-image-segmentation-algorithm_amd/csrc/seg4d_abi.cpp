@@ -694,7 +694,7 @@ API void seg4dNetScales(float* s_d, float* s_t) {
 API int seg4dSetSweepVariant(int v) {
     if (!g_eng) return -1;
     const int old = g_eng->sweep_variant_;
-    g_eng->sweep_variant_ = (v >= 0 && v <= 31 && (v & 24) != 24) ? v : 0;
+    g_eng->sweep_variant_ = (v >= 0 && v <= 63 && (v & 24) != 24) ? v : 0;
     return old;
 }
 
@@ -744,8 +744,9 @@ API int seg4dAutotune(int reps, int* runs, int* planes, float* best_ms) {
         fail("seg4dAutotune");
         return -1;
     }
-    if (runs) *runs = (g_eng->sweep_variant_ & 2) ? g_eng->sweep_tb_ : 0;
-    if (planes) *planes = (g_eng->sweep_variant_ & 2) ? g_eng->sweep_tk_ : 0;
+    const bool tiled = g_eng->tiled_order();
+    if (runs) *runs = tiled ? g_eng->sweep_tb_ : 0;
+    if (planes) *planes = tiled ? g_eng->sweep_tk_ : 0;
     if (best_ms) *best_ms = ms;
     return 0;
 }
@@ -848,7 +849,8 @@ API int seg4dBatchAutotune(int reps, float* best_ms) {
 API int seg4dBatchTile(int v, int* runs, int* planes) {
     if (!need_batch("seg4dBatchTile") || v < 0 || v >= g_batch->size()) return -1;
     s4d::Engine& E = g_batch->vol(v);
-    if (runs) *runs = (E.sweep_variant_ & 2) ? E.sweep_tb_ : 0;
-    if (planes) *planes = (E.sweep_variant_ & 2) ? E.sweep_tk_ : 0;
+    const bool tiled = E.tiled_order();
+    if (runs) *runs = tiled ? E.sweep_tb_ : 0;
+    if (planes) *planes = tiled ? E.sweep_tk_ : 0;
     return 0;
 }

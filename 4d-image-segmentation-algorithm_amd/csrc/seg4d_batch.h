@@ -79,7 +79,7 @@ private:
     BatchJob<float>* dj32_ = nullptr;
     std::vector<BatchJob<double>> j64_;
     std::vector<BatchJob<float>> j32_;
-    int grid_ = 0, cgrid_ = 0;
+    int grid_ = 0, grid32_ = 0, cgrid_ = 0;  // FP64 / FP32 SOR launches, element-wise
     bool jobs_ok_ = false;
     double jobs_c_ = 0.0, jobs_om_ = 0.0, jobs_om32_ = 0.0, jobs_h4_ = 0.0;
     int jobs_prec_ = -1;

@@ -112,14 +112,17 @@ int Batch::build_jobs(const StepParams& P) {
         return 0;
     j64_.assign(n, BatchJob<double>());
     j32_.assign(n, BatchJob<float>());
-    long b0 = 0, cb0 = 0;
+    long b0 = 0, b032 = 0, cb0 = 0;
     for (int v = 0; v < n; v++) {
         Engine& E = *E_[v];
         if (precision == 1 && E.ensure_mixed()) return -1;
-        const SweepMap M = E.sweep_map();
+        // the launch maps of the volume's own FP64 and FP32 passes
+        const SweepMap M = E.sweep_map(8), M32 = E.sweep_map(4);
         const int nbk = M.tiled ? ((M.ntiles + 7) / 8) * 8 * M.tb * M.tk : M.gx * M.gk;
-        const int nbp = M.gx * M.gk;
-        if ((long)nbp > E.npart_) {
+        const int nbk32 =
+            M32.tiled ? ((M32.ntiles + 7) / 8) * 8 * M32.tb * M32.tk : M32.gx * M32.gk;
+        const int nbp = M.gx * M.gk, nbp32 = M32.gx * M32.gk;
+        if ((long)nbp > E.npart_ || (long)nbp32 > E.npart_) {
             err = "batch: SOR grid exceeds a volume's partial buffer";
             set_error(err);
             return -1;
@@ -128,12 +131,16 @@ int Batch::build_jobs(const StepParams& P) {
         BatchJob<double>& A = j64_[v];
         BatchJob<float>& B = j32_[v];
         A.G = B.G = E.geometry();
-        A.M = B.M = M;
-        A.block0 = B.block0 = (int)b0;
-        A.nblocks = B.nblocks = nbk;
+        A.M = M;
+        B.M = M32;
+        A.block0 = (int)b0;
+        B.block0 = (int)b032;
+        A.nblocks = nbk;
+        B.nblocks = nbk32;
         A.cblock0 = B.cblock0 = (int)cb0;
         A.cnblocks = B.cnblocks = cnb;
         b0 += ((nbk + 7) / 8) * 8;  // volumes start on XCD 0
+        b032 += ((nbk32 + 7) / 8) * 8;
         cb0 += cnb;
         A.c = c;
         A.om = P.omega;
@@ -147,7 +154,8 @@ int Batch::build_jobs(const StepParams& P) {
         A.black = E.ub_;
         A.part_r = B.part_r = E.part_;
         A.part_b = B.part_b = E.part_ + E.npart_;
-        A.nb = B.nb = nbp;
+        A.nb = nbp;
+        B.nb = nbp32;
         A.np = B.np = E.np_;
         A.ctl = ctl_ + v;
         B.ctl = ctl2_ + v;
@@ -160,12 +168,13 @@ int Batch::build_jobs(const StepParams& P) {
         B.red[1] = E.dr_[1];
         B.black = E.db_;
     }
-    if (b0 >= (1L << 31) || cb0 >= (1L << 31)) {
+    if (b0 >= (1L << 31) || b032 >= (1L << 31) || cb0 >= (1L << 31)) {
         err = "batch: grid too large";
         set_error(err);
         return -1;
     }
     grid_ = (int)b0;
+    grid32_ = (int)b032;
     cgrid_ = (int)cb0;
     if (check(hipMemcpyAsync(dj64_, j64_.data(), (size_t)n * sizeof(BatchJob<double>),
                              hipMemcpyHostToDevice, st_), "jobs") ||
@@ -196,7 +205,7 @@ int Batch::sor_run(std::vector<BatchJob<T>>& hj, BatchJob<T>* dj, SorCtl* dctl, 
                    int cap, hipGraphExec_t* gexec, int* gcap) {
     (void)hj;
     const int n = size();
-    const dim3 grid((unsigned)grid_);
+    const dim3 grid((unsigned)(sizeof(T) == 4 ? grid32_ : grid_));
     int rc = 0;
     auto red = [&](int zr, int zw, bool resid) {
         if (resid) k_sor_batch<T, 1, 2><<<grid, kBlock, 0, st_>>>(dj, n, 0, zr, zw);

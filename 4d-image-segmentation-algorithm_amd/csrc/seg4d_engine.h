@@ -149,8 +149,11 @@ public:
     // SOR pass kernel variant (tuning knob, identical results), bit flags:
     // 1 = two l-steps per load phase, 2 = tiled XCD-resident launch order,
     // 4 = nontemporal operand-block loads, 8 = two k planes per block,
-    // 16 = four k planes per block.
-    int sweep_variant_ = 3;
+    // 16 = four k planes per block, 32 = automatic order: with automatic
+    // frame chunks (sweep_lc_ = 0), chunks of 2 frames in the linear order
+    // whenever the other colour's sliding window (3 k planes x 4 frames)
+    // fits in an XCD's L2, which replaces bit 2 and the chunk rule.
+    int sweep_variant_ = 35;
     // tile of the tiled order in blocks (runs x k groups); 16 x 8 measured
     // fastest at 256x256x128x16 (8 x 32: 1.401 ms, 16 x 16: 1.352 ms, 16 x 8:
     // 1.347 ms per red pass)
@@ -189,7 +192,9 @@ public:
     int launch_check(const char* what);
     int sumsq_interior(const double* d, double h, double* out);
     int reduce_to_host(int np, double* out);
-    struct SweepMap sweep_map() const;
+    // elem_bytes: 8 for the FP64 passes, 4 for the FP32 correction passes
+    struct SweepMap sweep_map(int elem_bytes = 8) const;
+    bool tiled_order() const;  // the FP64 passes run in the tiled order
     int ensure_mixed();
     int defect64(const StepParams& P, bool copy32, const float* dred = nullptr);
     int solve_mixed(const StepParams& P, SolveStats* st);

@@ -184,12 +184,17 @@ double seg4dLastTolerance(void);
 void seg4dProfile(int on);
 int seg4dProfileRead(double *out, int nout);
 void seg4dProfileReset(void);
-/* SOR pass kernel variant (tuning knob, identical results), bit flags:
- * 1 = two l-steps per load phase; 2 = tiled launch order (the blocks that
- * share neighbour planes are co-resident on one XCD); 4 = nontemporal loads
- * of the coefficient stream; 8 = two, 16 = four consecutive k planes per
- * block (8 and 16 exclusive).  Default 3.  Returns the previous value, -1
- * before _allocateMem_. */
+/* SOR pass kernel variant (tuning knob), bit flags: 1 = two l-steps per
+ * load phase; 2 = tiled launch order (the blocks that share neighbour planes
+ * are co-resident on one XCD); 4 = nontemporal loads of the coefficient
+ * stream; 8 = two, 16 = four consecutive k planes per block (8 and 16
+ * exclusive); 32 = automatic order: with automatic frame chunks
+ * (seg4dSetSweepChunk(0)), chunks of 2 frames in the linear order whenever
+ * the other colour's window of 3 k planes x 4 frames fits in 3.5 MB (one
+ * XCD's L2), replacing bit 2.  Default 35.  u and the sweep counts are those
+ * of any other variant; bits 8/16 and the frame chunk regroup the residual
+ * sums (last-bit differences in R).  Returns the previous value, -1 before
+ * _allocateMem_. */
 int seg4dSetSweepVariant(int v);
 /* Tile of the tiled launch order in blocks: `runs` runs x `planes` k-plane
  * groups per XCD residency round (default 16 x 8).  0 on success. */

@@ -34,7 +34,7 @@ def _single(lib, dims, img, u0, precision, warm, steps):
     out = []
     lib.allocate(*dims)
     # the batch's engines run the library defaults
-    lib.set_sweep_variant(3)
+    lib.set_sweep_variant(35)
     lib.lib.seg4dSetSweepTile(16, 8)
     lib.lib.seg4dSetSweepChunk(0)
     lib.lib.seg4dSetPipelineDepth(0)

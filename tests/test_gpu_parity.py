@@ -198,7 +198,7 @@ def test_fixed_sweep_count(seglib, oracle, sweeps, variant, chunk, depth):
         assert z_g == z_o == sweeps
         assert np.array_equal(lib.arr("u"), u)
     finally:
-        lib.set_sweep_variant(3)
+        lib.set_sweep_variant(35)
         lib.lib.seg4dSetSweepChunk(0)
         lib.lib.seg4dSetPipelineDepth(0)
         lib.deallocate()
@@ -238,7 +238,7 @@ def test_tolerance_driven_variants(seglib, oracle, variant, dims, chunk, depth):
             assert z_g == z_o
             assert np.array_equal(lib.arr("u"), u)
     finally:
-        lib.set_sweep_variant(3)
+        lib.set_sweep_variant(35)
         lib.lib.seg4dSetSweepChunk(0)
         lib.lib.seg4dSetPipelineDepth(0)
         lib.deallocate()
