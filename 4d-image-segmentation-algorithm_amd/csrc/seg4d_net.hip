@@ -483,6 +483,25 @@ int Net::forward(const NGeo& G, hipStream_t st, bool training) {
     const Taps Tt = temporal_taps(G, 1);
     const dim3 sgrid(spatial_blocks(G));
     const bf16* X = feat_;
+    if (!training) {
+        // inference: one fused kernel per layer, S kept on chip
+        const long nlb = (long)((G.I - 2 + 15) / 16) * ((G.J - 2 + 3) / 4) * ((G.K - 2 + 3) / 4) *
+                         (G.L / G.P);
+        for (int L = 0; L < kLayers; L++) {
+            bf16* Y = Y_[L & 1];
+            if (L == 0)
+                net::k_layer_fwd<8><<<dim3((unsigned)nlb), 256, 0, st>>>(
+                    G, X, wb_ + off_ws[L], wb_ + off_wt[L], wm_ + off_b[L], Y);
+            else
+                net::k_layer_fwd<16><<<dim3((unsigned)nlb), 256, 0, st>>>(
+                    G, X, wb_ + off_ws[L], wb_ + off_wt[L], wm_ + off_b[L], Y);
+            X = Y;
+        }
+        const long nv = (long)G.I * G.J * G.K * G.L;
+        net::k_project<<<(unsigned)cdivl(nv, 256), 256, 0, st>>>(G, X, wm_ + off_wp,
+                                                                 wm_ + off_bp, out_);
+        return check(hipGetLastError(), "net forward");
+    }
     for (int L = 0; L < kLayers; L++) {
         bf16* S = training ? S_[L] : S_[0];
         bf16* Y = training ? Y_[L] : Y_[L & 1];

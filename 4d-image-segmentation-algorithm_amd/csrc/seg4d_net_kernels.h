@@ -286,6 +286,177 @@ __global__ __launch_bounds__(256) void k_conv_s(NGeo G, int sign, const bf16* __
 }
 
 // ----------------------------------------------------------------------------
+// One inference layer fused: S(t) = spatial conv of X(t) (k_conv_s's tiling
+// and staging), then Y(t) = ReLU(bias + sum_dt Wt[dt] S(t + dt)), marching t
+// through each frame period.  A block keeps S of the frames t-2, t-1, t for
+// its 256 points in an LDS ring (ghost frames zero), so S never goes to
+// memory; the next frame's input patch is loaded into registers while the
+// current one is computed.  Each wave reads and writes only its own part of
+// the ring; the patch is shared by the 4 waves.
+// ----------------------------------------------------------------------------
+template <int CIN>
+__global__ __launch_bounds__(256) void k_layer_fwd(NGeo G, const bf16* __restrict__ X,
+                                                    const bf16* __restrict__ Ws,
+                                                    const bf16* __restrict__ Wt,
+                                                    const float* __restrict__ bias,
+                                                    bf16* __restrict__ Y) {
+    constexpr int NT = 27;
+    constexpr int TPS = 32 / CIN;
+    constexpr int NS = (NT + TPS - 1) / TPS;
+    constexpr int C8 = CIN / 8;
+    constexpr int NP = kSpZ * kSpY * kSpX;
+    constexpr int NE = NP * C8;
+    constexpr int PER = (NE + 255) / 256;        // patch chunks per thread
+    __shared__ bf16x8 sp[NE];
+    __shared__ bf16x8 ring[3][4][4][16][2];      // [t % 3][wave][row tile][point][8 channels]
+    const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = ln >> 4, m = ln & 15;
+    const int ntx = (G.I - 2 + 15) / 16, nj = (G.J - 2 + 3) / 4, nk = (G.K - 2 + 3) / 4;
+    long b = blockIdx.x;
+    const int tx = (int)(b % ntx);
+    b /= ntx;
+    const int tj = (int)(b % nj);
+    b /= nj;
+    const int tk = (int)(b % nk);
+    const int period = (int)(b / nk);
+    const int lbase = period * G.P;  // ghost frame 0 of this period
+    const int i0 = 1 + 16 * tx, j0 = 1 + 4 * tj, k0 = 1 + 4 * tk;
+    // spatial and temporal A operands, bias of this lane's 4 channels
+    bf16x8 ws[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        bf16x8 a;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int kk = s * 32 + 8 * g + q;
+            const int tap = kk / CIN, ci = kk % CIN;
+            a[q] = tap < NT ? Ws[(tap * CIN + ci) * kCh + m] : (bf16)0.0f;
+        }
+        ws[s] = a;
+    }
+    bf16x8 wt[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        bf16x8 a;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int kk = s * 32 + 8 * g + q;
+            const int tap = kk / kCh, ci = kk % kCh;
+            a[q] = tap < 3 ? Wt[(tap * kCh + ci) * kCh + m] : (bf16)0.0f;
+        }
+        wt[s] = a;
+    }
+    float bco[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) bco[r] = bias[4 * g + r];
+    const int tap_in_step = (8 * g) / CIN, chunk = ((8 * g) % CIN) / 8;
+    int toff[NS];
+    bool tok[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int t = s * TPS + tap_in_step;
+        toff[s] = ((t / 9 - 1) * kSpY + (t / 3 % 3 - 1)) * kSpX + (t % 3 - 1);
+        tok[s] = t < NT;
+    }
+    const int base = ((wv + 1) * kSpY + 1) * kSpX + (m + 1);
+    bf16x8 zero8;
+#pragma unroll
+    for (int q = 0; q < 8; q++) zero8[q] = (bf16)0.0f;
+    // S of the ghost frame 0
+    if (g < 2) {
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) ring[0][wv][jj][m][g] = zero8;
+    }
+    auto fetch = [&](int l, bf16x8* pre) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = threadIdx.x + 256 * q;
+            bf16x8 v = zero8;
+            if (e < NE) {
+                const int c = e % C8;
+                int r = e / C8;
+                const int x = r % kSpX;
+                r /= kSpX;
+                const int y = r % kSpY, z = r / kSpY;
+                const int i = i0 - 1 + x, j = j0 - 1 + y, k = k0 - 1 + z;
+                if (i <= G.I - 1 && j <= G.J - 1 && k <= G.K - 1)
+                    v = *reinterpret_cast<const bf16x8*>(X + nvox(G, i, j, k, lbase + l) * CIN +
+                                                         8 * c);
+            }
+            pre[q] = v;
+        }
+    };
+    bf16x8 pre[PER];
+    fetch(1, pre);
+    const int i = i0 + m, k = k0 + wv;
+    const bool outok = i <= G.I - 2 && k <= G.K - 2;
+    for (int l = 1; l <= G.P - 1; l++) {
+        const bool real = l <= G.P - 2;  // frame P-1 is a ghost: S = 0
+        // (every wave finished reading the previous patch before the last barrier)
+        if (real) {
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int e = threadIdx.x + 256 * q;
+                if (e < NE) sp[e] = pre[q];
+            }
+        }
+        if (l + 1 <= G.P - 2) fetch(l + 1, pre);
+        __syncthreads();
+        f32x4 acc[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) acc[jj] = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (real) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    const bf16x8 bv = tok[s] ? sp[(base + jj * kSpX + toff[s]) * C8 + chunk] : zero8;
+                    acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ws[s], bv, acc[jj], 0, 0, 0);
+                }
+            }
+        }
+        // S(l) as BF16 into the ring: this lane's channels 4g..4g+3 of point m
+        const int sl = l % 3;
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+            bf16 o[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) o[r] = (bf16)acc[jj][r];
+            *reinterpret_cast<uint2*>(reinterpret_cast<bf16*>(&ring[sl][wv][jj][m][0]) + 4 * g) =
+                *reinterpret_cast<const uint2*>(o);
+        }
+        __syncthreads();
+        // temporal conv of frame l - 1 from S(l-2), S(l-1), S(l)
+        if (l >= 2) {
+            f32x4 acc2[4];
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) acc2[jj] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                const int tap = 2 * s + g / 2;  // K = tap * 16 + channel
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    const bf16x8 bv = tap < 3 ? ring[(l - 2 + tap) % 3][wv][jj][m][g & 1] : zero8;
+                    acc2[jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wt[s], bv, acc2[jj], 0, 0, 0);
+                }
+            }
+            if (outok) {
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    const int j = j0 + jj;
+                    if (j > G.J - 2) continue;
+                    const long dst = nvox(G, i, j, k, lbase + l - 1) * kCh + 4 * g;
+                    bf16 o[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) o[r] = (bf16)fmaxf(acc2[jj][r] + bco[r], 0.0f);
+                    *reinterpret_cast<uint2*>(Y + dst) = *reinterpret_cast<const uint2*>(o);
+                }
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
 // Weight gradient dW[tap][ci][co] = sum_v X[v + d(tap)][ci] dY[v][co] over the
 // interior points of a range of rows, as per-block partials (FP32, summed by
 // k_reduce_rows).  K of the MFMA is 32 x points of one row; X and dY are
