@@ -292,7 +292,7 @@ int Batch::step(const StepParams& P) {
     if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return -1;
     int rc = check(hipEventRecord(e0, st_), "event");
     for (int v = 0; v < n && !rc; v++)
-        if (E_[v]->setup(P, true, true, ctl_ + v)) {
+        if (E_[v]->setup(P, true, true, ctl_ + v, precision == 1)) {
             err = E_[v]->err;
             rc = -1;
         }
@@ -311,23 +311,22 @@ int Batch::step(const StepParams& P) {
     } else if (!rc) {
         const dim3 grid((unsigned)grid_), cgrid((unsigned)cgrid_);
         std::vector<int> sweeps(n, 0);
-        // first defect of the step (with the FP32 coefficient copies), or
-        // the defect after a correction (u += d fused)
-        auto defect = [&](bool first) {
-            if (first) {
-                k_defect_batch<4><<<grid, kBlock, 0, st_>>>(dj64_, n, 0);
-                k_defect_batch<4><<<grid, kBlock, 0, st_>>>(dj64_, n, 1);
-            } else {
-                k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, n, 0);
-                k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, n, 1);
-            }
-            k_defect_decide<<<n, 1024, 0, st_>>>(dj64_, max_rounds);
+        // the first defect was formed by each volume's set-up kernel; later
+        // ones follow a correction (u += d fused into the defect)
+        auto sync_ctl = [&]() {
             int r = check(hipGetLastError(), "batched defect");
             r |= check(hipMemcpyAsync(hctl_, ctl_, (size_t)n * sizeof(SorCtl),
                                       hipMemcpyDeviceToHost, st_), "ctl copy");
             return r | check(hipStreamSynchronize(st_), "defect sync");
         };
-        rc = defect(true);
+        auto defect = [&]() {
+            k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, n, 0);
+            k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, n, 1);
+            k_defect_decide<<<n, 1024, 0, st_>>>(dj64_, max_rounds);
+            return sync_ctl();
+        };
+        k_decide_batch<<<(n + 63) / 64, 64, 0, st_>>>(dj64_, n);
+        rc = sync_ctl();
         for (;;) {
             if (rc) break;
             std::vector<char> active(n);
@@ -345,7 +344,7 @@ int Batch::step(const StepParams& P) {
             if (rc) break;
             for (int v = 0; v < n; v++)
                 if (active[v]) sweeps[v] += hctl2_[v].zfinal;
-            rc = defect(false);
+            rc = defect();
         }
         for (int v = 0; v < n && !rc; v++) {
             stats[v].z = sweeps[v];

@@ -69,27 +69,6 @@ static __global__ __launch_bounds__(kBlock) void k_sor_batch(const BatchJob<T>* 
                                       B.red[zr], B.opnd[1], B.part_b, nullptr);
 }
 
-// FP64 defect pass (RESID 3, or 4 with the FP32 coefficient copies) of every
-// volume whose FP64 control block has not stopped; the state is red[0] and
-// black (the FP32-refined form keeps no red ping-pong).
-template <int RESID>
-static __global__ __launch_bounds__(kBlock) void k_defect_batch(const BatchJob<double>* __restrict__ J,
-                                                         int nvol, int colour) {
-    const int v = batch_volume(J, nvol, blockIdx.x);
-    const BatchJob<double>& B = J[v];
-    if (B.ctl->stop) return;
-    const int lb = blockIdx.x - B.block0;
-    if (lb >= B.nblocks) return;
-    int bx, kb;
-    if (!map_block(B.M, lb, B.nblocks, bx, kb)) return;
-    if (colour == 0)
-        sor_block<double, RESID, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[0], nullptr,
-                                           B.black, B.opnd[0], B.part_r, B.defect_ob[0]);
-    else
-        sor_block<double, RESID, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, nullptr,
-                                           B.red[0], B.opnd[1], B.part_b, B.defect_ob[1]);
-}
-
 // FP64 defect of the corrected state u + d of every running volume (the
 // correction update fused into the defect, as in the single-volume form):
 // red stores u_r + d_r and reads black neighbours u_b + d_b; black stores
@@ -170,17 +149,30 @@ static __global__ void k_inner_ctl_batch(const BatchJob<double>* __restrict__ J6
     if (!outer->stop) outer->z = outer->z + 1;
 }
 
-// d = 0 (all three correction arrays, ghosts included) for running volumes.
+// The first decision of the FP32-refined form from the set-up's fused
+// defect (R already in the control block): stop once R <= tol.
+static __global__ void k_decide_batch(const BatchJob<double>* __restrict__ J64, int nvol) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvol) return;
+    SorCtl* ctl = J64[v].ctl;
+    if (!(ctl->R > ctl->tol)) {
+        ctl->stop = 1;
+        ctl->zfinal = 0;
+    }
+}
+
+// d = 0 (ghosts included; the second red buffer is written before it is
+// read) for running volumes from their second round on: the first round
+// starts from the set-up's d0 = start - u^n.
 static __global__ __launch_bounds__(kBlock) void k_zero_batch(const BatchJob<double>* __restrict__ J64,
                                                        const BatchJob<float>* __restrict__ J32,
                                                        int nvol) {
     const int v = batch_cvolume(J32, nvol, blockIdx.x);
-    if (J64[v].ctl->stop) return;
+    if (J64[v].ctl->stop || J64[v].ctl->z == 0) return;
     const BatchJob<float>& B = J32[v];
     const long stride = (long)B.cnblocks * kBlock;
     for (long q = (long)(blockIdx.x - B.cblock0) * kBlock + threadIdx.x; q < B.np; q += stride) {
         B.red[0][q] = 0.0f;
-        B.red[1][q] = 0.0f;
         B.black[q] = 0.0f;
     }
 }

@@ -95,7 +95,10 @@ public:
     // --- time step pieces -------------------------------------------------
     // Coefficients from u (and g if with_g) into the colour layout, u packed
     // into the colour arrays, and (if write_prev) prev = u.
-    int setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl_target = nullptr);
+    // fuse_defect (FP32-refined form): the set-up kernel also forms the first
+    // FP64 defect (b - A u^n) and the start's offset as the initial correction.
+    int setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl_target = nullptr,
+              bool fuse_defect = false);
     // Red-black SOR from the packed state to the stopping test.
     int solve(const StepParams& P, SolveStats* st);
     // Colour layout -> u (interior), with the Mmask pass fused if mask.
@@ -212,6 +215,8 @@ public:
     double* part_ = nullptr;     // partial sums (two halves)
     long npart_ = 0;
     double* dscal_ = nullptr;    // device scalar scratch
+    double* red2_ = nullptr;     // second-level partial sums (4096)
+    bool defect_fused_ = false;  // the last set-up formed the first defect
     double* hscal_ = nullptr;    // pinned host scalar
     SorCtl* ctl_ = nullptr;      // device control block
     SorCtl* hctl_ = nullptr;     // pinned host mirror

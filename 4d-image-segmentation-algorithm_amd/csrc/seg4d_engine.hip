@@ -99,7 +99,7 @@ int Engine::init(int I, int J, int K, int L, bool seg, bool mcf) {
     if (np < 4096) np = 4096;
     npart_ = np;
     if (alloc(&part_, 2 * np)) return -1;
-    if (alloc(&dscal_, 8)) return -1;
+    if (alloc(&dscal_, 8) || alloc(&red2_, 4096)) return -1;
     if (check(hipMalloc((void**)&ctl_, sizeof(SorCtl)), "hipMalloc ctl")) return -1;
     if (check(hipHostMalloc((void**)&hctl_, sizeof(SorCtl), hipHostMallocDefault),
               "hipHostMalloc"))
@@ -127,6 +127,8 @@ void Engine::release() {
     if (ub_) hipFree(ub_);
     if (part_) hipFree(part_);
     if (dscal_) hipFree(dscal_);
+    if (red2_) hipFree(red2_);
+    red2_ = nullptr;
     if (ctl_) hipFree(ctl_);
     if (hctl_) hipHostFree(hctl_);
     if (hscal_) hipHostFree(hscal_);
@@ -237,7 +239,8 @@ int Engine::download_coef(int f, double* host) {
 
 Geo Engine::geometry() const { return make_geo(I_, J_, K_, L_, H_); }
 
-int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl_target) {
+int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl_target,
+                  bool fuse_defect) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     const Scale S = make_scale(P.h);
     const bool p2 = is_pow2(P.h);
@@ -246,12 +249,18 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
     if (with_g)
         for (int f = 0; f < 8; f++) g.p[f] = full_[A_G0 + f];
     if (precision_ == 1 && ensure_mixed()) return -1;
+    const bool mixed = fuse_defect && precision_ == 1;
     SplitBufs sb;
     for (int c = 0; c < 2; c++) {
         sb.opnd[c] = op_[c];
         sb.ured[c] = ur_[c];
+        sb.opnd32[c] = mixed ? op32_[c] : nullptr;
     }
     sb.ublack = ub_;
+    sb.dred = mixed ? dr_[0] : nullptr;
+    sb.dblack = mixed ? db_ : nullptr;
+    const double csor = P.tau / (P.h * P.h);
+    double* part2 = part_ + npart_;
     double* prevf = write_prev ? full_[A_PREV] : nullptr;
     if (!write_prev) {
         prevf = scratch(A_TMP2);
@@ -314,7 +323,13 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
         (void)hipEventCreate(&e1);
         (void)hipEventRecord(e0, st_);
     }
-    if (with_g) {
+    if (mixed && with_g) {
+        if (p2) k_setup_step<true, true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm, csor, part2);
+        else k_setup_step<true, false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm, csor, part2);
+    } else if (mixed) {
+        if (p2) k_setup_step<false, true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm, csor, part2);
+        else k_setup_step<false, false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm, csor, part2);
+    } else if (with_g) {
         if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm);
         else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm);
     } else {
@@ -346,10 +361,15 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
     const long np = (long)PM.gx * PM.gk * PM.gl;
     const int chunk = 4096;
     const int nb2 = (int)cdiv(np, chunk);
-    if (nb2 > npart_) return check(hipErrorInvalidValue, "setup partials");
-    k_sum_chunks<<<nb2, 1024, 0, st_>>>(part_, np, chunk, part_ + npart_);
-    k_init_ctl<<<1, 1024, 0, st_>>>(part_ + npart_, nb2, ctl_target ? ctl_target : ctl_, P.gsTol,
-                                    P.rel2);
+    if (nb2 > 2048) return check(hipErrorInvalidValue, "setup partials");
+    SorCtl* tctl = ctl_target ? ctl_target : ctl_;
+    k_sum_chunks<<<nb2, 1024, 0, st_>>>(part_, np, chunk, red2_);
+    k_init_ctl<<<1, 1024, 0, st_>>>(red2_, nb2, tctl, P.gsTol, P.rel2);
+    if (mixed) {  // R of the fused first defect, same two-level order
+        k_sum_chunks<<<nb2, 1024, 0, st_>>>(part2, np, chunk, red2_ + 2048);
+        k_set_r0<<<1, 1024, 0, st_>>>(red2_ + 2048, nb2, tctl);
+    }
+    defect_fused_ = mixed;
     return launch_check("k_init_ctl");
 }
 
@@ -714,11 +734,27 @@ int Engine::solve_mixed(const StepParams& P, SolveStats* out) {
     J.kind0 = 3;
     long sweeps = 0;
     int rounds = 0;
-    if (defect64(P, true)) return -1;
+    // the first defect: formed by the set-up kernel (u = u^n, d = start - u^n
+    // already in place), or a defect pass of the start (with the FP32
+    // coefficient copies) when the set-up did not form it
+    const bool fused = defect_fused_;
+    defect_fused_ = false;
+    if (fused) {
+        if (check(hipMemcpyAsync(hctl_, ctl_, sizeof(SorCtl), hipMemcpyDeviceToHost, st_),
+                  "ctl copy") ||
+            check(hipStreamSynchronize(st_), "ctl sync"))
+            return -1;
+    } else if (defect64(P, true)) {
+        return -1;
+    }
     while (hctl_->R > hctl_->tol && rounds < max_rounds_) {
-        // d = 0 (ghosts stay 0: homogeneous boundary of the correction)
-        for (float* d : {dr_[0], dr_[1], db_})
-            if (check(hipMemsetAsync(d, 0, (size_t)np_ * sizeof(float), st_), "memset d")) return -1;
+        // d = 0 (ghosts stay 0: homogeneous boundary of the correction) unless
+        // the first round starts from the set-up's d0; the second red buffer
+        // needs no clearing (its interior is written before it is read)
+        if (rounds > 0 || !fused)
+            for (float* d : {dr_[0], db_})
+                if (check(hipMemsetAsync(d, 0, (size_t)np_ * sizeof(float), st_), "memset d"))
+                    return -1;
         k_inner_ctl<<<1, 64, 0, st_>>>(ctl_, ctl2_, inner_frac_, inner_floor2_);
         if (launch_check("k_inner_ctl")) return -1;
         int z = 0;
@@ -918,7 +954,7 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
         hipEventCreate(&e1);
         hipEventRecord(e0, st_);
     }
-    if (setup(P, true, true)) return -1;
+    if (setup(P, true, true, nullptr, precision_ == 1)) return -1;
     if (solve(P, st)) return -1;
     const bool has_balls = balls && balls->total > 0;
     if (has_balls) {

@@ -231,6 +231,11 @@ struct SplitBufs {
     double* opnd[2];     // [colour] operand blocks (coefficients + prev)
     double* ured[2];     // red ping-pong buffers
     double* ublack;
+    // FP32-refined form with the first defect fused (MIXED): FP32 operand
+    // blocks (coefficients + defect) and the initial corrections per colour
+    float* opnd32[2];
+    float* dred;
+    float* dblack;
 };
 
 // Block -> (run bx, plane k, frame l) map of full-layout kernels whose
@@ -264,16 +269,22 @@ __device__ __forceinline__ bool map_plane_block(const PlaneMap& M, int b, int& b
 // 4 = from the cubic one 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) (prev3
 // holds u^(n-3)).  Non-null history arrays shift by one level: prev3
 // receives u^(n-2), prev2 u^(n-1), prev_full u^n.
-template <bool WITH_G, bool POW2>
+// MIXED: the set-up of the FP32-refined form, fused with its first FP64
+// defect.  The colour arrays receive u^n and the FP32 correction arrays the
+// start's offset d0 = start - u^n (the correction solve starts from it); the
+// FP32 operand blocks receive the coefficients and -r for r = b - A u^n,
+// formed as the defect pass forms it (b = prev = u^n, the 8 face neighbours
+// from the stencil), and partials2 the block sums of r^2 h^4.  c = tau / h^2.
+template <bool WITH_G, bool POW2, bool MIXED = false>
 static __global__ __launch_bounds__(kBlock) void k_setup_step(
     Geo G, PlaneMap PM, Scale S, double eps2, double h4, const double* __restrict__ u,
     CPtr8 g, SplitBufs sb, const double* hist1, const double* hist2, const double* hist3,
     double* prev_out, const double* __restrict__ start_full, double* __restrict__ partials,
-    int warm) {
+    int warm, double cc = 0.0, double* __restrict__ partials2 = nullptr) {
     int bx, k, l;
     if (!map_plane_block(PM, blockIdx.x, bx, k, l)) return;
     const int t = bx * kBlock + threadIdx.x;
-    double acc = 0.0;
+    double acc = 0.0, acc2 = 0.0;
     if (t < G.I * G.J) {
         const int j = t / G.I, i = t - j * G.I;
         const int p = ((l * G.K + k) * G.J + j) * G.I + i;
@@ -292,7 +303,16 @@ static __global__ __launch_bounds__(kBlock) void k_setup_step(
         if (warm == 3 && in) start = (3.0 * uc - 3.0 * pf) + p2;
         if (warm == 4 && in) start = ((4.0 * uc - 6.0 * pf) + 4.0 * p2) - p3;
         prev_out[p] = uc;
-        if (c == 0) {
+        if constexpr (MIXED) {
+            // (at ghosts start = u^n, so d0 = 0 there)
+            if (c == 0) {
+                sb.ured[0][pp] = uc;
+                sb.dred[pp] = (float)(start - uc);
+            } else {
+                sb.ublack[pp] = uc;
+                sb.dblack[pp] = (float)(start - uc);
+            }
+        } else if (c == 0) {
             sb.ured[0][pp] = start;
             // the second red buffer's interior is written by the first red
             // pass before anything reads it: only its ghosts are set here
@@ -319,18 +339,47 @@ static __global__ __launch_bounds__(kBlock) void k_setup_step(
             const double avg =
                 (((((((N[0] + N[1]) + N[2]) + N[3]) + N[4]) + N[5]) + N[6]) + N[7]) / 8.0;
             const double vc = sqrt(avg * avg + eps2);
+            double cf[8];
 #pragma unroll
             for (int f = 0; f < 8; f++) {
-                double cf;
-                if constexpr (WITH_G) cf = vc * gl[f] * (1.0 / N[f]);
-                else cf = vc * (1.0 / N[f]);
-                ob[f * 64] = cf;
+                if constexpr (WITH_G) cf[f] = vc * gl[f] * (1.0 / N[f]);
+                else cf[f] = vc * (1.0 / N[f]);
+                ob[f * 64] = cf[f];
             }
             acc = uc * uc * h4;
+            if constexpr (MIXED) {
+                float* ob32 = (c ? sb.opnd32[1] : sb.opnd32[0]) + oi;
+#pragma unroll
+                for (int f = 0; f < 8; f++) ob32[f * 64] = (float)cf[f];
+                // numerator order e, w, s, n, b, t, bw, fw; diagonal e, w, n, s, t, b, fw, bw
+                double num = cf[0] * T.s[1][1][1][2];
+                num = num + cf[1] * T.s[1][1][1][0];
+                num = num + cf[3] * T.s[1][1][0][1];
+                num = num + cf[2] * T.s[1][1][2][1];
+                num = num + cf[5] * T.s[1][0][1][1];
+                num = num + cf[4] * T.s[1][2][1][1];
+                num = num + cf[7] * T.s[0][1][1][1];
+                num = num + cf[6] * T.s[2][1][1][1];
+                double dg = cf[0] + cf[1];
+                dg = dg + cf[2];
+                dg = dg + cf[3];
+                dg = dg + cf[4];
+                dg = dg + cf[5];
+                dg = dg + cf[6];
+                dg = dg + cf[7];
+                const double D = 1 + cc * dg;
+                const double r = uc * D - cc * num - uc;
+                ob32[8 * 64] = (float)(-r);
+                acc2 = r * r * h4;
+            }
         }
     }
     const double bs = block_sum256(acc);
     if (threadIdx.x == 0) partials[(l * PM.gk + k) * PM.gx + bx] = bs;
+    if constexpr (MIXED) {
+        const double bs2 = block_sum256(acc2);
+        if (threadIdx.x == 0) partials2[(l * PM.gk + k) * PM.gx + bx] = bs2;
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -690,6 +739,23 @@ static __global__ __launch_bounds__(1024) void k_init_ctl(const double* __restri
         ctl->z = 0;
         ctl->stop = 0;
         ctl->zfinal = 0;
+    }
+}
+
+// The FP64 residual of the set-up's fused defect: ctl->R = sum of np
+// partials (after k_init_ctl).
+static __global__ __launch_bounds__(1024) void k_set_r0(const double* __restrict__ partials,
+                                                       int np, SorCtl* ctl) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < np; q += 1024) s += partials[q];
+    s = wave_sum(s);
+    __shared__ double sm[16];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double R = sm[0];
+        for (int q = 1; q < 16; q++) R += sm[q];
+        ctl->R = R;
     }
 }
 
