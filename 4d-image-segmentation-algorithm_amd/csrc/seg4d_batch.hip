@@ -289,7 +289,11 @@ int Batch::step(const StepParams& P) {
     if (build_jobs(P)) return -1;
     const int cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
     hipEvent_t e0, e1;
-    if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return -1;
+    if (check(hipEventCreate(&e0), "event")) return -1;
+    if (check(hipEventCreate(&e1), "event")) {
+        (void)hipEventDestroy(e0);
+        return -1;
+    }
     int rc = check(hipEventRecord(e0, st_), "event");
     for (int v = 0; v < n && !rc; v++)
         if (E_[v]->setup(P, true, true, ctl_ + v, precision == 1)) {

@@ -384,7 +384,11 @@ int Engine::autotune(int reps, float* best_ms) {
     const int keep_v = sweep_variant_, keep_tb = sweep_tb_, keep_tk = sweep_tk_;
     const int keep_lc = sweep_lc_;
     hipEvent_t e0, e1;
-    if (check(hipEventCreate(&e0), "event") || check(hipEventCreate(&e1), "event")) return -1;
+    if (check(hipEventCreate(&e0), "event")) return -1;
+    if (check(hipEventCreate(&e1), "event")) {
+        (void)hipEventDestroy(e0);
+        return -1;
+    }
     float best = 1e30f;
     int bq = -1;
     int rc = 0;
