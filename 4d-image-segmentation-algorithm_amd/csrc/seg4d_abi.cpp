@@ -705,6 +705,13 @@ API int seg4dSetSweepChunk(int lsteps) {
     return old;
 }
 
+API int seg4dSetSweepBands(int nband) {
+    if (!g_eng) return -1;
+    const int old = g_eng->sweep_nband_;
+    g_eng->sweep_nband_ = (nband > 0 && nband <= 64) ? nband : 0;
+    return old;
+}
+
 API int seg4dSetPipelineDepth(int depth) {
     if (!g_eng) return -1;
     const int old = g_eng->pipe_depth_;

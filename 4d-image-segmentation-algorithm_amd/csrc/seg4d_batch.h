@@ -80,6 +80,7 @@ private:
     std::vector<BatchJob<double>> j64_;
     std::vector<BatchJob<float>> j32_;
     int grid_ = 0, grid32_ = 0, cgrid_ = 0;  // FP64 / FP32 SOR launches, element-wise
+    int rgrid_ = 0, rgrid32_ = 0;            // chunk-sum launches of the residual reduction
     bool jobs_ok_ = false;
     double jobs_c_ = 0.0, jobs_om_ = 0.0, jobs_om32_ = 0.0, jobs_h4_ = 0.0;
     int jobs_prec_ = -1;

@@ -112,15 +112,14 @@ int Batch::build_jobs(const StepParams& P) {
         return 0;
     j64_.assign(n, BatchJob<double>());
     j32_.assign(n, BatchJob<float>());
-    long b0 = 0, b032 = 0, cb0 = 0;
+    long b0 = 0, b032 = 0, cb0 = 0, r0 = 0, r032 = 0;
     for (int v = 0; v < n; v++) {
         Engine& E = *E_[v];
         if (precision == 1 && E.ensure_mixed()) return -1;
         // the launch maps of the volume's own FP64 and FP32 passes
         const SweepMap M = E.sweep_map(8), M32 = E.sweep_map(4);
-        const int nbk = M.tiled ? ((M.ntiles + 7) / 8) * 8 * M.tb * M.tk : M.gx * M.gk;
-        const int nbk32 =
-            M32.tiled ? ((M32.ntiles + 7) / 8) * 8 * M32.tb * M32.tk : M32.gx * M32.gk;
+        const int nbk = sweep_grid(M);
+        const int nbk32 = sweep_grid(M32);
         const int nbp = M.gx * M.gk, nbp32 = M32.gx * M32.gk;
         if ((long)nbp > E.npart_ || (long)nbp32 > E.npart_) {
             err = "batch: SOR grid exceeds a volume's partial buffer";
@@ -156,6 +155,13 @@ int Batch::build_jobs(const StepParams& P) {
         A.part_b = B.part_b = E.part_ + E.npart_;
         A.nb = nbp;
         B.nb = nbp32;
+        A.red2 = B.red2 = E.red2_;
+        resid_chunking(nbp, &A.nc, &A.ch);
+        resid_chunking(nbp32, &B.nc, &B.ch);
+        A.r1block0 = (int)r0;
+        B.r1block0 = (int)r032;
+        r0 += 2 * A.nc;
+        r032 += 2 * B.nc;
         A.np = B.np = E.np_;
         A.ctl = ctl_ + v;
         B.ctl = ctl2_ + v;
@@ -175,6 +181,8 @@ int Batch::build_jobs(const StepParams& P) {
     }
     grid_ = (int)b0;
     grid32_ = (int)b032;
+    rgrid_ = (int)r0;
+    rgrid32_ = (int)r032;
     cgrid_ = (int)cb0;
     if (check(hipMemcpyAsync(dj64_, j64_.data(), (size_t)n * sizeof(BatchJob<double>),
                              hipMemcpyHostToDevice, st_), "jobs") ||
@@ -213,8 +221,10 @@ int Batch::sor_run(std::vector<BatchJob<T>>& hj, BatchJob<T>* dj, SorCtl* dctl, 
     };
     auto black = [&](int zr) { k_sor_batch<T, 2, 2><<<grid, kBlock, 0, st_>>>(dj, n, 1, zr, 0); };
     // iteration z + 1 and the stopping test of iteration z
+    const dim3 rgrid((unsigned)(sizeof(T) == 4 ? rgrid32_ : rgrid_));
     auto iter = [&](int z) {
         red(z & 1, (z + 1) & 1, true);
+        k_resid_chunks_batch<T><<<rgrid, 1024, 0, st_>>>(dj, n);
         k_reduce_batch<T><<<n, 1024, 0, st_>>>(dj, cap);
         rc |= check(hipMemcpyAsync(hctl, dctl, (size_t)n * sizeof(SorCtl), hipMemcpyDeviceToHost,
                                    st_), "ctl copy");

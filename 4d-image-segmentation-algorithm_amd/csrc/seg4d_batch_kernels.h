@@ -32,6 +32,9 @@ struct BatchJob {
     long np;                // colour-array elements
     SorCtl* ctl;            // control block of this job's solve
     float* defect_ob[2];    // FP64 jobs: FP32 operand blocks (defect, copies)
+    double* red2;           // chunk sums of the two-level residual reduction
+    int nc, ch;             // nc chunks of ch partials per colour
+    int r1block0;           // this volume's first block of the chunk-sum launch
 };
 
 template <typename T>
@@ -94,14 +97,29 @@ static __global__ __launch_bounds__(kBlock) void k_defect_corr_batch(
                                        nullptr);
 }
 
+// First level of the residual reduction of every running volume: the chunk
+// sums of k_resid_chunks (same chunks, same order).
+template <typename T>
+static __global__ __launch_bounds__(1024) void k_resid_chunks_batch(const BatchJob<T>* __restrict__ J,
+                                                                   int nvol) {
+    int v = 0;
+    while (v + 1 < nvol && (int)blockIdx.x >= J[v + 1].r1block0) v++;
+    const BatchJob<T>& B = J[v];
+    if (B.ctl->stop) return;
+    const int q = blockIdx.x - B.r1block0;
+    const double S = chunk_sum(B.part_r, B.part_b, B.nb, B.nc, B.ch, q);
+    if (threadIdx.x == 0) B.red2[q] = S;
+}
+
 // Residual reduction and stopping test of every running volume (one block
-// per volume); the iteration index is the control block's own counter.
+// per volume, over its chunk sums); the iteration index is the control
+// block's own counter.
 template <typename T>
 static __global__ __launch_bounds__(1024) void k_reduce_batch(const BatchJob<T>* __restrict__ J, int cap) {
     const BatchJob<T>& B = J[blockIdx.x];
     SorCtl* ctl = B.ctl;
     if (ctl->stop) return;
-    const double R = resid_sum(B.part_r, B.nb, B.part_b, B.nb);
+    const double R = resid_sum(B.red2, B.nc, B.red2 + B.nc, B.nc);
     if (threadIdx.x == 0) {
         const int z = ctl->z + 1;
         ctl->R = R;

@@ -162,6 +162,7 @@ public:
     // 1.347 ms per red pass)
     int sweep_tb_ = 16, sweep_tk_ = 8;
     int sweep_lc_ = 0;     // l steps per block of a colour pass; 0 = automatic
+    int sweep_nband_ = 0;  // bands of the automatic linear order; 0 = as the L2 window needs
     int pipe_depth_ = 0;   // iterations enqueued ahead of the stopping test; 0 = automatic
     std::vector<hipEvent_t> evq_;  // one event per queued iteration
     // 0 = FP64 (the reference's arithmetic); 1 = FP32-refined: FP32 SOR on
@@ -236,6 +237,9 @@ public:
     Balls balls_sent_;
     bool balls_valid_ = false;
 };
+
+// Chunking of the two-level residual reduction over nb partials per colour.
+void resid_chunking(int nb, int* nc, int* ch);
 
 // Last error of the calling library (set by the ABI layers).
 void set_error(const std::string& msg);

@@ -31,6 +31,16 @@ Scale make_scale(double h) {
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
 }  // namespace
 
+// Chunking of the two-level residual reduction over nb partials per colour:
+// nc chunks of ch partials (2 nc <= 4096 chunk sums).
+void resid_chunking(int nb, int* nc, int* ch) {
+    int n = (int)cdiv(nb, 4096);
+    if (n < 1) n = 1;
+    if (n > 2048) n = 2048;
+    *nc = n;
+    *ch = (int)cdiv(nb, n);
+}
+
 void set_error(const std::string& msg) { g_last_error = msg; }
 const char* last_error() { return g_last_error.c_str(); }
 
@@ -399,8 +409,7 @@ int Engine::autotune(int reps, float* best_ms) {
         sweep_lc_ = cand[q][3];
         const SweepMap M = sweep_map();
         if ((long)M.gx * M.gk > npart_) continue;
-        const dim3 grid(M.tiled ? (unsigned)(((M.ntiles + 7) / 8) * 8 * M.tb * M.tk)
-                                : (unsigned)(M.gx * M.gk));
+        const dim3 grid((unsigned)sweep_grid(M));
         float cand_ms = 1e30f;
         for (int r = 0; r <= reps && rc == 0; r++) {
             rc |= check(hipEventRecord(e0, st_), "event");
@@ -438,6 +447,8 @@ bool Engine::tiled_order() const { return sweep_map().tiled != 0; }
 
 SweepMap Engine::sweep_map(int elem_bytes) const {
     SweepMap M;
+    M.nband = 1;
+    M.bpb = 0;
     M.kp = (sweep_variant_ & 16) ? 4 : (sweep_variant_ & 8) ? 2 : 1;
     M.gx = (int)cdiv((long)H_ * (J_ - 2), kBlock / M.kp);
     M.gk0 = (int)cdiv((long)(K_ - 2), M.kp);
@@ -448,13 +459,26 @@ SweepMap Engine::sweep_map(int elem_bytes) const {
         // automatic order: in the linear order every XCD works through a
         // contiguous range of (chunk, k) and re-reads the other colour's
         // k-1, k, k+1 planes of the chunk's frames and its two neighbours
-        // from L2; chunks of 2 frames keep that window within an XCD's 4 MB
-        // L2 wherever it fits (measured at 256x256x128x16: red pass 1.259 ms
-        // against 1.352 ms for whole marches in the tiled order)
+        // from L2 (and the Infinity Cache behind it); chunks of 2 frames keep
+        // that window small (measured at 256x256x128x16, 3.2 MB: red pass
+        // 1.259 ms against 1.352 ms for whole marches in the tiled order).
+        // Planes whose window exceeds 7 MB are split into bands of runs, each
+        // band walked through the k planes in turn (at 512x512x256x8, 12.7 MB:
+        // 2 bands 10.01 ms per red + black pass pair, 4 bands 10.11, 8 bands
+        // 10.14, whole marches in the tiled order 10.38).
         const long window = 3L * 4 * (long)H_ * J_ * elem_bytes;
-        if ((sweep_variant_ & 32) && window <= (7L << 19)) {
+        int nband = 1;
+        while (nband < 16 && window / nband > (7L << 20)) nband *= 2;
+        bool fits = window / nband <= (7L << 20) && M.gx >= 8 * nband;
+        if (sweep_nband_ > 0) {  // forced band count
+            nband = sweep_nband_ < M.gx ? sweep_nband_ : M.gx;
+            fits = true;
+        }
+        if ((sweep_variant_ & 32) && fits) {
             lc = 2;
             M.tiled = 0;
+            M.nband = nband;
+            M.bpb = (M.gx + nband - 1) / nband;
         } else {
             // a volume with fewer than ~4096 blocks of whole l-marches splits
             // its frames so that the launch fills the GPU (even chunk lengths
@@ -501,7 +525,9 @@ int sor_run(Engine& E, const SorJob<T>& J, int* zout) {
         std::fprintf(stderr, "seg4d: SOR grid of %d blocks exceeds the partial buffer\n", nb);
         return -1;
     }
-    const dim3 grid(M.tiled ? (unsigned)(((M.ntiles + 7) / 8) * 8 * M.tb * M.tk) : (unsigned)nb);
+    int rnc = 0, rch = 0;
+    resid_chunking(nb, &rnc, &rch);
+    const dim3 grid((unsigned)sweep_grid(M));
     double* part_r = E.part_;
     double* part_b = E.part_ + E.npart_;
     hipStream_t st = E.st_;
@@ -571,8 +597,10 @@ int sor_run(Engine& E, const SorJob<T>& J, int* zout) {
     for (;;) {
         while (queued < depth) {
             if (red(z + 1, true)) return -1;
-            k_reduce_resid<<<1, 1024, 0, st>>>(part_r, nb, part_b, nb, J.ctl, nullptr, z, J.cap,
-                                               1);
+            // two fixed-order levels: chunk sums (many blocks), then the test
+            k_resid_chunks<<<2 * rnc, 1024, 0, st>>>(part_r, part_b, nb, rnc, rch, E.red2_, J.ctl);
+            k_reduce_resid<<<1, 1024, 0, st>>>(E.red2_, rnc, E.red2_ + rnc, rnc, J.ctl, nullptr, z,
+                                               J.cap, 1);
             if (E.launch_check("k_reduce_resid")) return -1;
             if (E.check(hipMemcpyAsync(J.hctl, J.ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, st),
                         "ctl copy"))
@@ -673,7 +701,7 @@ int Engine::defect64(const StepParams& P, bool copy32, const float* dred) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     const SweepMap M = sweep_map();
     const int nb = M.gx * M.gk;
-    const dim3 grid(M.tiled ? (unsigned)(((M.ntiles + 7) / 8) * 8 * M.tb * M.tk) : (unsigned)nb);
+    const dim3 grid((unsigned)sweep_grid(M));
     const double c = P.tau / (P.h * P.h);
     const double h4 = P.h * P.h * P.h * P.h;
     double* part_r = part_;

@@ -409,14 +409,35 @@ struct SweepMap {
     int tiled;
     int tb, tk;      // tile shape; tb * tk blocks per tile
     int ntb, ntiles;
+    int nband, bpb;  // linear order: the runs of a plane in nband bands of bpb runs;
+                     // consecutive blocks walk one band through the k planes
 };
+
+// Blocks of a SOR launch with map M (some return at once when the shape
+// does not divide evenly).
+__host__ __device__ inline int sweep_grid(const SweepMap& M) {
+    if (M.tiled) return ((M.ntiles + 7) / 8) * 8 * M.tb * M.tk;
+    if (M.nband > 1) return M.nband * M.bpb * M.gk;
+    return M.gx * M.gk;
+}
 
 __device__ __forceinline__ bool map_block(const SweepMap& M, int b, int nb, int& bx, int& k) {
     if (!M.tiled) {
         const int lb = xcd_remap(b, nb);
-        bx = lb % M.gx;
-        k = lb / M.gx + 1;
-        return true;
+        if (M.nband <= 1) {
+            bx = lb % M.gx;
+            k = lb / M.gx + 1;
+            return true;
+        }
+        // lb = ((chunk * nband + band) * gk0 + k group) * bpb + run in band
+        const int r = lb % M.bpb;
+        int q = lb / M.bpb;
+        const int kg = q % M.gk0;
+        q /= M.gk0;
+        const int band = q % M.nband, ch = q / M.nband;
+        bx = band * M.bpb + r;
+        k = ch * M.gk0 + kg + 1;
+        return bx < M.gx && k <= M.gk;
     }
     const int x = b % 8, s = b / 8;
     const int per = M.tb * M.tk;
@@ -678,6 +699,37 @@ __device__ __forceinline__ double resid_sum(const double* __restrict__ pa, int n
     double R = sm[0];
     for (int q = 1; q < 16; q++) R += sm[q];
     return R;
+}
+
+// First level of the residual reduction of one SOR iteration: block q sums
+// chunk (q mod nc) of the red (q < nc) or black partials, ch partials each,
+// into out[q]; the stopping test then reduces the 2 nc chunk sums.  A queued
+// reduction of a stopped solve (ctl->stop) returns at once.
+__device__ __forceinline__ double chunk_sum(const double* __restrict__ pa,
+                                           const double* __restrict__ pb, int nb, int nc,
+                                           int ch, int q) {
+    const double* src = q < nc ? pa : pb;
+    const int c = q < nc ? q : q - nc;
+    const int lo = c * ch, hi = lo + ch < nb ? lo + ch : nb;
+    double s = 0.0;
+    for (int i = lo + (int)threadIdx.x; i < hi; i += 1024) s += src[i];
+    s = wave_sum(s);
+    __shared__ double sm[16];
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    double S = sm[0];
+    for (int w = 1; w < 16; w++) S += sm[w];
+    return S;
+}
+
+static __global__ __launch_bounds__(1024) void k_resid_chunks(const double* __restrict__ pa,
+                                                             const double* __restrict__ pb,
+                                                             int nb, int nc, int ch,
+                                                             double* __restrict__ out,
+                                                             const SorCtl* __restrict__ ctl) {
+    if (ctl->stop) return;
+    const double S = chunk_sum(pa, pb, nb, nc, ch, blockIdx.x);
+    if (threadIdx.x == 0) out[blockIdx.x] = S;
 }
 
 // Final residual reduction of one SOR iteration and the reference's stopping

@@ -155,7 +155,7 @@ class SegLib(_Lib):
         L.seg4dSetSweepTile.argtypes = [c_int, c_int]
         L.seg4dSetInnerOmega.restype = c_double
         L.seg4dSetInnerOmega.argtypes = [c_double]
-        for fn in ("seg4dSetSweepChunk", "seg4dSetPipelineDepth"):
+        for fn in ("seg4dSetSweepChunk", "seg4dSetPipelineDepth", "seg4dSetSweepBands"):
             getattr(L, fn).restype = c_int
             getattr(L, fn).argtypes = [c_int]
         L.seg4dSetPrecision.restype = c_int

@@ -189,9 +189,9 @@ void seg4dProfileReset(void);
  * are co-resident on one XCD); 4 = nontemporal loads of the coefficient
  * stream; 8 = two, 16 = four consecutive k planes per block (8 and 16
  * exclusive); 32 = automatic order: with automatic frame chunks
- * (seg4dSetSweepChunk(0)), chunks of 2 frames in the linear order whenever
- * the other colour's window of 3 k planes x 4 frames fits in 3.5 MB (one
- * XCD's L2), replacing bit 2.  Default 35.  u and the sweep counts are those
+ * (seg4dSetSweepChunk(0)), chunks of 2 frames in the linear order, the runs
+ * of each plane split into bands when the other colour's window of 3 k planes
+ * x 4 frames exceeds 7 MB (seg4dSetSweepBands), replacing bit 2.  Default 35.  u and the sweep counts are those
  * of any other variant; bits 8/16 and the frame chunk regroup the residual
  * sums (last-bit differences in R).  Returns the previous value, -1 before
  * _allocateMem_. */
@@ -204,6 +204,12 @@ int seg4dSetSweepTile(int runs, int planes);
  * then chunks of an even length; n > 0 = chunks of n frames.  Returns the
  * previous value. */
 int seg4dSetSweepChunk(int lsteps);
+/* Bands of the automatic order (variant bit 32, automatic chunks): 0
+ * (default) = as many as keep the other colour's window within 7 MB (planes
+ * larger than that are split into bands of runs, each band walked through
+ * the k planes in turn); n > 0 forces n bands.  Tuning knob: the
+ * same u and sweep counts.  Returns the previous value. */
+int seg4dSetSweepBands(int nband);
 /* SOR iterations enqueued ahead of the stopping test the host has seen
  * (1..16; 0 = automatic: 4 for volumes under 4M points per colour, else 1).
  * Passes enqueued after the stop are no-ops; results are identical.

@@ -201,15 +201,16 @@ def test_fixed_sweep_count(seglib, oracle, sweeps, variant, chunk, depth):
         lib.set_sweep_variant(35)
         lib.lib.seg4dSetSweepChunk(0)
         lib.lib.seg4dSetPipelineDepth(0)
+        lib.lib.seg4dSetSweepBands(0)
         lib.deallocate()
 
 
-@pytest.mark.parametrize("variant,dims,chunk,depth",
-                         [(11, (17, 11, 7, 3), 0, 0), (19, (17, 11, 7, 3), 0, 0),
-                          (10, (8, 6, 5, 4), 0, 0), (15, (30, 9, 6, 5), 0, 0),
-                          (3, (31, 10, 7, 6), 100, 1), (3, (31, 10, 7, 6), 2, 5),
-                          (3, (31, 10, 7, 6), 4, 2)])
-def test_tolerance_driven_variants(seglib, oracle, variant, dims, chunk, depth):
+@pytest.mark.parametrize("variant,dims,chunk,depth,bands",
+                         [(11, (17, 11, 7, 3), 0, 0, 0), (19, (17, 11, 7, 3), 0, 0, 0),
+                          (10, (8, 6, 5, 4), 0, 0, 0), (15, (30, 9, 6, 5), 0, 0, 0),
+                          (3, (31, 10, 7, 6), 100, 1, 0), (3, (31, 10, 7, 6), 2, 5, 0),
+                          (3, (31, 10, 7, 6), 4, 2, 0)])
+def test_tolerance_driven_variants(seglib, oracle, variant, dims, chunk, depth, bands):
     """Tolerance-driven steps (the fused residual and its per-block partials)
     with several k planes per block: same sweep count and bits as the oracle."""
     nx, ny, nz, nt = dims
@@ -224,6 +225,7 @@ def test_tolerance_driven_variants(seglib, oracle, variant, dims, chunk, depth):
     lib.set_sweep_variant(variant)
     lib.lib.seg4dSetSweepChunk(chunk)
     lib.lib.seg4dSetPipelineDepth(depth)
+    lib.lib.seg4dSetSweepBands(bands)
     try:
         lib.set(gsTol=1e-9, **kw)
         for f, gg in zip(FACES, gs):
@@ -241,4 +243,60 @@ def test_tolerance_driven_variants(seglib, oracle, variant, dims, chunk, depth):
         lib.set_sweep_variant(35)
         lib.lib.seg4dSetSweepChunk(0)
         lib.lib.seg4dSetPipelineDepth(0)
+        lib.lib.seg4dSetSweepBands(0)
+        lib.deallocate()
+
+
+def _smooth_volume(shape, centre):
+    L, K, J, I = shape
+    l, k, j, i = np.meshgrid(np.arange(L), np.arange(K), np.arange(J), np.arange(I), indexing="ij")
+    a = 1.0 / (np.sqrt((i - centre[0]) ** 2 + (j - centre[1]) ** 2 + (k - centre[2]) ** 2) + 1.0)
+    a[0] = a[-1] = 0
+    a[:, 0] = a[:, -1] = 0
+    a[:, :, 0] = a[:, :, -1] = 0
+    a[..., 0] = a[..., -1] = 0
+    return a
+
+
+@pytest.mark.parametrize("variant,bands,depth", [(35, 0, 0), (35, 2, 0), (35, 3, 1), (43, 4, 2),
+                                                 (51, 2, 0)])
+def test_banded_order_vs_oracle(seglib, oracle, variant, bands, depth):
+    """The automatic order's bands of runs (forced here; automatic only for
+    planes too large for L2), also with two or four k planes per block:
+    tolerance-driven steps with the oracle's sweep counts and bits."""
+    print(f"[gpu] banded order variant {variant} bands {bands} depth {depth}", flush=True)
+    nx, ny, nz, nt = 61, 40, 9, 5
+    shape = (nt + 2, nz + 2, ny + 2, nx + 2)
+    rng = np.random.default_rng(bands + 10 * variant)
+    u0 = _smooth_volume(shape, (nx / 2, ny / 3, nz / 2))
+    gs = []
+    for _ in range(8):
+        g = np.zeros(shape)
+        g[1:-1, 1:-1, 1:-1, 1:-1] = rng.uniform(0.5, 1.0, (nt, nz, ny, nx))
+        gs.append(g)
+    kw = dict(h=1.0, tau=0.5, eps2=1e-6, omega_c=1.8, maxNoGSIteration=5000)
+    p = params(shape, gsTol=1e-9, **kw)
+    lib = seglib
+    lib.allocate(nx, ny, nz, nt)
+    lib.set_sweep_variant(variant)
+    lib.lib.seg4dSetPipelineDepth(depth)
+    lib.lib.seg4dSetSweepBands(bands)
+    try:
+        lib.set(gsTol=1e-9, **kw)
+        for f, gg in zip(FACES, gs):
+            lib.arr("g_" + f)[...] = gg
+        lib.host_wrote("g_e")
+        lib.arr("Mmask")[...] = 1.0
+        lib.arr("u")[...] = u0
+        u, prev = u0.copy(), np.zeros(shape)
+        coef = zeros(shape, 8)
+        for st in range(2):
+            z_g, _ = lib.seg_time_step()
+            z_o, _ = oracle.seg_time_step(p, u, prev, gs, coef, np.ones(shape))
+            assert z_g == z_o < 5000
+            assert np.array_equal(lib.arr("u"), u)
+    finally:
+        lib.set_sweep_variant(35)
+        lib.lib.seg4dSetPipelineDepth(0)
+        lib.lib.seg4dSetSweepBands(0)
         lib.deallocate()
