@@ -341,7 +341,7 @@ def main():
             "volume": [nx, ny, nz, nt],
             "time_steps": args.steps,
             "solver": "colour-layout red-black SOR, operand blocks, residual fused into the passes",
-            "sweep_variant": args.variant if args.variant is not None else 3,
+            "sweep_variant": args.variant if args.variant is not None else 35,
         },
         "roofline": {
             "bound": "hbm",
