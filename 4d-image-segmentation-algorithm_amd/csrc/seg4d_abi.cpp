@@ -582,6 +582,20 @@ API int seg4dNetInit(uint64_t seed) {
     return 0;
 }
 
+API int seg4dNetInitShape(uint64_t seed, int layers, int base) {
+    if (!g_net) g_net = new s4d::Net();
+    if (g_net->init(seed, layers, base)) return net_fail("seg4dNetInitShape");
+    if (g_eng) g_eng->net_ = g_net;
+    return 0;
+}
+
+API int seg4dNetShape(int* layers, int* base) {
+    if (!need_net("seg4dNetShape")) return -1;
+    if (layers) *layers = g_net->layers();
+    if (base) *base = g_net->base();
+    return 0;
+}
+
 API int seg4dNetLoad(const char* path) {
     if (!g_net) g_net = new s4d::Net();
     if (g_net->load(path)) return net_fail("seg4dNetLoad");

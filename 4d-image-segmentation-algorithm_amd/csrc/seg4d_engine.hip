@@ -279,11 +279,17 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
     // the extrapolated and the learned start need prev_full = u^(n-1) from a
     // completed step; without it the solve starts from u^n.  The quadratic
     // and cubic starts need one / two more levels (prev2_, prev3_); with
-    // fewer, the highest order available is used.
+    // fewer, the highest order available is used.  The learned start
+    // corrects the extrapolation of its network's base order (linear or
+    // cubic) and keeps the history that base needs; until that history
+    // exists it starts like mode 4.
+    const int nbase = (warm_ == 2 && net_ && net_->ready()) ? net_->base() : 0;
+    const bool keep2 = (warm_ >= 3 || nbase == 3) && write_prev;  // history 2 / 3 levels deep
+    const bool keep3 = (warm_ == 4 || nbase == 3) && write_prev;
     int warm = 0;
-    if ((warm_ == 1 || warm_ >= 3) && hist_ok_ && write_prev) warm = 1;
-    if (warm_ >= 3 && hist_ok_ && hist2_ok_ && write_prev) warm = 3;
-    if (warm_ == 4 && hist_ok_ && hist2_ok_ && hist3_ok_ && write_prev) warm = 4;
+    if ((warm_ == 1 || keep2) && hist_ok_ && write_prev) warm = 1;
+    if (keep2 && hist_ok_ && hist2_ok_) warm = 3;
+    if (warm_ == 4 && keep3 && hist_ok_ && hist2_ok_ && hist3_ok_) warm = 4;
     // History ring: prev_full (u^(n-1)), prev2_ (u^(n-2)), prev3_ (u^(n-3)).
     // u^n is written over the oldest level the mode keeps, and the pointers
     // rotate after the launch (prev_full = u^n, prev2_ = u^(n-1), ...), so no
@@ -292,14 +298,15 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
     const double* h3 = nullptr;
     double* pout = prevf;
     shifted2_ = shifted3_ = false;
-    if (warm_ >= 3 && write_prev) {
+    const bool h2_valid = hist2_ok_, h3_valid = hist3_ok_;  // before this step shifts them
+    if (keep2) {
         if (!prev2_ && check(hipMalloc((void**)&prev2_, (size_t)n_ * sizeof(double)),
                              "hipMalloc prev2"))
             return -1;
         h2 = prev2_;
         pout = prev2_;
         shifted2_ = hist_ok_;  // prev2 becomes a valid u^(n-1)
-        if (warm_ == 4) {
+        if (keep3) {
             if (!prev3_ && check(hipMalloc((void**)&prev3_, (size_t)n_ * sizeof(double)),
                                  "hipMalloc prev3"))
                 return -1;
@@ -308,12 +315,13 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
             shifted3_ = hist_ok_ && hist2_ok_;  // prev3 becomes a valid u^(n-2)
         }
     }
-    const double* startf = nullptr;
-    if (warm_ == 2 && hist_ok_ && write_prev && net_ && net_->ready()) {
-        double* st = scratch(A_TMP);
-        if (!st) return -1;
-        if (net_->predict(*this, st, true)) return check(hipErrorUnknown, net_->err.c_str());
-        startf = st;
+    LearnedStart ls{nullptr, nullptr, 0.0, 1.f, 1};
+    if (nbase > 0 && hist_ok_ && write_prev && (nbase == 1 || (h2_valid && h3_valid))) {
+        // the ring is read before the set-up kernel rotates it; the kernel
+        // forms base + sigma c / s_t from the network's output
+        if (net_->predict(*this, full_[A_PREV], nbase == 3 ? prev2_ : nullptr,
+                          nbase == 3 ? prev3_ : nullptr, &ls))
+            return check(hipErrorUnknown, net_->err.c_str());
         warm = 2;
     }
     PlaneMap PM;
@@ -334,17 +342,17 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
         (void)hipEventRecord(e0, st_);
     }
     if (mixed && with_g) {
-        if (p2) k_setup_step<true, true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm, csor, part2);
-        else k_setup_step<true, false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm, csor, part2);
+        if (p2) k_setup_step<true, true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm, csor, part2);
+        else k_setup_step<true, false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm, csor, part2);
     } else if (mixed) {
-        if (p2) k_setup_step<false, true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm, csor, part2);
-        else k_setup_step<false, false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm, csor, part2);
+        if (p2) k_setup_step<false, true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm, csor, part2);
+        else k_setup_step<false, false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm, csor, part2);
     } else if (with_g) {
-        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm);
-        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm);
+        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm);
+        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm);
     } else {
-        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm);
-        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, startf, part_, warm);
+        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm);
+        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm);
     }
     if (launch_check("k_setup_step")) return -1;
     if (pout == prev3_ && prev3_) {

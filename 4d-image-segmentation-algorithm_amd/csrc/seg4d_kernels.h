@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "seg4d_net.h"  // LearnedStart
+
 namespace s4d {
 
 constexpr int kBlock = 256;
@@ -264,7 +266,7 @@ __device__ __forceinline__ bool map_plane_block(const PlaneMap& M, int b, int& b
 
 // warm: 0 = SOR starts from u^n (the reference); 1 = it starts from the
 // linear extrapolation 2 u^n - u^(n-1) (prev_full holds u^(n-1) on entry);
-// 2 = from start_full (interior points; the learned start); 3 = from the
+// 2 = from the learned start (interior points, LearnedStart); 3 = from the
 // quadratic extrapolation 3 u^n - 3 u^(n-1) + u^(n-2) (prev2 holds u^(n-2));
 // 4 = from the cubic one 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) (prev3
 // holds u^(n-3)).  Non-null history arrays shift by one level: prev3
@@ -279,7 +281,7 @@ template <bool WITH_G, bool POW2, bool MIXED = false>
 static __global__ __launch_bounds__(kBlock) void k_setup_step(
     Geo G, PlaneMap PM, Scale S, double eps2, double h4, const double* __restrict__ u,
     CPtr8 g, SplitBufs sb, const double* hist1, const double* hist2, const double* hist3,
-    double* prev_out, const double* __restrict__ start_full, double* __restrict__ partials,
+    double* prev_out, LearnedStart ls, double* __restrict__ partials,
     int warm, double cc = 0.0, double* __restrict__ partials2 = nullptr) {
     int bx, k, l;
     if (!map_plane_block(PM, blockIdx.x, bx, k, l)) return;
@@ -295,13 +297,17 @@ static __global__ __launch_bounds__(kBlock) void k_setup_step(
         const bool in = is_interior(G, i, j, k, l);
         // history u^(n-1), u^(n-2), u^(n-3) as the start needs it; u^n goes to
         // prev_out, which may be one of them (read first, same point)
-        const double pf = (warm == 1 || warm >= 3) ? hist1[p] : 0.0;
-        const double p2 = (warm >= 3) ? hist2[p] : 0.0;
-        const double p3 = (warm == 4) ? hist3[p] : 0.0;
+        const bool cub = warm == 4 || (warm == 2 && ls.order == 3);
+        const double pf = warm >= 1 ? hist1[p] : 0.0;
+        const double p2 = (warm == 3 || cub) ? hist2[p] : 0.0;
+        const double p3 = cub ? hist3[p] : 0.0;
         if (warm == 1 && in) start = 2.0 * uc - pf;
-        if (warm == 2 && in) start = start_full[p];
         if (warm == 3 && in) start = (3.0 * uc - 3.0 * pf) + p2;
-        if (warm == 4 && in) start = ((4.0 * uc - 6.0 * pf) + 4.0 * p2) - p3;
+        if (cub && in) start = ((4.0 * uc - 6.0 * pf) + 4.0 * p2) - p3;
+        if (warm == 2 && in) {
+            const double base = ls.order == 3 ? start : uc + (uc - pf);
+            start = base + sqrt(*ls.dsum * ls.inv_n) * (double)(ls.out[p] * ls.inv_st);
+        }
         prev_out[p] = uc;
         if constexpr (MIXED) {
             // (at ghosts start = u^n, so d0 = 0 there)

@@ -3,6 +3,7 @@
 // seg4d_net_kernels.h, Adam, and the prediction used by Engine::setup.
 #include "seg4d_net.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -26,6 +27,14 @@ constexpr int kPatch = kPatchCore + 2 * kHalo + 2;  // padded patch extent (74)
 constexpr int kSpatialTaps = 27, kTemporalTaps = 3;
 
 long cdivl(long a, long b) { return (a + b - 1) / b; }
+
+// Blocks of k_pred_moments over nv voxels: at most 2048, each a range of *per
+// voxels (a multiple of 256).
+int sumsq_blocks(long nv, long* per) {
+    const long nb = std::min<long>(2048, cdivl(nv, 256));
+    *per = cdivl(cdivl(nv, nb), 256) * 256;
+    return (int)cdivl(nv, *per);
+}
 
 uint64_t splitmix(uint64_t& s) {
     uint64_t z = (s += 0x9E3779B97F4A7C15ull);
@@ -100,79 +109,103 @@ __device__ __forceinline__ double step_sigma(const double* dsum, double inv_n) {
     return dsum ? sqrt(*dsum * inv_n) : 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_features(NGeo G, const double* __restrict__ u,
-                                                  const double* __restrict__ prev,
-                                                  const bf16* __restrict__ gm, float sd,
-                                                  const double* __restrict__ dsum, double inv_n,
+// The extrapolation the network corrects at one point: linear 2u - h1, or
+// cubic 4u - 6h1 + 4h2 - h3 (grouped like the solver's own cubic start).
+__device__ __forceinline__ double base_start(int order, double uc, double p1, double p2,
+                                             double p3) {
+    if (order == 3) return ((4.0 * uc - 6.0 * p1) + 4.0 * p2) - p3;
+    return uc + (uc - p1);
+}
+
+// Third difference u - 3 h1 + 3 h2 - h3 of four consecutive levels: the
+// error the quadratic extrapolation made at the last step, whose rms kappa
+// scales the cubic base's correction.
+__device__ __forceinline__ double third_diff(double uc, double p1, double p2, double p3) {
+    return ((uc - 3.0 * p1) + 3.0 * p2) - p3;
+}
+
+// Features of the prediction; sums[0] = sum of d^2, sums[1] = sum of the
+// third difference squared over the interior (order 3).
+__global__ __launch_bounds__(256) void k_features(NGeo G, int order, const double* __restrict__ u,
+                                                  const double* __restrict__ h1,
+                                                  const double* __restrict__ h2,
+                                                  const double* __restrict__ h3,
+                                                  const bf16* __restrict__ gm, float sd, float se,
+                                                  const double* __restrict__ sums, double inv_n,
                                                   bf16* __restrict__ feat) {
     const long v = (long)blockIdx.x * 256 + threadIdx.x;
     if (v >= (long)G.I * G.J * G.K * G.L) return;
     int i, j, k, l;
     if (!point_of(G, v, i, j, k, l)) return;
-    const double uc = u[v];
-    const double d = prev ? uc - prev[v] : 0.0;
-    const double sig = step_sigma(dsum, inv_n);
+    const double uc = u[v], p1 = h1[v];
+    const double sig = step_sigma(sums, inv_n);
     const double inv_sig = sig > 0.0 ? 1.0 / sig : 0.0;
     bf16x8 f;
     f[0] = (bf16)(float)uc;
-    f[1] = (bf16)((float)(d * inv_sig) * sd);
+    f[1] = (bf16)((float)((uc - p1) * inv_sig) * sd);
     f[2] = gm[v];
+    f[3] = (bf16)0.0f;
+    if (order == 3) {
+        const double kap = step_sigma(sums + 1, inv_n);
+        const double inv_kap = kap > 0.0 ? 1.0 / kap : 0.0;
+        f[3] = (bf16)((float)(third_diff(uc, p1, h2[v], h3[v]) * inv_kap) * se);
+    }
 #pragma unroll
-    for (int q = 3; q < 8; q++) f[q] = (bf16)0.0f;
+    for (int q = 4; q < 8; q++) f[q] = (bf16)0.0f;
     *reinterpret_cast<bf16x8*>(feat + v * 8) = f;
 }
 
-__global__ __launch_bounds__(256) void k_start(NGeo G, const double* __restrict__ u,
-                                               const double* __restrict__ prev,
-                                               const float* __restrict__ out, float inv_st,
-                                               const double* __restrict__ dsum, double inv_n,
-                                               double* __restrict__ start) {
-    const long v = (long)blockIdx.x * 256 + threadIdx.x;
-    if (v >= (long)G.I * G.J * G.K * G.L) return;
-    int i, j, k, l;
-    if (!point_of(G, v, i, j, k, l)) return;
-    const double uc = u[v];
-    const double d = prev ? uc - prev[v] : 0.0;
-    start[v] = uc + d + step_sigma(dsum, inv_n) * (double)(out[v] * inv_st);
-}
-
-// Per-block partial sums (FP64) of (a - b)^2 over the interior points.
-template <typename T>
-__global__ __launch_bounds__(256) void k_sumsq_diff(NGeo G, const T* __restrict__ a,
-                                                    const T* __restrict__ b,
-                                                    double* __restrict__ partial) {
-    __shared__ double sm[256];
-    const long v = (long)blockIdx.x * 256 + threadIdx.x;
-    double s = 0;
-    int i, j, k, l;
-    if (v < (long)G.I * G.J * G.K * G.L && point_of(G, v, i, j, k, l)) {
-        const double d = (double)a[v] - (double)b[v];
-        s = d * d;
+// Per-block partial sums (FP64) over the interior of d^2 = (u - h1)^2 and
+// (with h2, h3) of the third difference squared; block b covers the voxels
+// [b per, (b + 1) per) (per a multiple of 256), so that a volume gives at
+// most a few thousand partials: partial[2b], partial[2b + 1].
+__global__ __launch_bounds__(256) void k_pred_moments(NGeo G, const double* __restrict__ u,
+                                                      const double* __restrict__ h1,
+                                                      const double* __restrict__ h2,
+                                                      const double* __restrict__ h3, long per,
+                                                      double* __restrict__ partial) {
+    __shared__ double sm[2][256];
+    const long nv = (long)G.I * G.J * G.K * G.L;
+    const long v0 = (long)blockIdx.x * per, v1 = v0 + per < nv ? v0 + per : nv;
+    double s = 0, s3 = 0;
+    for (long v = v0 + threadIdx.x; v < v1; v += 256) {
+        int i, j, k, l;
+        if (point_of(G, v, i, j, k, l)) {
+            const double uc = u[v], p1 = h1[v];
+            s += (uc - p1) * (uc - p1);
+            if (h3) {
+                const double t = third_diff(uc, p1, h2[v], h3[v]);
+                s3 += t * t;
+            }
+        }
     }
-    sm[threadIdx.x] = s;
+    sm[0][threadIdx.x] = s;
+    sm[1][threadIdx.x] = s3;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        if (threadIdx.x < w) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + w];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + w];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
-}
-
-__global__ __launch_bounds__(256) void k_to_f32(const double* __restrict__ a, long n,
-                                                float* __restrict__ b) {
-    const long v = (long)blockIdx.x * 256 + threadIdx.x;
-    if (v < n) b[v] = (float)a[v];
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = sm[0][0];
+        partial[2 * blockIdx.x + 1] = sm[1][0];
+    }
 }
 
 // One patch of a training batch: patch point (pi, pj, pk, pl) of batch slot
 // b reads run point (ax + pi, ay + pj, az + pk, pl).  Patch ghosts are zero;
 // loss points are the interior points at least kHalo + 1 from a patch
-// ghost that is not also a ghost of the run.
+// ghost that is not also a ghost of the run.  um1..um3: levels n-1..n-3
+// (um2, um3 only for the cubic base), u0 level n, up1 level n+1.
 __global__ __launch_bounds__(256) void k_gather_patch(
-    NGeo PG, int b, int RI, int RJ, int RK, const float* __restrict__ um1,
-    const float* __restrict__ u0, const float* __restrict__ up1,
-    const bf16* __restrict__ gm, int ax, int ay, int az, float sd, float st, float inv_sig,
-    bf16* __restrict__ feat, float* __restrict__ target, unsigned char* __restrict__ lmask) {
+    NGeo PG, int b, int RI, int RJ, int RK, int order, const double* __restrict__ um3,
+    const double* __restrict__ um2, const double* __restrict__ um1, const double* __restrict__ u0,
+    const double* __restrict__ up1, const bf16* __restrict__ gm, int ax, int ay, int az, float sd,
+    float se, float st, double inv_sig, double inv_kap, bf16* __restrict__ feat,
+    float* __restrict__ target, unsigned char* __restrict__ lmask) {
     const long q = (long)blockIdx.x * 256 + threadIdx.x;
     const long per = (long)PG.I * PG.J * PG.K * PG.P;
     if (q >= per) return;
@@ -192,11 +225,18 @@ __global__ __launch_bounds__(256) void k_gather_patch(
     unsigned char lm = 0;
     if (!ghost) {
         const long rv = (((long)pl * RK + (az + pk)) * RJ + (ay + pj)) * RI + (ax + pi);
-        const float uc = u0[rv], um = um1[rv], up = up1[rv];
-        f[0] = (bf16)uc;
-        f[1] = (bf16)((uc - um) * inv_sig * sd);
+        const double uc = u0[rv], um = um1[rv], up = up1[rv];
+        f[0] = (bf16)(float)uc;
+        f[1] = (bf16)((float)((uc - um) * inv_sig) * sd);
         f[2] = gm[rv];
-        tg = ((up - uc) - (uc - um)) * inv_sig * st;
+        if (order == 3) {
+            const double p2 = um2[rv], p3 = um3[rv];
+            const double base = ((4.0 * uc - 6.0 * um) + 4.0 * p2) - p3;
+            f[3] = (bf16)((float)(third_diff(uc, um, p2, p3) * inv_kap) * se);
+            tg = (float)((up - base) * inv_kap) * st;
+        } else {
+            tg = (float)((up - (uc + (uc - um))) * inv_sig) * st;
+        }
         auto ok = [](int p, int n, int a, int rn) {
             return (p >= kHalo + 1 || a == 0) && (p <= n - kHalo - 2 || a + n == rn);
         };
@@ -207,35 +247,48 @@ __global__ __launch_bounds__(256) void k_gather_patch(
     lmask[v] = lm;
 }
 
-// Per-block partial sums (FP64) of d^2 and c^2 over the interior of a run
-// pair: d = u0 - um1, c = (up1 - u0) - d.
-__global__ __launch_bounds__(256) void k_moments(NGeo G, const float* __restrict__ um1,
-                                                 const float* __restrict__ u0,
-                                                 const float* __restrict__ up1,
+// Per-block partial sums (FP64) over the interior of a run level n of d^2,
+// e^2 and c^2: d = u^n - u^(n-1), e = the third difference (order 3; 0 for
+// the linear base), c = u^(n+1) - base.
+__global__ __launch_bounds__(256) void k_moments(NGeo G, int order, const double* __restrict__ um3,
+                                                 const double* __restrict__ um2,
+                                                 const double* __restrict__ um1,
+                                                 const double* __restrict__ u0,
+                                                 const double* __restrict__ up1,
                                                  double* __restrict__ partial) {
-    __shared__ double s1[256], s2[256];
+    __shared__ double s1[256], s2[256], s3[256];
     const long v = (long)blockIdx.x * 256 + threadIdx.x;
-    double a = 0, c = 0;
+    double a = 0, e2 = 0, c = 0;
     int i, j, k, l;
     if (v < (long)G.I * G.J * G.K * G.L && point_of(G, v, i, j, k, l)) {
-        const double d = (double)u0[v] - um1[v];
-        const double e = ((double)up1[v] - u0[v]) - d;
+        const double uc = u0[v], um = um1[v];
+        const double d = uc - um;
+        double base = uc + d, t3 = 0.0;
+        if (order == 3) {
+            base = ((4.0 * uc - 6.0 * um) + 4.0 * um2[v]) - um3[v];
+            t3 = third_diff(uc, um, um2[v], um3[v]);
+        }
+        const double e = up1[v] - base;
         a = d * d;
+        e2 = t3 * t3;
         c = e * e;
     }
     s1[threadIdx.x] = a;
-    s2[threadIdx.x] = c;
+    s2[threadIdx.x] = e2;
+    s3[threadIdx.x] = c;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             s1[threadIdx.x] += s1[threadIdx.x + s];
             s2[threadIdx.x] += s2[threadIdx.x + s];
+            s3[threadIdx.x] += s3[threadIdx.x + s];
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = s1[0];
-        partial[2 * blockIdx.x + 1] = s2[0];
+        partial[3 * blockIdx.x] = s1[0];
+        partial[3 * blockIdx.x + 1] = s2[0];
+        partial[3 * blockIdx.x + 2] = s3[0];
     }
 }
 
@@ -310,7 +363,7 @@ int Net::check(hipError_t e, const char* what) {
 
 int Net::layout() {
     int o = 0;
-    for (int L = 0; L < kLayers; L++) {
+    for (int L = 0; L < nlayers_; L++) {
         const int cin = L == 0 ? net::kCin0 : kCh;
         off_ws[L] = o;
         o += net::kMaxTaps * cin * kCh;
@@ -340,11 +393,11 @@ void Net::release() {
     fr(grad_);
     for (Run& r : runs_) {
         fr(r.gmean);
-        for (float*& s : r.snaps) fr(s);
+        for (double*& s : r.snaps) fr(s);
     }
     runs_.clear();
     fr(feat_);
-    for (int L = 0; L < kLayers; L++) {
+    for (int L = 0; L < kMaxLayers; L++) {
         fr(S_[L]);
         fr(Y_[L]);
     }
@@ -364,8 +417,14 @@ void Net::release() {
     adam_t_ = 0;
 }
 
-int Net::init(uint64_t seed) {
+int Net::init(uint64_t seed, int layers, int base) {
     release();
+    if (layers < 1 || layers > kMaxLayers || (base != 1 && base != 3)) {
+        err = "net: layers must be 1..4 and the base order 1 or 3";
+        return -1;
+    }
+    nlayers_ = layers;
+    base_ = base;
     layout();
     const size_t nb = (size_t)nparam;
     if (check(hipMalloc((void**)&wm_, nb * 4), "hipMalloc net") ||
@@ -386,9 +445,9 @@ int Net::init(uint64_t seed) {
         for (int q = 0; q < 12; q++) a += uniform01(s);
         return a - 6.0;
     };
-    for (int L = 0; L < kLayers; L++) {
+    for (int L = 0; L < nlayers_; L++) {
         const int cin = L == 0 ? net::kCin0 : kCh;
-        const int cin_used = L == 0 ? 3 : kCh;
+        const int cin_used = L == 0 ? (base_ == 3 ? 4 : 3) : kCh;
         const double ss = std::sqrt(2.0 / (kSpatialTaps * cin_used));
         for (int t = 0; t < kSpatialTaps; t++)
             for (int ci = 0; ci < cin_used; ci++)
@@ -400,7 +459,7 @@ int Net::init(uint64_t seed) {
                 for (int co = 0; co < kCh; co++)
                     w[off_wt[L] + (t * kCh + ci) * kCh + co] = (float)(st * normal());
     }
-    // the projection starts at zero: the initial prediction is the linear
+    // the projection starts at zero: the initial prediction is the base
     // extrapolation itself
     if (check(hipMemcpy(wm_, w.data(), nb * 4, hipMemcpyHostToDevice), "net weights") ||
         check(hipMemset(m1_, 0, nb * 4), "net memset") ||
@@ -409,7 +468,7 @@ int Net::init(uint64_t seed) {
         check(hipMemset(wb_, 0, nb * 2), "net memset") ||
         check(hipMemset(wtr_, 0, nb * 2), "net memset"))
         return -1;
-    s_d = s_t = 1.f;
+    s_d = s_t = s_e = 1.f;
     adam_t_ = 0;
     return refresh();
 }
@@ -422,7 +481,7 @@ int Net::ensure_acts(int I, int J, int K, int L, bool training) {
         p = nullptr;
     };
     fr(feat_);
-    for (int q = 0; q < kLayers; q++) {
+    for (int q = 0; q < kMaxLayers; q++) {
         fr(S_[q]);
         fr(Y_[q]);
     }
@@ -439,7 +498,7 @@ int Net::ensure_acts(int I, int J, int K, int L, bool training) {
         return check(hipMemset(*p, 0, bytes), "net memset");
     };
     if (al(&feat_, pts * 8 * 2)) return -1;
-    const int nS = training ? kLayers : 1, nY = training ? kLayers : 2;
+    const int nS = training ? nlayers_ : 1, nY = training ? nlayers_ : 2;
     for (int q = 0; q < nS; q++)
         if (al(&S_[q], pts * kCh * 2)) return -1;
     for (int q = 0; q < nY; q++)
@@ -487,22 +546,27 @@ int Net::forward(const NGeo& G, hipStream_t st, bool training) {
         // inference: one fused kernel per layer, S kept on chip
         const long nlb = (long)((G.I - 2 + 15) / 16) * ((G.J - 2 + 3) / 4) * ((G.K - 2 + 3) / 4) *
                          (G.L / G.P);
-        for (int L = 0; L < kLayers; L++) {
+        // the last layer's epilogue is the 1x1 projection (out_, FP32)
+        for (int L = 0; L < nlayers_; L++) {
             bf16* Y = Y_[L & 1];
-            if (L == 0)
-                net::k_layer_fwd<8><<<dim3((unsigned)nlb), 256, 0, st>>>(
-                    G, X, wb_ + off_ws[L], wb_ + off_wt[L], wm_ + off_b[L], Y);
+            const bool last = L == nlayers_ - 1;
+            const dim3 lg((unsigned)nlb);
+            const bf16 *ws = wb_ + off_ws[L], *wt = wb_ + off_wt[L];
+            const float* b = wm_ + off_b[L];
+            const float *wp = wm_ + off_wp, *bp = wm_ + off_bp;
+            if (L == 0 && last)
+                net::k_layer_fwd<8, true><<<lg, 256, 0, st>>>(G, X, ws, wt, b, Y, wp, bp, out_);
+            else if (L == 0)
+                net::k_layer_fwd<8, false><<<lg, 256, 0, st>>>(G, X, ws, wt, b, Y, wp, bp, out_);
+            else if (last)
+                net::k_layer_fwd<16, true><<<lg, 256, 0, st>>>(G, X, ws, wt, b, Y, wp, bp, out_);
             else
-                net::k_layer_fwd<16><<<dim3((unsigned)nlb), 256, 0, st>>>(
-                    G, X, wb_ + off_ws[L], wb_ + off_wt[L], wm_ + off_b[L], Y);
+                net::k_layer_fwd<16, false><<<lg, 256, 0, st>>>(G, X, ws, wt, b, Y, wp, bp, out_);
             X = Y;
         }
-        const long nv = (long)G.I * G.J * G.K * G.L;
-        net::k_project<<<(unsigned)cdivl(nv, 256), 256, 0, st>>>(G, X, wm_ + off_wp,
-                                                                 wm_ + off_bp, out_);
         return check(hipGetLastError(), "net forward");
     }
-    for (int L = 0; L < kLayers; L++) {
+    for (int L = 0; L < nlayers_; L++) {
         bf16* S = training ? S_[L] : S_[0];
         bf16* Y = training ? Y_[L] : Y_[L & 1];
         if (L == 0)
@@ -531,7 +595,7 @@ int Net::backward(const NGeo& G, hipStream_t st, float inv_n) {
     const Taps Ttb = temporal_taps(G, -1);
     const int nbl = (int)cdivl(nv, 256);
     // loss and projection
-    net::k_loss_project_bwd<<<nbl, 256, 0, st>>>(G, out_, target_, lmask_, Y_[kLayers - 1],
+    net::k_loss_project_bwd<<<nbl, 256, 0, st>>>(G, out_, target_, lmask_, Y_[nlayers_ - 1],
                                                  wm_ + off_wp, 2.f * inv_n, dA_, partial_);
     // partial columns: 0 = sum d^2, 1..16 = sum d a_c, 17 = sum d; the loss
     // is sum d^2 / N, so the projection's gradients are (2/N) times the sums
@@ -541,7 +605,7 @@ int Net::backward(const NGeo& G, hipStream_t st, float inv_n) {
     const int nblk = (int)std::min<long>(2048, nrow);
     const int rpb = (int)cdivl(nrow, nblk);
     const long dw_stride = (long)net::kMaxTaps * 16 * kCh;
-    for (int L = kLayers - 1; L >= 0; L--) {
+    for (int L = nlayers_ - 1; L >= 0; L--) {
         // dA_ holds the gradient at the pre-activation of layer L
         net::k_bias_grad<<<nblk, 256, 0, st>>>(G, dA_, partial_, rpb);
         k_reduce_cols<<<kCh, 256, 0, st>>>(partial_, nblk, kCh, 0, kCh, 1.f, grad_ + off_b[L]);
@@ -596,69 +660,89 @@ int Net::record(Engine& E, bool new_run) {
         err = "net: record with other dimensions than the run";
         return -1;
     }
-    float* s = nullptr;
-    if (check(hipMalloc((void**)&s, pts * 4), "hipMalloc pool")) return -1;
-    k_to_f32<<<(unsigned)cdivl(pts, 256), 256, 0, E.st_>>>(E.full(A_U), pts, s);
+    double* s = nullptr;
+    if (check(hipMalloc((void**)&s, pts * sizeof(double)), "hipMalloc pool") ||
+        check(hipMemcpyAsync(s, E.full(A_U), pts * sizeof(double), hipMemcpyDeviceToDevice,
+                             E.st_), "pool copy"))
+        return -1;
     r.snaps.push_back(s);
-    // rms increment of this level over the interior (0 for the first level)
-    double sig = 0.0;
+    // rms increment and rms third difference of this level over the interior
+    // (0 while the run has too few levels)
+    double sig = 0.0, kap = 0.0;
     if (r.snaps.size() >= 2) {
         const NGeo G{r.I, r.J, r.K, r.L, r.L};
-        const int nb = (int)cdivl(pts, 256);
+        long per = 0;
+        const int nb = sumsq_blocks(pts, &per);
         double* part = nullptr;
-        if (check(hipMalloc((void**)&part, (size_t)nb * sizeof(double)), "hipMalloc")) return -1;
-        k_sumsq_diff<float><<<nb, 256, 0, E.st_>>>(G, s, r.snaps[r.snaps.size() - 2], part);
-        k_reduce_d<<<1, 256, 0, E.st_>>>(part, nb, 1, dscal_);
-        const int rc = check(hipMemcpyAsync(hscal_, dscal_, sizeof(double), hipMemcpyDeviceToHost,
-                                            E.st_), "record") ||
+        if (check(hipMalloc((void**)&part, (size_t)nb * 2 * sizeof(double)), "hipMalloc"))
+            return -1;
+        const size_t n = r.snaps.size() - 1;  // this level
+        const double* h2 = n >= 3 ? r.snaps[n - 2] : nullptr;
+        const double* h3 = n >= 3 ? r.snaps[n - 3] : nullptr;
+        k_pred_moments<<<nb, 256, 0, E.st_>>>(G, s, r.snaps[n - 1], h2, h3, per, part);
+        k_reduce_d<<<1, 256, 0, E.st_>>>(part, nb, 2, dscal_);
+        const int rc = check(hipMemcpyAsync(hscal_, dscal_, 2 * sizeof(double),
+                                            hipMemcpyDeviceToHost, E.st_), "record") ||
                        check(hipStreamSynchronize(E.st_), "record");
         (void)hipFree(part);
         if (rc) return -1;
         const double nint = (double)(r.I - 2) * (r.J - 2) * (r.K - 2) * (r.L - 2);
         sig = std::sqrt(hscal_[0] / nint);
+        kap = std::sqrt(hscal_[1] / nint);
     }
     r.sigma.push_back(sig);
+    r.kappa.push_back(kap);
     return check(hipStreamSynchronize(E.st_), "net record");
 }
 
 long Net::pool_pairs() const {
+    // levels n with the base's history n-1..n-base and a successor n+1
     long n = 0;
-    for (const Run& r : runs_) n += r.snaps.size() > 2 ? (long)r.snaps.size() - 2 : 0;
+    for (const Run& r : runs_) {
+        const long m = (long)r.snaps.size() - 1 - base_;
+        n += m > 0 ? m : 0;
+    }
     return n;
 }
 
 int Net::compute_scales(hipStream_t st) {
-    double sd = 0, sc = 0;
+    double sd = 0, se = 0, sc = 0;
     long npts = 0;
-    std::vector<double> host(2);
     for (const Run& r : runs_) {
         const NGeo G{r.I, r.J, r.K, r.L, r.L};
         const long pts = (long)r.I * r.J * r.K * r.L;
         const int nb = (int)cdivl(pts, 256);
         double* part = nullptr;
-        if (check(hipMalloc((void**)&part, (size_t)nb * 2 * sizeof(double)), "hipMalloc")) return -1;
-        for (size_t n = 1; n + 1 < r.snaps.size(); n++) {
-            k_moments<<<nb, 256, 0, st>>>(G, r.snaps[n - 1], r.snaps[n], r.snaps[n + 1], part);
-            k_reduce_d<<<1, 256, 0, st>>>(part, nb, 2, dscal_);
-            if (check(hipMemcpyAsync(hscal_, dscal_, 2 * sizeof(double), hipMemcpyDeviceToHost, st),
+        if (check(hipMalloc((void**)&part, (size_t)nb * 3 * sizeof(double)), "hipMalloc")) return -1;
+        for (size_t n = (size_t)base_; n + 1 < r.snaps.size(); n++) {
+            const double* um3 = base_ == 3 ? r.snaps[n - 3] : nullptr;
+            const double* um2 = base_ == 3 ? r.snaps[n - 2] : nullptr;
+            k_moments<<<nb, 256, 0, st>>>(G, base_, um3, um2, r.snaps[n - 1], r.snaps[n],
+                                          r.snaps[n + 1], part);
+            k_reduce_d<<<1, 256, 0, st>>>(part, nb, 3, dscal_);
+            if (check(hipMemcpyAsync(hscal_, dscal_, 3 * sizeof(double), hipMemcpyDeviceToHost, st),
                       "moments") ||
                 check(hipStreamSynchronize(st), "moments")) {
                 (void)hipFree(part);
                 return -1;
             }
-            // features and targets are divided by the level's rms increment
+            // features by the level's rms increment / third difference, targets
+            // by the latter (cubic base) or the former (linear base)
             const double s2 = r.sigma[n] > 0 ? r.sigma[n] * r.sigma[n] : 1.0;
+            const double k2 = r.kappa[n] > 0 ? r.kappa[n] * r.kappa[n] : 1.0;
             sd += hscal_[0] / s2;
-            sc += hscal_[1] / s2;
+            se += hscal_[1] / k2;
+            sc += hscal_[2] / (base_ == 3 ? k2 : s2);
             npts += (long)(r.I - 2) * (r.J - 2) * (r.K - 2) * (r.L - 2);
         }
         (void)hipFree(part);
     }
     if (npts == 0 || !(sd > 0) || !(sc > 0)) {
-        err = "net: training pool needs runs of at least 3 time levels with motion";
+        err = "net: training pool needs runs with motion and enough time levels for the base";
         return -1;
     }
     s_d = (float)(1.0 / std::sqrt(sd / npts));
+    s_e = se > 0 ? (float)(1.0 / std::sqrt(se / npts)) : 1.f;
     s_t = (float)(1.0 / std::sqrt(sc / npts));
     return 0;
 }
@@ -688,10 +772,10 @@ int Net::train(int iters, int batch, float lr, uint64_t seed, int log_every, dou
     if (adam_t_ == 0) rc = compute_scales(st);
     const NGeo G{kPatch, kPatch, kPatch, batch * P, P};
     if (!rc) rc = ensure_acts(G.I, G.J, G.K, G.L, true);
-    // (run, level) pairs with a predecessor and a successor
+    // (run, level) pairs with the base's history and a successor
     std::vector<std::pair<int, int>> pairs;
     for (int q = 0; q < (int)runs_.size(); q++)
-        for (int n = 1; n + 1 < (int)runs_[q].snaps.size(); n++) pairs.push_back({q, n});
+        for (int n = base_; n + 1 < (int)runs_[q].snaps.size(); n++) pairs.push_back({q, n});
     uint64_t s = seed;
     double tail = 0;
     int ntail = 0;
@@ -706,10 +790,12 @@ int Net::train(int iters, int batch, float lr, uint64_t seed, int log_every, dou
             const int ax = (int)(uniform01(s) * (r.I - kPatch + 1));
             const int ay = (int)(uniform01(s) * (r.J - kPatch + 1));
             const int az = (int)(uniform01(s) * (r.K - kPatch + 1));
-            const float* um1 = r.snaps[n - 1];
+            const double* um3 = base_ == 3 ? r.snaps[n - 3] : nullptr;
+            const double* um2 = base_ == 3 ? r.snaps[n - 2] : nullptr;
             k_gather_patch<<<(unsigned)cdivl(per, 256), 256, 0, st>>>(
-                G, b, r.I, r.J, r.K, um1, r.snaps[n], r.snaps[n + 1], r.gmean, ax, ay, az, s_d, s_t,
-                r.sigma[n] > 0 ? (float)(1.0 / r.sigma[n]) : 0.f, feat_, target_, lmask_);
+                G, b, r.I, r.J, r.K, base_, um3, um2, r.snaps[n - 1], r.snaps[n], r.snaps[n + 1],
+                r.gmean, ax, ay, az, s_d, s_e, s_t, r.sigma[n] > 0 ? 1.0 / r.sigma[n] : 0.0,
+                r.kappa[n] > 0 ? 1.0 / r.kappa[n] : 0.0, feat_, target_, lmask_);
             auto cnt = [&](int a, int rn) {
                 const int lo = a == 0 ? 1 : kHalo + 1;
                 const int hi = a + kPatch == rn ? kPatch - 2 : kPatch - kHalo - 2;
@@ -727,7 +813,7 @@ int Net::train(int iters, int batch, float lr, uint64_t seed, int log_every, dou
         const float c2 = 1.f - std::pow(b2, (float)adam_t_);
         net::k_adam<<<(unsigned)cdivl(nparam, 256), 256, 0, st>>>(wm_, m1_, m2_, grad_, nparam, lr,
                                                                   b1, b2, 1e-8f, c1, c2, wb_);
-        for (int L = 0; L < kLayers; L++) {
+        for (int L = 0; L < nlayers_; L++) {
             if (L > 0)
                 k_transpose_taps<<<(kSpatialTaps * 256 + 255) / 256, 256, 0, st>>>(
                     wb_ + off_ws[L], kSpatialTaps, wtr_ + off_ws[L]);
@@ -743,8 +829,7 @@ int Net::train(int iters, int batch, float lr, uint64_t seed, int log_every, dou
             ntail++;
         }
         if (log_every > 0 && (it % log_every == 0 || it == iters - 1)) {
-            std::printf("[net] step %d loss %.5f\n", it, lval);
-            std::fflush(stdout);
+            std::fprintf(stderr, "[net] step %d loss %.5f\n", it, lval);
         }
     }
     if (!rc) rc = check(hipStreamSynchronize(st), "train");
@@ -800,9 +885,14 @@ int Net::grad_batch(int I, int J, int K, int L, int P, const void* feat_dev,
     return 0;
 }
 
-int Net::predict(Engine& E, double* start, bool history) {
+int Net::predict(Engine& E, const double* h1, const double* h2, const double* h3,
+                 LearnedStart* ls) {
     if (!ready()) {
         err = "net: not initialised";
+        return -1;
+    }
+    if (!h1 || (base_ == 3 && (!h2 || !h3))) {
+        err = "net: the prediction needs the base's history levels";
         return -1;
     }
     const NGeo G{E.I_, E.J_, E.K_, E.L_, E.L_};
@@ -828,21 +918,25 @@ int Net::predict(Engine& E, double* start, bool history) {
                                                               g[4], g[5], g[6], g[7]);
         g_version_ = E.g_version_;
     }
-    const double* prev = history ? E.full(A_PREV) : nullptr;
+    const double* u = E.full(A_U);
     const double inv_n = 1.0 / ((double)(E.I_ - 2) * (E.J_ - 2) * (E.K_ - 2) * (E.L_ - 2));
-    const double* dsum = nullptr;
-    if (prev) {
-        // rms increment of this step, left on the device for the two kernels
-        const int nb = (int)cdivl(pts, 256);
-        k_sumsq_diff<double><<<nb, 256, 0, E.st_>>>(G, E.full(A_U), prev, (double*)partial_);
-        k_reduce_d<<<1, 256, 0, E.st_>>>((double*)partial_, nb, 1, dscal_ + 8);
-        dsum = dscal_ + 8;
-    }
-    k_features<<<(unsigned)cdivl(pts, 256), 256, 0, E.st_>>>(G, E.full(A_U), prev, gmean_, s_d,
-                                                             dsum, inv_n, feat_);
+    // sums of the increment and of the third difference squared, left on the
+    // device for the features and the set-up kernel
+    long per = 0;
+    const int nb = sumsq_blocks(pts, &per);
+    k_pred_moments<<<nb, 256, 0, E.st_>>>(G, u, h1, base_ == 3 ? h2 : nullptr,
+                                          base_ == 3 ? h3 : nullptr, per, (double*)partial_);
+    k_reduce_d<<<1, 256, 0, E.st_>>>((double*)partial_, nb, 2, dscal_ + 8);
+    const double* dsum = dscal_ + 8;
+    const unsigned grid = (unsigned)cdivl(pts, 256);
+    k_features<<<grid, 256, 0, E.st_>>>(G, base_, u, h1, h2, h3, gmean_, s_d, s_e, dsum, inv_n,
+                                        feat_);
     if (forward(G, E.st_, false)) return -1;
-    k_start<<<(unsigned)cdivl(pts, 256), 256, 0, E.st_>>>(G, E.full(A_U), prev, out_,
-                                                          1.f / s_t, dsum, inv_n, start);
+    ls->out = out_;
+    ls->dsum = base_ == 3 ? dsum + 1 : dsum;  // the target's scale: kappa or sigma
+    ls->inv_n = inv_n;
+    ls->inv_st = 1.f / s_t;
+    ls->order = base_;
     (void)hipEventRecord(e1, E.st_);
     if (check(hipEventSynchronize(e1), "net predict")) return -1;
     float ms = 0.f;
@@ -866,12 +960,16 @@ int Net::save(const char* path) {
         err = std::string("net: cannot create ") + path;
         return -1;
     }
-    const char magic[8] = {'S', '4', 'D', 'N', 'E', 'T', '0', '1'};
-    const int32_t np = nparam;
+    // S4DNET02: magic, parameter count, layers, base order, s_d, s_t, s_e,
+    // Adam step, FP32 parameters (little-endian)
+    const char magic[8] = {'S', '4', 'D', 'N', 'E', 'T', '0', '2'};
+    const int32_t np = nparam, nl = nlayers_, bo = base_;
     const int64_t t = adam_t_;
     bool ok = std::fwrite(magic, 1, 8, f) == 8 && std::fwrite(&np, 4, 1, f) == 1 &&
+              std::fwrite(&nl, 4, 1, f) == 1 && std::fwrite(&bo, 4, 1, f) == 1 &&
               std::fwrite(&s_d, 4, 1, f) == 1 && std::fwrite(&s_t, 4, 1, f) == 1 &&
-              std::fwrite(&t, 8, 1, f) == 1 && std::fwrite(w.data(), 4, w.size(), f) == w.size();
+              std::fwrite(&s_e, 4, 1, f) == 1 && std::fwrite(&t, 8, 1, f) == 1 &&
+              std::fwrite(w.data(), 4, w.size(), f) == w.size();
     ok = (std::fclose(f) == 0) && ok;
     if (!ok) err = "net: write failed";
     return ok ? 0 : -1;
@@ -883,17 +981,25 @@ int Net::load(const char* path) {
         err = std::string("net: cannot open ") + path;
         return -1;
     }
+    // S4DNET01 (four layers, linear base) or S4DNET02 (shape in the header)
     char magic[8];
-    int32_t np = 0;
-    float sd = 1, stv = 1;
+    int32_t np = 0, nl = 4, bo = 1;
+    float sd = 1, stv = 1, se = 1;
     int64_t t = 0;
-    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "S4DNET01", 8) == 0 &&
-              std::fread(&np, 4, 1, f) == 1 && std::fread(&sd, 4, 1, f) == 1 &&
-              std::fread(&stv, 4, 1, f) == 1 && std::fread(&t, 8, 1, f) == 1;
-    if (!ok || init(0)) {
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "S4DNET0", 7) == 0 &&
+              (magic[7] == '1' || magic[7] == '2') && std::fread(&np, 4, 1, f) == 1;
+    const bool v2 = ok && magic[7] == '2';
+    if (ok && v2) ok = std::fread(&nl, 4, 1, f) == 1 && std::fread(&bo, 4, 1, f) == 1;
+    ok = ok && std::fread(&sd, 4, 1, f) == 1 && std::fread(&stv, 4, 1, f) == 1;
+    if (ok && v2) ok = std::fread(&se, 4, 1, f) == 1;
+    ok = ok && std::fread(&t, 8, 1, f) == 1;
+    if (!ok) {
         std::fclose(f);
-        if (ok) return -1;
         err = "net: not a weights file";
+        return -1;
+    }
+    if (init(0, nl, bo)) {
+        std::fclose(f);
         return -1;
     }
     if (np != nparam) {
@@ -911,13 +1017,14 @@ int Net::load(const char* path) {
     if (check(hipMemcpy(wm_, w.data(), w.size() * 4, hipMemcpyHostToDevice), "net load")) return -1;
     s_d = sd;
     s_t = stv;
+    s_e = se;
     adam_t_ = t;
     return refresh();
 }
 
 int Net::refresh() {
     k_f32_to_bf16<<<(unsigned)cdivl(nparam, 256), 256>>>(wm_, nparam, wb_);
-    for (int L = 0; L < kLayers; L++) {
+    for (int L = 0; L < nlayers_; L++) {
         if (L > 0)
             k_transpose_taps<<<(kSpatialTaps * 256 + 255) / 256, 256>>>(wb_ + off_ws[L],
                                                                      kSpatialTaps,

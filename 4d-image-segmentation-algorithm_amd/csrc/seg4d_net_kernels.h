@@ -1,7 +1,8 @@
 // seg4d_net_kernels.h -- HIP kernels of the learned warm start: a separable
 // 4D convolutional network (3x3x3 spatial conv, then 3-tap temporal conv,
-// bias, ReLU; four such layers of 16 channels and a 1x1 projection) that
-// predicts the SOR start of a time step from (u^n, u^n - u^(n-1), g).
+// bias, ReLU; one to four such layers of 16 channels and a 1x1 projection)
+// that predicts the correction of an extrapolated SOR start from u^n, its
+// recent increments and g.
 //
 // Activations live on the solver's padded grid (I x J x K x L points, x
 // fastest), channel-last in BF16: element (v, c) at v * C + c with C = 8 for
@@ -31,7 +32,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kCh = 16;       // hidden channels
-constexpr int kCin0 = 8;      // input feature channels (3 used)
+constexpr int kCin0 = 8;      // input feature channels (3 used, 4 with the cubic base)
 constexpr int kMaxTaps = 28;  // 27 spatial taps padded to an even count
 constexpr int kWave = 64;
 
@@ -294,12 +295,19 @@ __global__ __launch_bounds__(256) void k_conv_s(NGeo G, int sign, const bf16* __
 // current one is computed.  Each wave reads and writes only its own part of
 // the ring; the patch is shared by the 4 waves.
 // ----------------------------------------------------------------------------
-template <int CIN>
+// PROJ (the network's last layer): instead of Y, the 1x1 projection
+// out[v] = sum_c wp[c] y[v][c] + bp[0] (FP32) of the layer output rounded to
+// BF16 as Y would hold it; the 16 channels of a point are spread over the 4
+// lanes m, m + 16, m + 32, m + 48 and summed with two lane exchanges.
+template <int CIN, bool PROJ>
 __global__ __launch_bounds__(256) void k_layer_fwd(NGeo G, const bf16* __restrict__ X,
                                                     const bf16* __restrict__ Ws,
                                                     const bf16* __restrict__ Wt,
                                                     const float* __restrict__ bias,
-                                                    bf16* __restrict__ Y) {
+                                                    bf16* __restrict__ Y,
+                                                    const float* __restrict__ wp,
+                                                    const float* __restrict__ bp,
+                                                    float* __restrict__ out) {
     constexpr int NT = 27;
     constexpr int TPS = 32 / CIN;
     constexpr int NS = (NT + TPS - 1) / TPS;
@@ -346,9 +354,13 @@ __global__ __launch_bounds__(256) void k_layer_fwd(NGeo G, const bf16* __restric
         }
         wt[s] = a;
     }
-    float bco[4];
+    float bco[4], wpr[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) bco[r] = bias[4 * g + r];
+    for (int r = 0; r < 4; r++) {
+        bco[r] = bias[4 * g + r];
+        wpr[r] = PROJ ? wp[4 * g + r] : 0.f;
+    }
+    const float bpo = PROJ ? bp[0] : 0.f;
     const int tap_in_step = (8 * g) / CIN, chunk = ((8 * g) % CIN) / 8;
     int toff[NS];
     bool tok[NS];
@@ -440,7 +452,21 @@ __global__ __launch_bounds__(256) void k_layer_fwd(NGeo G, const bf16* __restric
                     acc2[jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wt[s], bv, acc2[jj], 0, 0, 0);
                 }
             }
-            if (outok) {
+            if constexpr (PROJ) {
+                // every lane takes part in the exchanges; only in-range points store
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    float part = 0.f;
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        part += wpr[r] * (float)(bf16)fmaxf(acc2[jj][r] + bco[r], 0.0f);
+                    part += __shfl_xor(part, 16, 64);
+                    part += __shfl_xor(part, 32, 64);
+                    const int j = j0 + jj;
+                    if (outok && g == 0 && j <= G.J - 2)
+                        out[nvox(G, i, j, k, lbase + l - 1)] = part + bpo;
+                }
+            } else if (outok) {
 #pragma unroll
                 for (int jj = 0; jj < 4; jj++) {
                     const int j = j0 + jj;

@@ -167,6 +167,10 @@ class SegLib(_Lib):
         L.seg4dSetWarmStart.argtypes = [c_int]
         L.seg4dNetInit.restype = c_int
         L.seg4dNetInit.argtypes = [c_uint64]
+        L.seg4dNetInitShape.restype = c_int
+        L.seg4dNetInitShape.argtypes = [c_uint64, c_int, c_int]
+        L.seg4dNetShape.restype = c_int
+        L.seg4dNetShape.argtypes = [POINTER(c_int), POINTER(c_int)]
         for fn in ("seg4dNetLoad", "seg4dNetSave"):
             getattr(L, fn).restype = c_int
             getattr(L, fn).argtypes = [c_char_p]
@@ -361,9 +365,22 @@ class SegLib(_Lib):
         return self.lib.seg4dSetWarmStart(int(mode))
 
     # -- learned warm start ---------------------------------------------------
-    def net_init(self, seed=0):
-        if self.lib.seg4dNetInit(int(seed)) != 0:
+    def net_init(self, seed=0, layers=None, base=None):
+        """Fresh network weights; `layers` (1..4, default 4) layers correcting
+        the extrapolation of order `base` (1 linear, default; 3 cubic)."""
+        if layers is None and base is None:
+            rc = self.lib.seg4dNetInit(int(seed))
+        else:
+            rc = self.lib.seg4dNetInitShape(int(seed), int(layers or 4), int(base or 1))
+        if rc != 0:
             raise RuntimeError(f"seg4dNetInit failed: {self.last_error()}")
+
+    def net_shape(self):
+        """(layers, base order) of the current network."""
+        a, b = c_int(), c_int()
+        if self.lib.seg4dNetShape(ctypes.byref(a), ctypes.byref(b)) != 0:
+            raise RuntimeError(f"seg4dNetShape failed: {self.last_error()}")
+        return a.value, b.value
 
     def net_load(self, path):
         if self.lib.seg4dNetLoad(str(path).encode()) != 0:

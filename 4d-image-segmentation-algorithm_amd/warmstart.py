@@ -7,8 +7,9 @@ time-periodic radii, wedge gaps, noise sigma 25) fill the network's pool
 with the solution after every time step; the network is then fitted with
 Adam on patches of those runs and its weights saved.
 
-    python warmstart.py [--runs 8] [--steps 20] [--dims 160 160 96 8]
-                        [--iters 2500] [--batch 8] [--out weights.bin]
+    python warmstart.py [--runs 6] [--steps 30] [--dims 160 160 96 16]
+                        [--iters 2500] [--batch 8] [--layers 1] [--base 3]
+                        [--out weights.bin]
 """
 from __future__ import annotations
 
@@ -53,7 +54,7 @@ def record_runs(lib, runs, steps, dims, seed0=1000, log=print):
             lib.arr("u")[...] = u0
             lib.upload("u")
             lib.upload("Mmask")
-            lib.set_warm_start(1)  # the plain solver's system, fewer sweeps
+            lib.set_warm_start(4)  # the plain solver's system, fewer sweeps
             lib.net_record(new_run=True)
             for s in range(steps):
                 lib.set(numberOfTimeStep=s + 1)
@@ -67,10 +68,11 @@ def record_runs(lib, runs, steps, dims, seed0=1000, log=print):
         log(f"[warmstart] run {r + 1}/{runs}: {steps} steps in {time.perf_counter() - t0:.1f} s")
 
 
-def train(lib, runs=8, steps=20, dims=(160, 160, 96, 8), iters=2500, batch=8, lr=1e-3,
-          seed=1, out=DEFAULT_WEIGHTS, log=print):
-    """Records the runs, trains, saves; returns a summary dict."""
-    lib.net_init(seed)
+def train(lib, runs=6, steps=30, dims=(160, 160, 96, 16), iters=2500, batch=8, lr=1e-3,
+          seed=1, out=DEFAULT_WEIGHTS, layers=1, base=3, log=print):
+    """Records the runs, trains a network of `layers` layers correcting the
+    extrapolation of order `base`, saves; returns a summary dict."""
+    lib.net_init(seed, layers=layers, base=base)
     t0 = time.perf_counter()
     record_runs(lib, runs, steps, dims, log=log)
     t1 = time.perf_counter()
@@ -84,6 +86,7 @@ def train(lib, runs=8, steps=20, dims=(160, 160, 96, 8), iters=2500, batch=8, lr
         "runs": runs, "steps_per_run": steps, "run_volume": list(dims),
         "pairs": int(lib.net_pool_pairs()), "iterations": iters, "batch": batch,
         "patches": iters * batch, "lr": lr, "final_loss": loss,
+        "layers": layers, "base": base,
         "record_s": t1 - t0, "train_s": t2 - t1, "s_d": s_d, "s_t": s_t,
         "weights": out,
     }
@@ -91,13 +94,15 @@ def train(lib, runs=8, steps=20, dims=(160, 160, 96, 8), iters=2500, batch=8, lr
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--runs", type=int, default=8)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--dims", type=int, nargs=4, default=[160, 160, 96, 8])
+    ap.add_argument("--runs", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--dims", type=int, nargs=4, default=[160, 160, 96, 16])
     ap.add_argument("--iters", type=int, default=2500)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--layers", type=int, default=1)
+    ap.add_argument("--base", type=int, default=3, choices=(1, 3))
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     a = ap.parse_args()
     sys.path.insert(0, PKG_DIR)
@@ -106,7 +111,7 @@ def main():
     lib = seg4d.SegLib()
     lib.set_verbose(False)
     res = train(lib, a.runs, a.steps, tuple(a.dims), a.iters, a.batch, a.lr, a.seed, a.out,
-                log=lambda m: print(m, flush=True))
+                layers=a.layers, base=a.base, log=lambda m: print(m, flush=True))
     print(res, flush=True)
 
 

@@ -131,7 +131,12 @@ def main():
                          "save it to --net-out")
     ap.add_argument("--net-out", default=None)
     ap.add_argument("--train-iters", type=int, default=2500)
-    ap.add_argument("--train-runs", type=int, default=8)
+    ap.add_argument("--train-runs", type=int, default=6)
+    ap.add_argument("--net-layers", type=int, default=1,
+                    help="layers of a network trained with --train-net")
+    ap.add_argument("--net-base", type=int, default=3, choices=(1, 3),
+                    help="order of the extrapolation a network trained with --train-net "
+                         "corrects (1 linear, 3 cubic)")
     ap.add_argument("--inner-omega", type=float, default=None,
                     help="SOR factor of the FP32 correction solves (default: omega_c)")
     ap.add_argument("--batch", action="store_true",
@@ -162,7 +167,8 @@ def main():
     if args.train_net and not args.no_mixed:
         log("[bench] training the learned warm start ...")
         training = warmstart.train(lib, runs=args.train_runs, iters=args.train_iters,
-                                   out=args.net_out or net_path, log=log)
+                                   out=args.net_out or net_path, layers=args.net_layers,
+                                   base=args.net_base, log=log)
         net_path = args.net_out or net_path
         log(f"[bench] trained: {training}")
     spec = lib.gen_spec(args.preset, args.preset)
@@ -302,6 +308,10 @@ def main():
         if os.path.exists(net_path):
             lib.net_load(net_path)
             learned = run_refined(2)
+            layers, base = lib.net_shape()
+            learned["network"] = {"layers": layers, "channels": 16,
+                                  "corrects": "2u^n - u^(n-1)" if base == 1 else
+                                  "4u^n - 6u^(n-1) + 4u^(n-2) - u^(n-3)"}
             learned["weights"] = os.path.relpath(net_path, ROOT)
             learned["training"] = training
         else:

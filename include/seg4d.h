@@ -249,8 +249,10 @@ int seg4dLastRounds(void);
 /* Start of the SOR solve.  0 (default): u^n, as the reference.  1: the
  * linear extrapolation 2 u^n - u^(n-1), used when the previous call was a
  * completed _segTimeStep_ whose state stayed on the device (resident mode).
- * 2: the prediction of the learned network (seg4dNet*), under the same
- * condition as 1 (otherwise the solve starts from u^n).  3: the quadratic
+ * 2: the prediction of the learned network (seg4dNet*): the network's
+ * correction of the linear extrapolation (under the condition of 1) or of
+ * the cubic one (under the condition of 4; before that, as 4), by the
+ * network's base order.  3: the quadratic
  * extrapolation 3 u^n - 3 u^(n-1) + u^(n-2) after two such steps (after
  * one, as 1; one more padded FP64 array on the device).  4: the cubic one
  * 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) after three (else as 3; two more
@@ -262,22 +264,30 @@ int seg4dLastRounds(void);
 int seg4dSetWarmStart(int mode);
 
 /* ---- learned warm start (no counterpart in the reference) ---------------
- * A separable 4D convolutional network (4 x {3x3x3 spatial conv, 3-tap
+ * A separable 4D convolutional network (1 to 4 x {3x3x3 spatial conv, 3-tap
  * temporal conv, bias, ReLU}, 16 channels, 1x1 projection; BF16 MFMA
- * kernels, FP32 accumulation) predicts the correction of the linear
- * extrapolation 2 u^n - u^(n-1) from (u^n, u^n - u^(n-1), mean g).  It is
- * trained on solver runs: seg4dNetRecord after each _segTimeStep_ (and once
- * before the first) fills the pool; seg4dNetTrain fits it with Adam on
- * patches of 64^3 points x all frames.  The network outlives
- * _deallocateMem_, so runs of several volumes can be pooled.  Functions
- * return 0 (or the documented value) on success, -1 on failure. */
-int seg4dNetInit(uint64_t seed);          /* fresh weights, empty pool      */
+ * kernels, FP32 accumulation) predicts the correction of an extrapolation
+ * of the solution -- the linear 2 u^n - u^(n-1) (base order 1) or the cubic
+ * 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) (base order 3) -- from u^n, the
+ * increment u^n - u^(n-1), the cubic's difference from the linear one
+ * (order 3) and the mean g.  It is trained on solver runs: seg4dNetRecord
+ * after each _segTimeStep_ (and once before the first) fills the pool;
+ * seg4dNetTrain fits it with Adam on patches of 64^3 points x all frames.
+ * The network outlives _deallocateMem_, so runs of several volumes can be
+ * pooled.  Functions return 0 (or the documented value) on success, -1 on
+ * failure. */
+int seg4dNetInit(uint64_t seed);          /* fresh weights: 4 layers, order 1 */
+/* Fresh weights of `layers` (1..4) layers correcting the extrapolation of
+ * order `base` (1 or 3); empties the pool. */
+int seg4dNetInitShape(uint64_t seed, int layers, int base);
+int seg4dNetShape(int *layers, int *base);  /* shape of the current network */
 int seg4dNetLoad(const char *path);       /* weights file of seg4dNetSave   */
 int seg4dNetSave(const char *path);
 /* Snapshot of the current u (the device copy; uploaded first in mirror
  * mode); new_run != 0 starts a run and snapshots the mean g. */
 int seg4dNetRecord(int new_run);
-long seg4dNetPoolPairs(void);             /* (u^n, u^(n+1)) pairs pooled    */
+long seg4dNetPoolPairs(void);             /* training levels pooled (with the
+                                             base's history and u^(n+1))  */
 /* iters Adam steps (lr) on batches of `batch` patches; progress line every
  * log_every steps (0: silent).  Returns the mean loss of the last 10% of
  * steps, -1 on failure.  Runs need equal frame counts and >= 74 padded

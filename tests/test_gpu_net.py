@@ -1,6 +1,7 @@
 """Learned warm start: the BF16 MFMA convolution kernels against a plain
-PyTorch FP32 reference of the same network, training on solver runs, and the
-solver started from the network's prediction against the FP64 oracle.
+PyTorch FP32 reference of the same network (1, 2 and 4 layers), training on
+solver runs with the linear and the cubic base, and the solver started from
+the network's prediction against the FP64 oracle.
 
 Tolerances.  The kernels store activations and gradients in BF16 (8-bit
 mantissa, relative rounding 2^-9) and accumulate in FP32; the reference
@@ -25,9 +26,9 @@ KW = dict(h=1.0, tau=0.5, eps2=1e-6, K=1000.0, omega_c=1.8, omega_sigma=1.3, sig
           theta_img=1.0, rho_thr=0.0, maxNoGSIteration=100000, maxNoGSIteration_sigma=100000.0)
 
 
-def layout():
+def layout(layers=4):
     o, off = 0, {}
-    for L in range(4):
+    for L in range(layers):
         cin = 8 if L == 0 else 16
         off[("ws", L)] = (o, 28 * cin * 16)
         o += 28 * cin * 16
@@ -52,16 +53,16 @@ def to_bits(x):
     return t.view(torch.int16).numpy().view(np.uint16)
 
 
-def reference(p, feat, P):
+def reference(p, feat, P, layers=4):
     """Forward pass in FP32 with the kernels' BF16 rounding points.  p: torch
     parameter vector; feat: (L, K, J, I, 8) torch float32 (BF16 values).
     Returns out on the interior points, shape (B, nl, nk, nj, ni)."""
-    off, _ = layout()
+    off, _ = layout(layers)
     L_, K_, J_, I_, _ = feat.shape
     B = L_ // P
     x = feat.reshape(B, P, K_, J_, I_, 8)[:, 1:-1, 1:-1, 1:-1, 1:-1, :]
     nl, nk, nj, ni = x.shape[1:5]
-    for layer in range(4):
+    for layer in range(layers):
         cin = 8 if layer == 0 else 16
         o, n = off[("ws", layer)]
         ws = bf(p[o:o + 27 * cin * 16]).view(3, 3, 3, cin, 16).permute(4, 3, 0, 1, 2)
@@ -79,10 +80,10 @@ def reference(p, feat, P):
     return (x * p[o:o + 16]).sum(-1) + p[ob]
 
 
-def random_params(rng, scale=1.0):
-    off, n = layout()
+def random_params(rng, scale=1.0, layers=4):
+    off, n = layout(layers)
     w = np.zeros(n, dtype=np.float32)
-    for L in range(4):
+    for L in range(layers):
         cin = 8 if L == 0 else 16
         o, m = off[("ws", L)]
         blk = rng.normal(0, np.sqrt(2.0 / (27 * cin)), size=(28, cin, 16)).astype(np.float32)
@@ -109,6 +110,7 @@ def batch(rng, dims, P, B):
         blk[..., 0] = rng.uniform(0, 1, size=blk.shape[:-1])
         blk[..., 1] = rng.normal(0, 1, size=blk.shape[:-1])
         blk[..., 2] = rng.uniform(0, 1, size=blk.shape[:-1])
+        blk[..., 3] = rng.normal(0, 1, size=blk.shape[:-1])  # the cubic base's feature
     bits = to_bits(f)
     fr = torch.from_numpy(bits.view(np.int16).copy()).view(torch.bfloat16).to(torch.float32)
     return bits, fr
@@ -119,17 +121,21 @@ def interior(a, P):
     return a.reshape(L_ // P, P, K_, J_, I_)[:, 1:-1, 1:-1, 1:-1, 1:-1]
 
 
-@pytest.mark.parametrize("dims,P,B", [((12, 11, 10), 6, 2), ((35, 9, 8), 5, 1)])
-def test_forward_vs_torch(seglib, dims, P, B):
-    print(f"[gpu] net forward {dims} P={P} B={B}", flush=True)
+@pytest.mark.parametrize("dims,P,B,layers", [((12, 11, 10), 6, 2, 4), ((35, 9, 8), 5, 1, 4),
+                                             ((21, 10, 9), 7, 2, 1), ((17, 13, 6), 4, 1, 2)])
+def test_forward_vs_torch(seglib, dims, P, B, layers):
+    """Inference path: one fused kernel per layer, the last one with the
+    projection in its epilogue."""
+    print(f"[gpu] net forward {dims} P={P} B={B} layers={layers}", flush=True)
     rng = np.random.default_rng(1)
     lib = seglib
-    lib.net_init(3)
-    w = random_params(rng)
+    lib.net_init(3, layers=layers, base=3)
+    assert lib.net_shape() == (layers, 3)
+    w = random_params(rng, layers=layers)
     lib.net_set_params(w)
     bits, fr = batch(rng, dims, P, B)
     out = lib.net_forward(bits, P)
-    ref = reference(torch.from_numpy(w), fr, P).numpy()
+    ref = reference(torch.from_numpy(w), fr, P, layers).numpy()
     got = interior(out, P)
     err = np.abs(got - ref).max()
     print(f"  max|out| {np.abs(ref).max():.4f}, max err {err:.2e}", flush=True)
@@ -140,12 +146,13 @@ def test_forward_vs_torch(seglib, dims, P, B):
     assert np.array_equal(out, full)
 
 
-def test_gradient_vs_torch(seglib):
-    print("[gpu] net gradient", flush=True)
+@pytest.mark.parametrize("layers", [4, 1])
+def test_gradient_vs_torch(seglib, layers):
+    print(f"[gpu] net gradient layers={layers}", flush=True)
     rng = np.random.default_rng(2)
     lib = seglib
-    lib.net_init(4)
-    w = random_params(rng)
+    lib.net_init(4, layers=layers, base=1 if layers == 4 else 3)
+    w = random_params(rng, layers=layers)
     lib.net_set_params(w)
     dims, P, B = (13, 12, 11), 6, 2
     bits, fr = batch(rng, dims, P, B)
@@ -158,7 +165,7 @@ def test_gradient_vs_torch(seglib):
     mi[..., 2:-2, 2:-2, 2:-2] = 1  # a core, as in training
     g, loss = lib.net_grad(bits, target, lmask, P)
     pt = torch.from_numpy(w.copy()).requires_grad_(True)
-    out = reference(pt, fr, P)
+    out = reference(pt, fr, P, layers)
     tt = torch.from_numpy(np.ascontiguousarray(ti))
     mt = torch.from_numpy(np.ascontiguousarray(mi)).bool()
     d = (out - tt)[mt]
@@ -167,7 +174,7 @@ def test_gradient_vs_torch(seglib):
     gref = pt.grad.numpy()
     print(f"  loss {loss:.6f} vs {lref.item():.6f}", flush=True)
     assert loss == pytest.approx(lref.item(), rel=2e-2)
-    off, _ = layout()
+    off, _ = layout(layers)
     for key, (o, n) in off.items():
         a, b = g[o:o + n], gref[o:o + n]
         nb = np.linalg.norm(b)
@@ -187,13 +194,14 @@ def _volume(lib, dims, seed):
     return shape, img, u0
 
 
-def test_train_on_solver_runs(seglib, tmp_path):
+@pytest.mark.parametrize("layers,base", [(4, 1), (1, 3)])
+def test_train_on_solver_runs(seglib, tmp_path, layers, base):
     """Pool two short runs, train, and check that the loss falls below that
-    of the linear extrapolation (the untrained network's prediction, loss 1
-    by the choice of the target scale); save / load round trip."""
-    print("[gpu] net training", flush=True)
+    of the base extrapolation (the untrained network's prediction, loss 1 by
+    the choice of the target scale); save / load round trip."""
+    print(f"[gpu] net training layers={layers} base={base}", flush=True)
     lib = seglib
-    lib.net_init(5)
+    lib.net_init(5, layers=layers, base=base)
     dims = (72, 72, 72, 4)
     for seed in (1, 2):
         shape, img, u0 = _volume(lib, dims, seed)
@@ -217,31 +225,34 @@ def test_train_on_solver_runs(seglib, tmp_path):
             lib.set_resident(False)
             lib.set_relative_tolerance(0.0)
             lib.deallocate()
-    assert lib.net_pool_pairs() == 10  # levels 1..5 of each run
+    # levels base..5 of each run's 7 (the base's history and a successor)
+    assert lib.net_pool_pairs() == 2 * (6 - base)
     loss = lib.net_train(60, batch=2, lr=1e-3, seed=1, log_every=20)
     s_d, s_t = lib.net_scales()
     print(f"  tail loss {loss:.4f}, scales {s_d:.3g} {s_t:.3g}", flush=True)
-    assert np.isfinite(loss) and loss < 0.9
+    assert np.isfinite(loss) and loss < (0.9 if base == 1 else 1.0)
     w = lib.net_params()
     path = tmp_path / "net.bin"
     lib.net_save(path)
     lib.net_init(9)
     lib.net_load(path)
+    assert lib.net_shape() == (layers, base)
     assert np.array_equal(lib.net_params(), w)
     assert lib.net_scales() == (s_d, s_t)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_learned_start_vs_oracle(seglib, oracle, precision):
+@pytest.mark.parametrize("precision,layers,base", [(0, 4, 1), (1, 4, 1), (0, 1, 3), (1, 2, 3)])
+def test_learned_start_vs_oracle(seglib, oracle, precision, layers, base):
     """SOR started from the network's prediction (random weights with a
     non-zero projection, so the start differs from every other mode): same
-    stopping test, trajectory within the stated error of the FP64 oracle."""
-    print(f"[gpu] learned start precision={precision}", flush=True)
+    stopping test, trajectory within the stated error of the FP64 oracle.
+    The cubic base predicts once three completed steps precede."""
+    print(f"[gpu] learned start precision={precision} layers={layers} base={base}", flush=True)
     lib = seglib
     rng = np.random.default_rng(7)
-    lib.net_init(7)
-    w = random_params(rng, 0.5)
-    off, _ = layout()
+    lib.net_init(7, layers=layers, base=base)
+    w = random_params(rng, 0.5, layers)
+    off, _ = layout(layers)
     o, m = off["wp"]
     w[o:o + m] *= 0.01
     lib.net_set_params(w, 10.0, 100.0)
@@ -265,7 +276,8 @@ def test_learned_start_vs_oracle(seglib, oracle, precision):
         u, prev, coef = u0.copy(), np.zeros(shape), zeros(shape, 8)
         zs = []
         ms0, n0 = lib.net_predict_stats()
-        for st in range(4):
+        steps = 4 if base == 1 else 6
+        for st in range(steps):
             p = params(shape, gsTol=1e-12 * oracle.sumsq_interior(params(shape, **KW), u), **KW)
             oracle.seg_time_step(p, u, prev, g, coef, np.ones(shape))
             z, R = lib.seg_time_step()
@@ -273,7 +285,8 @@ def test_learned_start_vs_oracle(seglib, oracle, precision):
             assert R <= lib.lib.seg4dLastTolerance()
         ms, n = lib.net_predict_stats()
         ms, n = ms - ms0, n - n0
-        assert n == 3  # steps 2..4: the first step has no u^(n-1)
+        # from the step after the base's history exists: 2..4 (linear), 4..6 (cubic)
+        assert n == 3
         lib.download("u")
         ug = lib.arr("u")
         assert np.abs(ug - u).max() <= 1e-4 * np.abs(u).max()
