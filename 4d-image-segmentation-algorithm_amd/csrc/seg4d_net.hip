@@ -93,13 +93,16 @@ __global__ __launch_bounds__(256) void k_gmean(NGeo G, net::bf16* __restrict__ o
     out[v] = (bf16)(float)(s * 0.125);
 }
 
+// (i, j, k, l) of voxel v; 32-bit arithmetic (the grids here hold fewer than
+// 2^31 voxels, checked on the host).
 __device__ __forceinline__ bool point_of(const NGeo& G, long v, int& i, int& j, int& k, int& l) {
-    i = (int)(v % G.I);
-    const long r1 = v / G.I;
-    j = (int)(r1 % G.J);
-    const long r2 = r1 / G.J;
-    k = (int)(r2 % G.K);
-    l = (int)(r2 / G.K);
+    const unsigned uv = (unsigned)v;
+    const unsigned r1 = uv / (unsigned)G.I;
+    i = (int)(uv - r1 * (unsigned)G.I);
+    const unsigned r2 = r1 / (unsigned)G.J;
+    j = (int)(r1 - r2 * (unsigned)G.J);
+    l = (int)(r2 / (unsigned)G.K);
+    k = (int)(r2 - (unsigned)l * (unsigned)G.K);
     return net::interior(G, i, j, k, l);
 }
 
@@ -109,14 +112,6 @@ __device__ __forceinline__ double step_sigma(const double* dsum, double inv_n) {
     return dsum ? sqrt(*dsum * inv_n) : 0.0;
 }
 
-// The extrapolation the network corrects at one point: linear 2u - h1, or
-// cubic 4u - 6h1 + 4h2 - h3 (grouped like the solver's own cubic start).
-__device__ __forceinline__ double base_start(int order, double uc, double p1, double p2,
-                                             double p3) {
-    if (order == 3) return ((4.0 * uc - 6.0 * p1) + 4.0 * p2) - p3;
-    return uc + (uc - p1);
-}
-
 // Third difference u - 3 h1 + 3 h2 - h3 of four consecutive levels: the
 // error the quadratic extrapolation made at the last step, whose rms kappa
 // scales the cubic base's correction.
@@ -124,35 +119,99 @@ __device__ __forceinline__ double third_diff(double uc, double p1, double p2, do
     return ((uc - 3.0 * p1) + 3.0 * p2) - p3;
 }
 
-// Features of the prediction; sums[0] = sum of d^2, sums[1] = sum of the
-// third difference squared over the interior (order 3).
+// Features of the prediction, unscaled: [u^n, d, g_mean, t (order 3), 0 x 4]
+// with d = u^n - u^(n-1) and t the third difference (the network's first
+// layer applies s_d / sigma and s_e / kappa through its weights, see
+// k_scale_w0), and per-block partial sums of d^2 and t^2 over the interior
+// (partial[2b], partial[2b + 1]): one pass over the four levels.
 __global__ __launch_bounds__(256) void k_features(NGeo G, int order, const double* __restrict__ u,
                                                   const double* __restrict__ h1,
                                                   const double* __restrict__ h2,
                                                   const double* __restrict__ h3,
-                                                  const bf16* __restrict__ gm, float sd, float se,
-                                                  const double* __restrict__ sums, double inv_n,
-                                                  bf16* __restrict__ feat) {
+                                                  const bf16* __restrict__ gm,
+                                                  bf16* __restrict__ feat,
+                                                  double* __restrict__ partial) {
+    __shared__ double sm[2][256];
     const long v = (long)blockIdx.x * 256 + threadIdx.x;
-    if (v >= (long)G.I * G.J * G.K * G.L) return;
+    double s = 0.0, s3 = 0.0;
     int i, j, k, l;
-    if (!point_of(G, v, i, j, k, l)) return;
-    const double uc = u[v], p1 = h1[v];
-    const double sig = step_sigma(sums, inv_n);
-    const double inv_sig = sig > 0.0 ? 1.0 / sig : 0.0;
-    bf16x8 f;
-    f[0] = (bf16)(float)uc;
-    f[1] = (bf16)((float)((uc - p1) * inv_sig) * sd);
-    f[2] = gm[v];
-    f[3] = (bf16)0.0f;
-    if (order == 3) {
-        const double kap = step_sigma(sums + 1, inv_n);
-        const double inv_kap = kap > 0.0 ? 1.0 / kap : 0.0;
-        f[3] = (bf16)((float)(third_diff(uc, p1, h2[v], h3[v]) * inv_kap) * se);
-    }
+    if (v < (long)G.I * G.J * G.K * G.L && point_of(G, v, i, j, k, l)) {
+        const double uc = u[v], p1 = h1[v];
+        const double d = uc - p1;
+        bf16x8 f;
+        f[0] = (bf16)(float)uc;
+        f[1] = (bf16)(float)d;
+        f[2] = gm[v];
+        f[3] = (bf16)0.0f;
+        s = d * d;
+        if (order == 3) {
+            const double t = third_diff(uc, p1, h2[v], h3[v]);
+            f[3] = (bf16)(float)t;
+            s3 = t * t;
+        }
 #pragma unroll
-    for (int q = 4; q < 8; q++) f[q] = (bf16)0.0f;
-    *reinterpret_cast<bf16x8*>(feat + v * 8) = f;
+        for (int q = 4; q < 8; q++) f[q] = (bf16)0.0f;
+        *reinterpret_cast<bf16x8*>(feat + v * 8) = f;
+    }
+    sm[0][threadIdx.x] = s;
+    sm[1][threadIdx.x] = s3;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + w];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = sm[0][0];
+        partial[2 * blockIdx.x + 1] = sm[1][0];
+    }
+}
+
+// First level of a fixed-order two-column reduction: out[2c + q] = sum of
+// partial[2b + q] over the chunk b in [c chunk, (c + 1) chunk).
+__global__ __launch_bounds__(256) void k_chunk_sums2(const double* __restrict__ partial, long nb,
+                                                     long chunk, double* __restrict__ out) {
+    __shared__ double sm[2][256];
+    const long b0 = (long)blockIdx.x * chunk, b1 = b0 + chunk < nb ? b0 + chunk : nb;
+    double s = 0.0, s3 = 0.0;
+    for (long b = b0 + threadIdx.x; b < b1; b += 256) {
+        s += partial[2 * b];
+        s3 += partial[2 * b + 1];
+    }
+    sm[0][threadIdx.x] = s;
+    sm[1][threadIdx.x] = s3;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + w];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = sm[0][0];
+        out[2 * blockIdx.x + 1] = sm[1][0];
+    }
+}
+
+// BF16 spatial weights of the first layer with the feature scales of the
+// prediction folded in: w[t][ci][co] * sc(ci), sc(1) = s_d / sigma,
+// sc(3) = s_e / kappa (0 where that rms is 0), 1 otherwise; sigma and kappa
+// from the device sums (sums[0], sums[1]) over n_int interior points.
+__global__ __launch_bounds__(256) void k_scale_w0(const float* __restrict__ w, int n,
+                                                  const double* __restrict__ sums, double inv_n,
+                                                  float sd, float se, bf16* __restrict__ out) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const int ci = (e / kCh) % net::kCin0;
+    float sc = 1.f;
+    if (ci == 1 || ci == 3) {
+        const double r = sqrt(sums[ci == 1 ? 0 : 1] * inv_n);
+        sc = r > 0.0 ? (float)((ci == 1 ? sd : se) / r) : 0.f;
+    }
+    out[e] = (bf16)(w[e] * sc);
 }
 
 // Per-block partial sums (FP64) over the interior of d^2 = (u - h1)^2 and
@@ -391,6 +450,7 @@ void Net::release() {
     fr(m1_);
     fr(m2_);
     fr(grad_);
+    fr(w0s_);
     for (Run& r : runs_) {
         fr(r.gmean);
         for (double*& s : r.snaps) fr(s);
@@ -433,6 +493,8 @@ int Net::init(uint64_t seed, int layers, int base) {
         check(hipMalloc((void**)&m1_, nb * 4), "hipMalloc net") ||
         check(hipMalloc((void**)&m2_, nb * 4), "hipMalloc net") ||
         check(hipMalloc((void**)&grad_, nb * 4), "hipMalloc net") ||
+        check(hipMalloc((void**)&w0s_, (size_t)net::kMaxTaps * net::kCin0 * kCh * 2),
+              "hipMalloc net") ||
         check(hipMalloc((void**)&dscal_, 64 * sizeof(double)), "hipMalloc net") ||
         check(hipHostMalloc((void**)&hscal_, 64 * sizeof(double), hipHostMallocDefault),
               "hipHostMalloc net"))
@@ -509,9 +571,12 @@ int Net::ensure_acts(int I, int J, int K, int L, bool training) {
             al(&lmask_, pts))
             return -1;
     }
-    // partials: weight gradients (2048 blocks x 28 x 16 x 16) or loss rows
+    // partials (floats): weight gradients (2048 blocks x 28 x 16 x 16), loss
+    // rows (18 per block of 256 points), or the prediction's two FP64 sums per
+    // block of 256 points and per chunk of 1024 blocks
     const long nb_loss = cdivl((long)pts, 256);
     npartial_ = std::max<long>(2048L * net::kMaxTaps * 16 * kCh, nb_loss * 18);
+    npartial_ = std::max<long>(npartial_, (nb_loss * 2 + cdivl(nb_loss, 1024) * 2) * 2);
     if (al(&partial_, (size_t)npartial_ * 4)) return -1;
     aI_ = I;
     aJ_ = J;
@@ -534,7 +599,7 @@ static int conv_waves(long nrow, int* rows_per_wave) {
     return (int)cdivl(nrow, *rows_per_wave);
 }
 
-int Net::forward(const NGeo& G, hipStream_t st, bool training) {
+int Net::forward(const NGeo& G, hipStream_t st, bool training, const bf16* w0) {
     const long nrow = (long)(G.J - 2) * (G.K - 2) * ((G.L / G.P) * (G.P - 2));
     int rpw = 1;
     const int waves = conv_waves(nrow, &rpw);
@@ -551,7 +616,7 @@ int Net::forward(const NGeo& G, hipStream_t st, bool training) {
             bf16* Y = Y_[L & 1];
             const bool last = L == nlayers_ - 1;
             const dim3 lg((unsigned)nlb);
-            const bf16 *ws = wb_ + off_ws[L], *wt = wb_ + off_wt[L];
+            const bf16 *ws = (L == 0 && w0) ? w0 : wb_ + off_ws[L], *wt = wb_ + off_wt[L];
             const float* b = wm_ + off_b[L];
             const float *wp = wm_ + off_wp, *bp = wm_ + off_bp;
             if (L == 0 && last)
@@ -641,6 +706,10 @@ int Net::record(Engine& E, bool new_run) {
             err = "net: the volume has no edge-indicator arrays";
             return -1;
         }
+    if (pts >= (1L << 31)) {
+        err = "net: volumes of 2^31 or more padded points are not supported";
+        return -1;
+    }
     if (new_run || runs_.empty()) {
         Run r;
         r.I = E.I_;
@@ -918,20 +987,33 @@ int Net::predict(Engine& E, const double* h1, const double* h2, const double* h3
                                                               g[4], g[5], g[6], g[7]);
         g_version_ = E.g_version_;
     }
+    if (pts >= (1L << 31)) {
+        err = "net: volumes of 2^31 or more padded points are not supported";
+        return -1;
+    }
     const double* u = E.full(A_U);
     const double inv_n = 1.0 / ((double)(E.I_ - 2) * (E.J_ - 2) * (E.K_ - 2) * (E.L_ - 2));
-    // sums of the increment and of the third difference squared, left on the
-    // device for the features and the set-up kernel
-    long per = 0;
-    const int nb = sumsq_blocks(pts, &per);
-    k_pred_moments<<<nb, 256, 0, E.st_>>>(G, u, h1, base_ == 3 ? h2 : nullptr,
-                                          base_ == 3 ? h3 : nullptr, per, (double*)partial_);
-    k_reduce_d<<<1, 256, 0, E.st_>>>((double*)partial_, nb, 2, dscal_ + 8);
-    const double* dsum = dscal_ + 8;
+    // one pass: the features and the block sums of the increment and of the
+    // third difference squared, reduced in two fixed-order levels into
+    // dscal_[8], dscal_[9] (left on the device for the set-up kernel); the
+    // first layer's weights take the feature scales s_d / sigma, s_e / kappa
     const unsigned grid = (unsigned)cdivl(pts, 256);
-    k_features<<<grid, 256, 0, E.st_>>>(G, base_, u, h1, h2, h3, gmean_, s_d, s_e, dsum, inv_n,
-                                        feat_);
-    if (forward(G, E.st_, false)) return -1;
+    const long chunk = 1024;
+    const long nc = cdivl((long)grid, chunk);
+    if ((long)grid * 2 + nc * 2 > npartial_ / 2) {
+        err = "net: prediction partials";
+        return -1;
+    }
+    double* part = (double*)partial_;
+    k_features<<<grid, 256, 0, E.st_>>>(G, base_, u, h1, base_ == 3 ? h2 : nullptr,
+                                        base_ == 3 ? h3 : nullptr, gmean_, feat_, part);
+    k_chunk_sums2<<<(unsigned)nc, 256, 0, E.st_>>>(part, (long)grid, chunk, part + 2 * grid);
+    k_reduce_d<<<1, 256, 0, E.st_>>>(part + 2 * grid, (int)nc, 2, dscal_ + 8);
+    const double* dsum = dscal_ + 8;
+    const int nw0 = net::kMaxTaps * net::kCin0 * kCh;
+    k_scale_w0<<<(unsigned)cdivl(nw0, 256), 256, 0, E.st_>>>(wm_ + off_ws[0], nw0, dsum, inv_n,
+                                                              s_d, s_e, w0s_);
+    if (forward(G, E.st_, false, w0s_)) return -1;
     ls->out = out_;
     ls->dsum = base_ == 3 ? dsum + 1 : dsum;  // the target's scale: kappa or sigma
     ls->inv_n = inv_n;
