@@ -328,9 +328,13 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
     PM.gx = (int)cdiv((long)I_ * J_, kBlock);
     PM.gk = K_;
     PM.gl = L_;
-    PM.tx = 8;
+    // tiles of 32 runs x 8 planes x 1 frame per XCD residency round (4% faster
+    // than 8 x 8 x 4 at 256x256x128x16); planes of fewer than 32 runs keep
+    // 8 x 8 x 4, which leaves fewer idle blocks
+    const bool wide = PM.gx >= 32;
+    PM.tx = wide ? 32 : 8;
     PM.tk = 8;
-    PM.tl = 4;
+    PM.tl = wide ? 1 : 4;
     PM.ntx = (PM.gx + PM.tx - 1) / PM.tx;
     PM.ntk = (PM.gk + PM.tk - 1) / PM.tk;
     PM.ntiles = PM.ntx * PM.ntk * ((PM.gl + PM.tl - 1) / PM.tl);

@@ -76,11 +76,14 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Sum over a 256-thread block; result valid in thread 0.
+// Sum over a 256-thread block; result valid in thread 0.  Every call of a
+// block shares `sm`: the leading barrier keeps a call's writes behind thread
+// 0's reads of the previous call.
 __device__ __forceinline__ double block_sum256(double v) {
     __shared__ double sm[kBlock / 64];
     v = wave_sum(v);
     const int w = threadIdx.x >> 6;
+    __syncthreads();
     if ((threadIdx.x & 63) == 0) sm[w] = v;
     __syncthreads();
     double r = 0.0;
