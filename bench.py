@@ -132,6 +132,8 @@ def main():
     ap.add_argument("--net-out", default=None)
     ap.add_argument("--train-iters", type=int, default=2500)
     ap.add_argument("--train-runs", type=int, default=6)
+    ap.add_argument("--train-dims", type=int, nargs=4, default=None,
+                    help="interior size nx ny nz nt of the training runs (default: warmstart.py's)")
     ap.add_argument("--net-layers", type=int, default=1,
                     help="layers of a network trained with --train-net")
     ap.add_argument("--net-base", type=int, default=3, choices=(1, 3),
@@ -166,9 +168,10 @@ def main():
     training = None
     if args.train_net and not args.no_mixed:
         log("[bench] training the learned warm start ...")
+        kw = {"dims": tuple(args.train_dims)} if args.train_dims else {}
         training = warmstart.train(lib, runs=args.train_runs, iters=args.train_iters,
                                    out=args.net_out or net_path, layers=args.net_layers,
-                                   base=args.net_base, log=log)
+                                   base=args.net_base, log=log, **kw)
         net_path = args.net_out or net_path
         log(f"[bench] trained: {training}")
     spec = lib.gen_spec(args.preset, args.preset)
