@@ -336,6 +336,7 @@ int Batch::step(const StepParams& P) {
         auto defect = [&]() {
             k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, n, 0);
             k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, n, 1);
+            k_resid_chunks_batch<double><<<(unsigned)rgrid_, 1024, 0, st_>>>(dj64_, n);
             k_defect_decide<<<n, 1024, 0, st_>>>(dj64_, max_rounds);
             return sync_ctl();
         };

@@ -131,15 +131,16 @@ static __global__ __launch_bounds__(1024) void k_reduce_batch(const BatchJob<T>*
     }
 }
 
-// FP64 residual of the FP32-refined form and its decision: a volume stops
-// once R <= tol (or R is not a number) or after max_rounds corrections; the
-// control block's z counts the corrections.
+// FP64 residual of the FP32-refined form (over the chunk sums of
+// k_resid_chunks_batch, the single-volume defect's two levels) and its
+// decision: a volume stops once R <= tol (or R is not a number) or after
+// max_rounds corrections; the control block's z counts the corrections.
 static __global__ __launch_bounds__(1024) void k_defect_decide(const BatchJob<double>* __restrict__ J,
                                                         int max_rounds) {
     const BatchJob<double>& B = J[blockIdx.x];
     SorCtl* ctl = B.ctl;
     if (ctl->stop) return;
-    const double R = resid_sum(B.part_r, B.nb, B.part_b, B.nb);
+    const double R = resid_sum(B.red2, B.nc, B.red2 + B.nc, B.nc);
     if (threadIdx.x == 0) {
         ctl->R = R;
         if (!(R > ctl->tol) || ctl->z >= max_rounds) {

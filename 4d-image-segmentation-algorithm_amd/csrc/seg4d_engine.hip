@@ -738,9 +738,14 @@ int Engine::defect64(const StepParams& P, bool copy32, const float* dred) {
     }
     if (launch_check("k_sor_sweep defect")) return -1;
     if (prof.on) (void)hipEventRecord(e1, st_);
-    // z = 1, cap = 1: the reduction always records R and raises stop; the
-    // outer decision is taken on the host from R and tol
-    k_reduce_resid<<<1, 1024, 0, st_>>>(part_r, nb, part_b, nb, ctl_, nullptr, 1, 1, 0);
+    // two fixed-order levels, as in the solve (the chunk sums take no control
+    // block: the stop flag of the previous round is still raised); z = 1,
+    // cap = 1: the reduction always records R and raises stop, the outer
+    // decision is taken on the host from R and tol
+    int rnc = 0, rch = 0;
+    resid_chunking(nb, &rnc, &rch);
+    k_resid_chunks<<<2 * rnc, 1024, 0, st_>>>(part_r, part_b, nb, rnc, rch, red2_, nullptr);
+    k_reduce_resid<<<1, 1024, 0, st_>>>(red2_, rnc, red2_ + rnc, rnc, ctl_, nullptr, 1, 1, 0);
     if (launch_check("k_reduce_resid")) return -1;
     if (check(hipMemcpyAsync(hctl_, ctl_, sizeof(SorCtl), hipMemcpyDeviceToHost, st_), "ctl copy"))
         return -1;

@@ -736,7 +736,7 @@ static __global__ __launch_bounds__(1024) void k_resid_chunks(const double* __re
                                                              int nb, int nc, int ch,
                                                              double* __restrict__ out,
                                                              const SorCtl* __restrict__ ctl) {
-    if (ctl->stop) return;
+    if (ctl != nullptr && ctl->stop) return;  // (no control block: always)
     const double S = chunk_sum(pa, pb, nb, nc, ch, blockIdx.x);
     if (threadIdx.x == 0) out[blockIdx.x] = S;
 }
