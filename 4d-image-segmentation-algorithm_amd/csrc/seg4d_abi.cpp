@@ -698,6 +698,20 @@ API int seg4dNetForward(int I, int J, int K, int L, int P, const uint16_t* feat_
     return rc;
 }
 
+API int seg4dNetPreview(float* c, double* q) {
+    if (!need_net("seg4dNetPreview") || !need_engine("seg4dNetPreview")) return -1;
+    if (!c || !q) {
+        s4d::set_error("seg4dNetPreview: null output");
+        return -1;
+    }
+    g_eng->net_ = g_net;
+    if (g_eng->learned_preview(c, q)) {
+        fail("seg4dNetPreview");
+        return -1;
+    }
+    return 0;
+}
+
 API double seg4dNetPredictMs(void) { return g_net ? g_net->predict_ms : 0.0; }
 API long seg4dNetPredicts(void) { return g_net ? g_net->predicts : 0; }
 API void seg4dNetScales(float* s_d, float* s_t) {

@@ -103,6 +103,11 @@ public:
     int solve(const StepParams& P, SolveStats* st);
     // Colour layout -> u (interior), with the Mmask pass fused if mask.
     int unpack(const double* dmask);
+    // Test hook of the learned start: the network's correction c of the next
+    // solve's start (FP32, full layout, interior points) and its scale q, as
+    // the set-up kernel would receive them; needs the history the network's
+    // base uses.  Leaves the state unchanged.
+    int learned_preview(float* host_c, double* host_q);
     int rescale(double* d, const Balls& balls, bool apply_mask);
 
     // --- initialisation (segmentation4D.c:876-1148) -----------------------------

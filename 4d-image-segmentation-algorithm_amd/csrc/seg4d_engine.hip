@@ -830,6 +830,26 @@ int Engine::solve_mixed(const StepParams& P, SolveStats* out) {
     return 0;
 }
 
+int Engine::learned_preview(float* host_c, double* host_q) {
+    if (!net_ || !net_->ready()) return check(hipErrorInvalidValue, "learned preview: no network");
+    const int nb = net_->base();
+    if (!hist_ok_ || (nb == 3 && !(hist2_ok_ && hist3_ok_) ))
+        return check(hipErrorInvalidValue, "learned preview: the network's history does not exist");
+    LearnedStart ls{nullptr, nullptr, 0.0, 1.f, 1};
+    if (net_->predict(*this, full_[A_PREV], nb == 3 ? prev2_ : nullptr, nb == 3 ? prev3_ : nullptr,
+                      &ls))
+        return check(hipErrorUnknown, net_->err.c_str());
+    double sum = 0.0;
+    if (check(hipMemcpyAsync(host_c, ls.out, (size_t)n_ * sizeof(float), hipMemcpyDeviceToHost, st_),
+              "preview copy") ||
+        check(hipMemcpyAsync(&sum, ls.dsum, sizeof(double), hipMemcpyDeviceToHost, st_),
+              "preview copy") ||
+        check(hipStreamSynchronize(st_), "preview sync"))
+        return -1;
+    *host_q = std::sqrt(sum * ls.inv_n);
+    return 0;
+}
+
 int Engine::unpack(const double* dmask) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     dim3 grid(cdiv((long)(I_ - 2) * (J_ - 2), kBlock), K_ - 2, L_ - 2);

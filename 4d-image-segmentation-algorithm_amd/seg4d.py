@@ -194,6 +194,8 @@ class SegLib(_Lib):
                                                  POINTER(ctypes.c_float),
                                                  POINTER(ctypes.c_ubyte),
                                                  POINTER(ctypes.c_float), POINTER(c_double)]
+        L.seg4dNetPreview.restype = c_int
+        L.seg4dNetPreview.argtypes = [POINTER(ctypes.c_float), POINTER(c_double)]
         L.seg4dNetPredictMs.restype = c_double
         L.seg4dNetPredicts.restype = ctypes.c_long
         L.seg4dDeviceInfo.restype = c_int
@@ -447,6 +449,15 @@ class SegLib(_Lib):
                                  ctypes.byref(loss)) != 0:
             raise RuntimeError(f"seg4dNetGrad failed: {self.last_error()}")
         return g, loss.value
+
+    def net_preview(self, shape):
+        """(c, q): the network's correction of the next start (FP32, padded
+        shape) and its scale, from the current resident state."""
+        c = np.zeros(shape, dtype=np.float32)
+        q = c_double()
+        if self.lib.seg4dNetPreview(c.ctypes.data_as(POINTER(ctypes.c_float)), ctypes.byref(q)) != 0:
+            raise RuntimeError(f"seg4dNetPreview failed: {self.last_error()}")
+        return c, q.value
 
     def net_predict_stats(self):
         return self.lib.seg4dNetPredictMs(), self.lib.seg4dNetPredicts()

@@ -306,6 +306,13 @@ int seg4dNetForward(int I, int J, int K, int L, int P, const uint16_t *feat_bf16
 int seg4dNetGrad(int I, int J, int K, int L, int P, const uint16_t *feat_bf16,
                  const float *target, const unsigned char *lmask, float *grad_out,
                  double *loss);
+/* Test hook: the network's correction c of the next solve's start (FP32,
+ * padded layout, interior points; the caller's array holds one float per
+ * padded point) and its scale q (rms increment for base order 1, rms third
+ * difference for order 3), so that start = base + q c / s_t, from the
+ * device state after completed resident time steps; needs the history the
+ * base uses.  The state is left unchanged. */
+int seg4dNetPreview(float *c, double *q);
 double seg4dNetPredictMs(void);           /* total time of the predictions  */
 long seg4dNetPredicts(void);
 /* Launch-order autotuner of the current volume (no counterpart in the

@@ -300,3 +300,84 @@ def test_learned_start_vs_oracle(seglib, oracle, precision, layers, base):
         lib.set_resident(False)
         lib.set_relative_tolerance(0.0)
         lib.deallocate()
+
+
+def test_prediction_vs_torch(seglib):
+    """The prediction path of the cubic-base network on a solver state: the
+    features (u^n, the increment, mean g, the third difference), their
+    scales (sigma, kappa from device reductions, folded into the first
+    layer's BF16 weights), the fused one-layer network and its projection,
+    against numpy features and the PyTorch reference network with the same
+    BF16 rounding points (5e-3 of max|c|, as the forward test), and the
+    scale kappa against numpy (rel 1e-10)."""
+    print("[gpu] learned prediction vs torch", flush=True)
+    lib = seglib
+    rng = np.random.default_rng(11)
+    layers = 1
+    lib.net_init(3, layers=layers, base=3)
+    w = random_params(rng, 0.5, layers)
+    s_d, s_t = 0.7, 3.0
+    lib.net_set_params(w, s_d, s_t)
+    dims = (37, 30, 18, 6)
+    shape, img, u0 = _volume(lib, dims, 5)
+    lib.allocate(*dims)
+    try:
+        lib.set(gsTol=0.0, **KW)
+        lib.set_relative_tolerance(1e-6)
+        lib.arr("i0")[...] = img
+        lib.arr("u")[...] = 0.0
+        lib.arr("Mmask")[...] = 1.0
+        lib.gf("i0", "u")
+        g = [lib.arr("g_" + f).copy() for f in ("e", "w", "n", "s", "t", "b", "fw", "bw")]
+        lib.set_resident(True)
+        lib.arr("u")[...] = u0
+        lib.upload("u")
+        lib.upload("Mmask")
+        assert lib.set_warm_start(2) == 0
+        levels = [u0.copy()]
+        for _ in range(3):  # the cubic base needs u^(n-3): three completed steps
+            lib.seg_time_step()
+            lib.download("u")
+            levels.append(lib.arr("u").copy())
+        c, q = lib.net_preview(shape)
+    finally:
+        lib.set_warm_start(0)
+        lib.set_resident(False)
+        lib.set_relative_tolerance(0.0)
+        lib.deallocate()
+    u3, u2, u1, u0_ = levels[3], levels[2], levels[1], levels[0]
+    inner = (slice(1, -1),) * 4
+    d = u3 - u2
+    t = ((u3 - 3.0 * u2) + 3.0 * u1) - u0_
+    sig = np.sqrt(np.mean(d[inner] ** 2))
+    kap = np.sqrt(np.mean(t[inner] ** 2))
+    assert q == pytest.approx(kap, rel=1e-10)
+    gm = g[0].copy()
+    for a in g[1:]:
+        gm = gm + a
+    gm = gm * 0.125
+    # unscaled features in BF16, the scales folded into the first layer's weights
+    feat = np.zeros(shape + (8,), dtype=np.float32)
+    feat[..., 0] = u3
+    feat[..., 1] = d
+    feat[..., 2] = gm
+    feat[..., 3] = t
+    ghost = np.ones(shape, dtype=bool)
+    ghost[inner] = False
+    feat[ghost] = 0.0
+    fr = torch.from_numpy(feat).to(torch.bfloat16).to(torch.float32)
+    off, _ = layout(layers)
+    ws = w.copy()
+    o, m = off[("ws", 0)]
+    blk = ws[o:o + m].reshape(28, 8, 16)
+    blk[:, 1, :] = blk[:, 1, :] * np.float32(float(np.float32(s_d)) / sig)  # as k_scale_w0
+    blk[:, 3, :] = blk[:, 3, :] * np.float32(1.0 / kap)  # s_e = 1 after net_init
+    ws[o:o + m] = blk.ravel()
+    ref = reference(torch.from_numpy(ws), fr, shape[0], layers).numpy()[0]
+    got = c[inner]
+    err = np.abs(got - ref).max()
+    print(f"  kappa {q:.3e} (sigma {sig:.3e}), max|c| {np.abs(ref).max():.4f}, max err {err:.2e}",
+          flush=True)
+    assert np.abs(ref).max() > 0
+    assert err <= 5e-3 * np.abs(ref).max()
+    assert not np.any(c[ghost])
