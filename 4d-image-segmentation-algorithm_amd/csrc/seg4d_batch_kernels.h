@@ -87,12 +87,13 @@ static __global__ __launch_bounds__(kBlock) void k_defect_corr_batch(
     int bx, kb;
     if (!map_block(B.M, lb, B.nblocks, bx, kb)) return;
     const BatchJob<float>& D = J32[v];
+    // split operands: the batch's set-up is the fused FP32-refined one
     if (colour == 0)
-        sor_block<double, 5, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[0], B.red[0],
+        sor_block<double, 6, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[0], B.red[0],
                                        B.black, B.opnd[0], B.part_r, B.defect_ob[0],
                                        D.red[D.ctl->zfinal & 1], D.black);
     else
-        sor_block<double, 5, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, B.black,
+        sor_block<double, 6, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, B.black,
                                        B.red[0], B.opnd[1], B.part_b, B.defect_ob[1], D.black,
                                        nullptr);
 }

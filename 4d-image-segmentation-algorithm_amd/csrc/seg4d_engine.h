@@ -229,6 +229,7 @@ public:
     hipEvent_t ev_ = nullptr;
     std::vector<hipEvent_t> pev_; // profiling events (pairs)
     int red_final_ = 0;          // which red buffer holds the accepted state
+    bool split_ = false;         // op_ holds the split operands of the FP32-refined set-up
     // FP32-refined form (allocated on first use)
     float* op32_[2] = {};        // FP32 operand blocks: 8 coefficients + defect
     float* dr_[2] = {};          // correction, red ping-pong

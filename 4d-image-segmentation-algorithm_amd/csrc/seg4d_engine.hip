@@ -242,7 +242,9 @@ int Engine::download_coef(int f, double* host) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     if (!scratch(A_TMP)) return -1;
     dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
-    k_unpack_coef<<<grid, kBlock, 0, st_>>>(G, op_[0], op_[1], f, full_[A_TMP]);
+    k_unpack_coef<<<grid, kBlock, 0, st_>>>(G, op_[0], op_[1], f, full_[A_TMP],
+                                            split_ ? op32_[0] : nullptr,
+                                            split_ ? op32_[1] : nullptr);
     if (launch_check("k_unpack_coef")) return -1;
     return download(A_TMP, host);
 }
@@ -359,6 +361,7 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
         else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm);
     }
     if (launch_check("k_setup_step")) return -1;
+    split_ = mixed;
     if (pout == prev3_ && prev3_) {
         double* o1 = full_[A_PREV];
         double* o2 = prev2_;
@@ -724,14 +727,23 @@ int Engine::defect64(const StepParams& P, bool copy32, const float* dred) {
         (void)hipEventCreate(&e1);
         (void)hipEventRecord(e0, st_);
     }
+    // full FP64 operand blocks for the first defect (RESID 4 / 3); the split
+    // operands of the fused set-up only come with a correction (RESID 6)
+    if ((copy32 || !dred) && split_)
+        return check(hipErrorInvalidValue, "defect: split operands without a correction");
     if (copy32) {
         k_sor_sweep<double, 4, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
         k_sor_sweep<double, 4, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);
     } else if (dred) {
         // red: own u_r + d_r (stored), black neighbours u_b + d_b; black: own
         // u_b + d_b (stored), red neighbours already corrected
-        k_sor_sweep<double, 5, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], ur_[0], ub_, op_[0], part_r, nullptr, op32_[0], dred, db_);
-        k_sor_sweep<double, 5, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[0], op_[1], part_b, nullptr, op32_[1], db_, nullptr);
+        if (split_) {
+            k_sor_sweep<double, 6, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], ur_[0], ub_, op_[0], part_r, nullptr, op32_[0], dred, db_);
+            k_sor_sweep<double, 6, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[0], op_[1], part_b, nullptr, op32_[1], db_, nullptr);
+        } else {
+            k_sor_sweep<double, 5, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], ur_[0], ub_, op_[0], part_r, nullptr, op32_[0], dred, db_);
+            k_sor_sweep<double, 5, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, ub_, ur_[0], op_[1], part_b, nullptr, op32_[1], db_, nullptr);
+        }
     } else {
         k_sor_sweep<double, 3, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 0, ur_[0], nullptr, ub_, op_[0], part_r, nullptr, op32_[0]);
         k_sor_sweep<double, 3, 2><<<grid, kBlock, 0, st_>>>(G, M, c, P.omega, h4, 1, ub_, nullptr, ur_[0], op_[1], part_b, nullptr, op32_[1]);

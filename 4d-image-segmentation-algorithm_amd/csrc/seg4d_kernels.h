@@ -48,6 +48,28 @@ __device__ __forceinline__ long opnd_index(const Geo& G, int k, int l, int t) {
     return plane * G.RP * kOpnd + (long)(t >> 6) * (64 * kOpnd) + (t & 63);
 }
 
+// Split operands of the FP32-refined form (written by its set-up kernel): the
+// FP32 operand blocks hold the HIGH 32-bit word of each FP64 coefficient
+// (components 0..7; the FP32 passes use it as a 20-bit-mantissa coefficient,
+// converted exactly) and the FP32 right-hand side (component 8); the FP64
+// operand buffer holds, per 64-point group, the LOW words of the 8
+// coefficients and the two words of prev as 32-bit words [10][64]
+// (kSplit components, in the same allocation).  High and low word give the
+// FP64 coefficient back exactly, so the FP64 defect reads the same 72 bytes
+// per point as from full FP64 blocks, while the set-up writes 32 bytes per
+// point less.
+constexpr int kSplit = 10;
+
+__device__ __forceinline__ long split_index(const Geo& G, int k, int l, int t) {
+    const long plane = (long)(l - 1) * (G.K - 2) + (k - 1);
+    return plane * G.RP * kSplit + (long)(t >> 6) * (64 * kSplit) + (t & 63);
+}
+
+// FP32 coefficient of a split high word (exact: 20 mantissa bits fit in 23).
+__device__ __forceinline__ float hiword_coef(float bits) {
+    return (float)__hiloint2double(__float_as_int(bits), 0);
+}
+
 struct Ptr8 {
     double* p[8];
 };
@@ -330,9 +352,18 @@ static __global__ __launch_bounds__(kBlock) void k_setup_step(
             sb.ublack[pp] = start;
         }
         if (is_interior(G, i, j, k, l)) {
-            const long oi = opnd_index(G, k, l, (j - 1) * G.H + (i >> 1));
+            const int t2 = (j - 1) * G.H + (i >> 1);
+            const long oi = opnd_index(G, k, l, t2);
             double* ob = (c ? sb.opnd[1] : sb.opnd[0]) + oi;
-            ob[8 * 64] = uc;
+            // MIXED: split words (split_index) instead of full FP64 blocks
+            unsigned* ow = reinterpret_cast<unsigned*>(c ? sb.opnd[1] : sb.opnd[0]) +
+                           split_index(G, k, l, t2);
+            if constexpr (MIXED) {
+                ow[8 * 64] = (unsigned)__double2hiint(uc);
+                ow[9 * 64] = (unsigned)__double2loint(uc);
+            } else {
+                ob[8 * 64] = uc;
+            }
             Stencil33 T;
             load_stencil33(u, p, G, T);
             double gl[8];
@@ -353,13 +384,14 @@ static __global__ __launch_bounds__(kBlock) void k_setup_step(
             for (int f = 0; f < 8; f++) {
                 if constexpr (WITH_G) cf[f] = vc * gl[f] * (1.0 / N[f]);
                 else cf[f] = vc * (1.0 / N[f]);
-                ob[f * 64] = cf[f];
+                if constexpr (MIXED) ow[f * 64] = (unsigned)__double2loint(cf[f]);
+                else ob[f * 64] = cf[f];
             }
             acc = uc * uc * h4;
             if constexpr (MIXED) {
                 float* ob32 = (c ? sb.opnd32[1] : sb.opnd32[0]) + oi;
 #pragma unroll
-                for (int f = 0; f < 8; f++) ob32[f * 64] = (float)cf[f];
+                for (int f = 0; f < 8; f++) ob32[f * 64] = __int_as_float(__double2hiint(cf[f]));
                 // numerator order e, w, s, n, b, t, bw, fw; diagonal e, w, n, s, t, b, fw, bw
                 double num = cf[0] * T.s[1][1][1][2];
                 num = num + cf[1] * T.s[1][1][1][0];
@@ -486,12 +518,17 @@ __device__ __forceinline__ T other(const T* __restrict__ y, const float* __restr
     else return y[q];
 }
 
-template <typename T, bool NTO = false, bool CORR = false>
+// SPLIT (FP64 only): the coefficients from the split operands, high words in
+// hi (the FP32 operand block of the point) and low words and prev in ow.
+// FP32: the operand block's components 0..7 are split high words.
+template <typename T, bool NTO = false, bool CORR = false, bool SPLIT = false>
 __device__ __forceinline__ void load_point(const Geo& G, const T* __restrict__ ob,
                                            const T* __restrict__ y, const T* xin, int pp,
                                            int sft, bool act, PointIn<T>& P,
                                            const float* __restrict__ dx = nullptr,
-                                           const float* __restrict__ dy = nullptr) {
+                                           const float* __restrict__ dy = nullptr,
+                                           const float* __restrict__ hi = nullptr,
+                                           const unsigned* __restrict__ ow = nullptr) {
     P.pp = pp;
     P.sft = sft;
     P.act = act;
@@ -501,7 +538,12 @@ __device__ __forceinline__ void load_point(const Geo& G, const T* __restrict__ o
         P.yN = other<T, CORR>(y, dy, pp + G.py);
         P.yB = other<T, CORR>(y, dy, pp - G.pz);
         P.yT = other<T, CORR>(y, dy, pp + G.pz);
-        if constexpr (NTO) {
+        if constexpr (SPLIT) {
+#pragma unroll
+            for (int f = 0; f < 8; f++)
+                P.c[f] = __hiloint2double(__float_as_int(hi[f * 64]), (int)ow[f * 64]);
+            P.pv = __hiloint2double((int)ow[8 * 64], (int)ow[9 * 64]);
+        } else if constexpr (NTO) {
 #pragma unroll
             for (int f = 0; f < 8; f++) P.c[f] = __builtin_nontemporal_load(ob + f * 64);
             P.pv = __builtin_nontemporal_load(ob + 8 * 64);
@@ -509,6 +551,10 @@ __device__ __forceinline__ void load_point(const Geo& G, const T* __restrict__ o
 #pragma unroll
             for (int f = 0; f < 8; f++) P.c[f] = ob[f * 64];
             P.pv = ob[8 * 64];
+        }
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int f = 0; f < 8; f++) P.c[f] = (T)hiword_coef((float)P.c[f]);
         }
         if constexpr (CORR) P.uo = xin[pp] + (T)dx[pp];
         else P.uo = xin[pp];
@@ -519,9 +565,10 @@ __device__ __forceinline__ void load_point(const Geo& G, const T* __restrict__ o
 // residual term r = u (1 + c sum) - c sum coef u_nb - prev (segmentation4D.c:
 // 290-307) before (RESID 1) or after (RESID 2) the update.  RESID 3 does not
 // update: it writes -r (the defect, FP32) to *defect and accumulates r^2 h^4;
-// RESID 4 also writes the coefficients, rounded to FP32, 8..1 components
-// before *defect; RESID 5 is RESID 3 on the corrected state u + d, whose own
-// value it stores (the defect-correction update fused into the next defect).
+// RESID 4 also writes the high words of the coefficients (split operands),
+// 8..1 components before *defect; RESID 5 is RESID 3 on the corrected state
+// u + d, whose own value it stores (the defect-correction update fused into
+// the next defect); RESID 6 is RESID 5 on split operands.
 template <typename T, int RESID>
 __device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart, T yfw, T c,
                                              T om, double h4, T* xout, float* defect,
@@ -553,11 +600,12 @@ __device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart
         acc += rd * rd * h4;
         if constexpr (RESID >= 3) {
             *defect = (float)(-r);
-            if constexpr (RESID == 4) {  // FP32 copies of the 8 coefficients
+            if constexpr (RESID == 4) {  // high words of the 8 coefficients
 #pragma unroll
-                for (int f = 0; f < 8; f++) defect[(f - 8) * 64] = (float)P.c[f];
+                for (int f = 0; f < 8; f++)
+                    defect[(f - 8) * 64] = __int_as_float(__double2hiint((double)P.c[f]));
             }
-            if constexpr (RESID == 5) xout[P.pp] = P.uo;
+            if constexpr (RESID >= 5) xout[P.pp] = P.uo;
             return;
         }
     }
@@ -585,7 +633,10 @@ __device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart
 // neighbours are final for z).  segmentation4D.c:287-308.  3: residual only,
 // the defect -r written as FP32 into component 8 of the operand blocks
 // `defect_ob` (the right-hand side of the FP32 correction equation); 4: as 3,
-// and the FP32 copies of the 8 coefficients into components 0..7.
+// and the high words of the 8 coefficients into components 0..7; 5: as 3 on
+// u + d (own correction dx, the other colour's dy), storing u + d; 6: as 5
+// with the coefficients and prev from the split operands (high words in
+// `defect_ob`, low words and prev in `opnd` as 32-bit words).
 // ----------------------------------------------------------------------------
 // The work of one block (run bx, k group / l chunk kb), shared by the
 // single-volume and the batched kernels; every thread of the block calls it.
@@ -596,7 +647,8 @@ __device__ __forceinline__ void sor_block(
     const T* __restrict__ opnd, double* __restrict__ partials,
     float* __restrict__ defect_ob, const float* __restrict__ dx = nullptr,
     const float* __restrict__ dy = nullptr) {
-    constexpr bool CORR = RESID == 5;  // dx: own correction, dy: the other colour's (or none)
+    constexpr bool CORR = RESID >= 5;  // dx: own correction, dy: the other colour's (or none)
+    constexpr bool SPLIT = RESID == 6;
     const int lb = (kb - 1) * M.gx + bx;
     // block index kb - 1 = l chunk * gk0 + k group; the block's kp planes:
     // threads [sub * per, (sub + 1) * per) take plane kgrp kp + sub + 1, run bx
@@ -620,6 +672,12 @@ __device__ __forceinline__ void sor_block(
     const T* ob0 = opnd + ob_off;
     const long ostride = (long)(G.K - 2) * G.RP * kOpnd;
     float* def0 = (RESID >= 3) ? defect_ob + ob_off + 8 * 64 : nullptr;
+    // split operands: high words in the FP32 block (same index), low words and
+    // prev in the FP64 buffer's [10][64] word groups, sstride words per plane
+    const float* hi0 = SPLIT ? defect_ob + ob_off : nullptr;
+    const unsigned* ow0 =
+        SPLIT ? reinterpret_cast<const unsigned*>(opnd) + split_index(G, k, 1, t) : nullptr;
+    const long sstride = (long)(G.K - 2) * G.RP * kSplit;
     // window w[q] = other colour at column m, row l-1+q
     T w[U + 2];
     w[0] = rowok ? other<T, CORR>(y, dy, base + (l0 - 1) * G.pw) : T(0);
@@ -635,8 +693,10 @@ __device__ __forceinline__ void sor_block(
             w[q + 2] = rowok ? other<T, CORR>(y, dy, pp + G.pw) : T(0);
             const int sft = colour ^ ((j + k + lq) & 1);
             const int i = 2 * m + sft;
-            load_point<T, NTO, CORR>(G, ob0 + (lq - 1) * ostride, y, xin, pp, sft,
-                                     rowok && i >= 1 && i <= G.I - 2, P[q], dx, dy);
+            load_point<T, NTO, CORR, SPLIT>(G, ob0 + (lq - 1) * ostride, y, xin, pp, sft,
+                                            rowok && i >= 1 && i <= G.I - 2, P[q], dx, dy,
+                                            SPLIT ? hi0 + (lq - 1) * ostride : nullptr,
+                                            SPLIT ? ow0 + (lq - 1) * sstride : nullptr);
         }
 #pragma unroll
         for (int q = 0; q < U; q++)
@@ -651,8 +711,10 @@ __device__ __forceinline__ void sor_block(
         w[2] = rowok ? other<T, CORR>(y, dy, pp + G.pw) : T(0);
         const int sft = colour ^ ((j + k + l) & 1);
         const int i = 2 * m + sft;
-        load_point<T, NTO, CORR>(G, ob0 + (l - 1) * ostride, y, xin, pp, sft,
-                                 rowok && i >= 1 && i <= G.I - 2, P, dx, dy);
+        load_point<T, NTO, CORR, SPLIT>(G, ob0 + (l - 1) * ostride, y, xin, pp, sft,
+                                        rowok && i >= 1 && i <= G.I - 2, P, dx, dy,
+                                        SPLIT ? hi0 + (l - 1) * ostride : nullptr,
+                                        SPLIT ? ow0 + (l - 1) * sstride : nullptr);
         update_point<T, RESID>(P, w[0], w[1], w[2], c, om, h4, xout,
                                def0 ? def0 + (l - 1) * ostride : nullptr, acc);
         w[0] = w[1];
@@ -1081,9 +1143,13 @@ static __global__ __launch_bounds__(kBlock) void k_pack_ghosts(Geo G, const doub
 // Colour layout -> full layout for one coefficient face (interior only),
 // used when the host asks for the coefficient arrays.
 // Component f (0..7 coefficient, 8 prev) of the operand blocks -> full layout.
+// With split operands (hr, hb non-null: the FP32 blocks with the high words)
+// the value is put together from its high and low words.
 static __global__ __launch_bounds__(kBlock) void k_unpack_coef(Geo G, const double* __restrict__ orr,
                                                        const double* __restrict__ obl, int f,
-                                                       double* __restrict__ out) {
+                                                       double* __restrict__ out,
+                                                       const float* __restrict__ hr = nullptr,
+                                                       const float* __restrict__ hb = nullptr) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
     const int k = blockIdx.y, l = blockIdx.z;
     if (t >= G.I * G.J) return;
@@ -1091,8 +1157,21 @@ static __global__ __launch_bounds__(kBlock) void k_unpack_coef(Geo G, const doub
     const int p = ((l * G.K + k) * G.J + j) * G.I + i;
     double v = 0.0;
     if (is_interior(G, i, j, k, l)) {
-        const long q = opnd_index(G, k, l, (j - 1) * G.H + (i >> 1)) + f * 64;
-        v = ((i + j + k + l) & 1) ? obl[q] : orr[q];
+        const int t2 = (j - 1) * G.H + (i >> 1);
+        const bool blk = (i + j + k + l) & 1;
+        if (hr != nullptr) {
+            const unsigned* ow = reinterpret_cast<const unsigned*>(blk ? obl : orr) +
+                                 split_index(G, k, l, t2);
+            if (f < 8) {
+                const float* h = (blk ? hb : hr) + opnd_index(G, k, l, t2) + f * 64;
+                v = __hiloint2double(__float_as_int(*h), (int)ow[f * 64]);
+            } else {
+                v = __hiloint2double((int)ow[8 * 64], (int)ow[9 * 64]);
+            }
+        } else {
+            const long q = opnd_index(G, k, l, t2) + f * 64;
+            v = blk ? obl[q] : orr[q];
+        }
     }
     out[p] = v;
 }
