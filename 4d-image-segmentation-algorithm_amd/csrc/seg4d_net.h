@@ -121,7 +121,7 @@ public:
     float* m1_ = nullptr;
     float* m2_ = nullptr;
     float* grad_ = nullptr;
-    __bf16* w0s_ = nullptr; // first-layer spatial weights with the feature scales
+    __bf16* w0s_ = nullptr; // first-layer spatial weights (4 channels) with the feature scales
     long adam_t_ = 0;
 
 private:
@@ -130,8 +130,9 @@ private:
     int check(hipError_t e, const char* what);
     int layout();
     int ensure_acts(int I, int J, int K, int L, bool training);
-    // w0: spatial weights of the first layer to use instead of wb_'s
-    // (inference only; the prediction's scaled copy)
+    // w0: spatial weights of the first layer for 4 input channels, used
+    // instead of wb_'s 8-channel ones (inference only: the prediction's
+    // scaled copy, with feat_ holding 4 channels per point)
     int forward(const net::NGeo& G, hipStream_t st, bool training, const __bf16* w0 = nullptr);
     int backward(const net::NGeo& G, hipStream_t st, float inv_n);
     int compute_scales(hipStream_t st);

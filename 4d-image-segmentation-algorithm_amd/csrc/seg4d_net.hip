@@ -119,11 +119,12 @@ __device__ __forceinline__ double third_diff(double uc, double p1, double p2, do
     return ((uc - 3.0 * p1) + 3.0 * p2) - p3;
 }
 
-// Features of the prediction, unscaled: [u^n, d, g_mean, t (order 3), 0 x 4]
-// with d = u^n - u^(n-1) and t the third difference (the network's first
-// layer applies s_d / sigma and s_e / kappa through its weights, see
-// k_scale_w0), and per-block partial sums of d^2 and t^2 over the interior
-// (partial[2b], partial[2b + 1]): one pass over the four levels.
+// Features of the prediction, unscaled: [u^n, d, g_mean, t (order 3)] as 4
+// BF16 channels per point (every point: ghosts zero), with d = u^n - u^(n-1)
+// and t the third difference (the network's first layer applies s_d / sigma
+// and s_e / kappa through its weights, see k_scale_w0), and per-block partial
+// sums of d^2 and t^2 over the interior (partial[2b], partial[2b + 1]): one
+// pass over the four levels.
 __global__ __launch_bounds__(256) void k_features(NGeo G, int order, const double* __restrict__ u,
                                                   const double* __restrict__ h1,
                                                   const double* __restrict__ h2,
@@ -135,23 +136,24 @@ __global__ __launch_bounds__(256) void k_features(NGeo G, int order, const doubl
     const long v = (long)blockIdx.x * 256 + threadIdx.x;
     double s = 0.0, s3 = 0.0;
     int i, j, k, l;
-    if (v < (long)G.I * G.J * G.K * G.L && point_of(G, v, i, j, k, l)) {
-        const double uc = u[v], p1 = h1[v];
-        const double d = uc - p1;
-        bf16x8 f;
-        f[0] = (bf16)(float)uc;
-        f[1] = (bf16)(float)d;
-        f[2] = gm[v];
-        f[3] = (bf16)0.0f;
-        s = d * d;
-        if (order == 3) {
-            const double t = third_diff(uc, p1, h2[v], h3[v]);
-            f[3] = (bf16)(float)t;
-            s3 = t * t;
-        }
+    if (v < (long)G.I * G.J * G.K * G.L) {
+        net::bf16x4 f;
 #pragma unroll
-        for (int q = 4; q < 8; q++) f[q] = (bf16)0.0f;
-        *reinterpret_cast<bf16x8*>(feat + v * 8) = f;
+        for (int q = 0; q < 4; q++) f[q] = (bf16)0.0f;
+        if (point_of(G, v, i, j, k, l)) {
+            const double uc = u[v], p1 = h1[v];
+            const double d = uc - p1;
+            f[0] = (bf16)(float)uc;
+            f[1] = (bf16)(float)d;
+            f[2] = gm[v];
+            s = d * d;
+            if (order == 3) {
+                const double t = third_diff(uc, p1, h2[v], h3[v]);
+                f[3] = (bf16)(float)t;
+                s3 = t * t;
+            }
+        }
+        *reinterpret_cast<net::bf16x4*>(feat + v * 4) = f;
     }
     sm[0][threadIdx.x] = s;
     sm[1][threadIdx.x] = s3;
@@ -196,22 +198,23 @@ __global__ __launch_bounds__(256) void k_chunk_sums2(const double* __restrict__ 
     }
 }
 
-// BF16 spatial weights of the first layer with the feature scales of the
-// prediction folded in: w[t][ci][co] * sc(ci), sc(1) = s_d / sigma,
-// sc(3) = s_e / kappa (0 where that rms is 0), 1 otherwise; sigma and kappa
-// from the device sums (sums[0], sums[1]) over n_int interior points.
+// BF16 spatial weights of the first layer for the prediction's 4 input
+// channels, the feature scales folded in: out[t][ci][co] = w[t][ci][co] sc(ci)
+// (ci < 4 of the 8 stored), sc(1) = s_d / sigma, sc(3) = s_e / kappa (0 where
+// that rms is 0), 1 otherwise; sigma and kappa from the device sums
+// (sums[0], sums[1]) over the interior points (inv_n = 1 / their count).
 __global__ __launch_bounds__(256) void k_scale_w0(const float* __restrict__ w, int n,
                                                   const double* __restrict__ sums, double inv_n,
                                                   float sd, float se, bf16* __restrict__ out) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= n) return;
-    const int ci = (e / kCh) % net::kCin0;
+    const int co = e % kCh, ci = (e / kCh) % 4, t = e / (kCh * 4);
     float sc = 1.f;
     if (ci == 1 || ci == 3) {
         const double r = sqrt(sums[ci == 1 ? 0 : 1] * inv_n);
         sc = r > 0.0 ? (float)((ci == 1 ? sd : se) / r) : 0.f;
     }
-    out[e] = (bf16)(w[e] * sc);
+    out[e] = (bf16)(w[(t * net::kCin0 + ci) * kCh + co] * sc);
 }
 
 // Per-block partial sums (FP64) over the interior of d^2 = (u - h1)^2 and
@@ -619,7 +622,11 @@ int Net::forward(const NGeo& G, hipStream_t st, bool training, const bf16* w0) {
             const bf16 *ws = (L == 0 && w0) ? w0 : wb_ + off_ws[L], *wt = wb_ + off_wt[L];
             const float* b = wm_ + off_b[L];
             const float *wp = wm_ + off_wp, *bp = wm_ + off_bp;
-            if (L == 0 && last)
+            if (L == 0 && w0 && last)  // the prediction's 4-channel features
+                net::k_layer_fwd<4, true><<<lg, 256, 0, st>>>(G, X, ws, wt, b, Y, wp, bp, out_);
+            else if (L == 0 && w0)
+                net::k_layer_fwd<4, false><<<lg, 256, 0, st>>>(G, X, ws, wt, b, Y, wp, bp, out_);
+            else if (L == 0 && last)
                 net::k_layer_fwd<8, true><<<lg, 256, 0, st>>>(G, X, ws, wt, b, Y, wp, bp, out_);
             else if (L == 0)
                 net::k_layer_fwd<8, false><<<lg, 256, 0, st>>>(G, X, ws, wt, b, Y, wp, bp, out_);
@@ -1010,7 +1017,7 @@ int Net::predict(Engine& E, const double* h1, const double* h2, const double* h3
     k_chunk_sums2<<<(unsigned)nc, 256, 0, E.st_>>>(part, (long)grid, chunk, part + 2 * grid);
     k_reduce_d<<<1, 256, 0, E.st_>>>(part + 2 * grid, (int)nc, 2, dscal_ + 8);
     const double* dsum = dscal_ + 8;
-    const int nw0 = net::kMaxTaps * net::kCin0 * kCh;
+    const int nw0 = net::kMaxTaps * 4 * kCh;
     k_scale_w0<<<(unsigned)cdivl(nw0, 256), 256, 0, E.st_>>>(wm_ + off_ws[0], nw0, dsum, inv_n,
                                                               s_d, s_e, w0s_);
     if (forward(G, E.st_, false, w0s_)) return -1;

@@ -29,6 +29,7 @@ namespace net {
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kCh = 16;       // hidden channels
@@ -308,14 +309,19 @@ __global__ __launch_bounds__(256) void k_layer_fwd(NGeo G, const bf16* __restric
                                                     const float* __restrict__ wp,
                                                     const float* __restrict__ bp,
                                                     float* __restrict__ out) {
+    // CIN = 4 (the prediction's unscaled features): a point is one 8-byte
+    // bf16x4; a lane's 8 K values are 2 taps x 4 channels, two 8-byte reads
+    static_assert(CIN == 4 || CIN % 8 == 0, "input channels: 4 or a multiple of 8");
+    constexpr bool C4 = CIN == 4;
     constexpr int NT = 27;
     constexpr int TPS = 32 / CIN;
     constexpr int NS = (NT + TPS - 1) / TPS;
-    constexpr int C8 = CIN / 8;
+    constexpr int C8 = C4 ? 1 : CIN / 8;
     constexpr int NP = kSpZ * kSpY * kSpX;
-    constexpr int NE = NP * C8;
-    constexpr int PER = (NE + 255) / 256;        // patch chunks per thread
-    __shared__ bf16x8 sp[NE];
+    constexpr int NE = C4 ? NP : NP * C8;        // staged elements (C4: 8-byte points)
+    constexpr int PER = (NE + 255) / 256;        // patch elements per thread
+    __shared__ bf16x8 sp[C4 ? (NP + 1) / 2 : NE];
+    bf16x4* sp4 = reinterpret_cast<bf16x4*>(sp);
     __shared__ bf16x8 ring[3][4][4][16][2];      // [t % 3][wave][row tile][point][8 channels]
     const int ln = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = ln >> 4, m = ln & 15;
@@ -361,14 +367,17 @@ __global__ __launch_bounds__(256) void k_layer_fwd(NGeo G, const bf16* __restric
         wpr[r] = PROJ ? wp[4 * g + r] : 0.f;
     }
     const float bpo = PROJ ? bp[0] : 0.f;
-    const int tap_in_step = (8 * g) / CIN, chunk = ((8 * g) % CIN) / 8;
-    int toff[NS];
-    bool tok[NS];
+    const int tap_in_step = (8 * g) / CIN, chunk = C4 ? 0 : ((8 * g) % CIN) / 8;
+    auto tap_off = [](int t) { return ((t / 9 - 1) * kSpY + (t / 3 % 3 - 1)) * kSpX + (t % 3 - 1); };
+    int toff[NS], toff2[NS];
+    bool tok[NS], tok2[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         const int t = s * TPS + tap_in_step;
-        toff[s] = ((t / 9 - 1) * kSpY + (t / 3 % 3 - 1)) * kSpX + (t % 3 - 1);
+        toff[s] = tap_off(t);
         tok[s] = t < NT;
+        toff2[s] = tap_off(t + 1);  // C4: the lane's second tap
+        tok2[s] = C4 && t + 1 < NT;
     }
     const int base = ((wv + 1) * kSpY + 1) * kSpX + (m + 1);
     bf16x8 zero8;
@@ -379,11 +388,15 @@ __global__ __launch_bounds__(256) void k_layer_fwd(NGeo G, const bf16* __restric
 #pragma unroll
         for (int jj = 0; jj < 4; jj++) ring[0][wv][jj][m][g] = zero8;
     }
-    auto fetch = [&](int l, bf16x8* pre) {
+    bf16x4 zero4;
+#pragma unroll
+    for (int q = 0; q < 4; q++) zero4[q] = (bf16)0.0f;
+    auto fetch = [&](int l, bf16x8* pre, bf16x4* pre4) {
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int e = threadIdx.x + 256 * q;
             bf16x8 v = zero8;
+            bf16x4 v4 = zero4;
             if (e < NE) {
                 const int c = e % C8;
                 int r = e / C8;
@@ -391,15 +404,21 @@ __global__ __launch_bounds__(256) void k_layer_fwd(NGeo G, const bf16* __restric
                 r /= kSpX;
                 const int y = r % kSpY, z = r / kSpY;
                 const int i = i0 - 1 + x, j = j0 - 1 + y, k = k0 - 1 + z;
-                if (i <= G.I - 1 && j <= G.J - 1 && k <= G.K - 1)
-                    v = *reinterpret_cast<const bf16x8*>(X + nvox(G, i, j, k, lbase + l) * CIN +
-                                                         8 * c);
+                if (i <= G.I - 1 && j <= G.J - 1 && k <= G.K - 1) {
+                    if constexpr (C4)
+                        v4 = *reinterpret_cast<const bf16x4*>(X + nvox(G, i, j, k, lbase + l) * 4);
+                    else
+                        v = *reinterpret_cast<const bf16x8*>(X + nvox(G, i, j, k, lbase + l) * CIN +
+                                                             8 * c);
+                }
             }
-            pre[q] = v;
+            if constexpr (C4) pre4[q] = v4;
+            else pre[q] = v;
         }
     };
-    bf16x8 pre[PER];
-    fetch(1, pre);
+    bf16x8 pre[C4 ? 1 : PER];
+    bf16x4 pre4[C4 ? PER : 1];
+    fetch(1, pre, pre4);
     const int i = i0 + m, k = k0 + wv;
     const bool outok = i <= G.I - 2 && k <= G.K - 2;
     for (int l = 1; l <= G.P - 1; l++) {
@@ -409,10 +428,13 @@ __global__ __launch_bounds__(256) void k_layer_fwd(NGeo G, const bf16* __restric
 #pragma unroll
             for (int q = 0; q < PER; q++) {
                 const int e = threadIdx.x + 256 * q;
-                if (e < NE) sp[e] = pre[q];
+                if (e < NE) {
+                    if constexpr (C4) sp4[e] = pre4[q];
+                    else sp[e] = pre[q];
+                }
             }
         }
-        if (l + 1 <= G.P - 2) fetch(l + 1, pre);
+        if (l + 1 <= G.P - 2) fetch(l + 1, pre, pre4);
         __syncthreads();
         f32x4 acc[4];
 #pragma unroll
@@ -422,7 +444,15 @@ __global__ __launch_bounds__(256) void k_layer_fwd(NGeo G, const bf16* __restric
             for (int s = 0; s < NS; s++) {
 #pragma unroll
                 for (int jj = 0; jj < 4; jj++) {
-                    const bf16x8 bv = tok[s] ? sp[(base + jj * kSpX + toff[s]) * C8 + chunk] : zero8;
+                    bf16x8 bv;
+                    if constexpr (C4) {
+                        const int pt = base + jj * kSpX;
+                        const bf16x4 lo = tok[s] ? sp4[pt + toff[s]] : zero4;
+                        const bf16x4 hi = tok2[s] ? sp4[pt + toff2[s]] : zero4;
+                        bv = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+                    } else {
+                        bv = tok[s] ? sp[(base + jj * kSpX + toff[s]) * C8 + chunk] : zero8;
+                    }
                     acc[jj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ws[s], bv, acc[jj], 0, 0, 0);
                 }
             }
