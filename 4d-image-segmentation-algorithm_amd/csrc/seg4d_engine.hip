@@ -709,7 +709,8 @@ int Engine::ensure_mixed() {
 // FP64 residual of the reference system for the current colour-layout state
 // (red in ur_[0], black in ub_); writes the FP32 defect into the FP32 operand
 // blocks and the residual into ctl_->R (host copy in hctl_).
-// copy32: also write the FP32 coefficient copies (first defect of a step);
+// copy32: also write the coefficients' high words into the FP32 blocks (first
+// defect of a step whose set-up wrote full FP64 blocks);
 // dred != null: first apply the correction (red: dred, black: db_), i.e.
 // u += d fused into the defect passes.
 int Engine::defect64(const StepParams& P, bool copy32, const float* dred) {

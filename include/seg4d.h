@@ -218,7 +218,7 @@ int seg4dSetPipelineDepth(int depth);
 /* Arithmetic of the SOR solve.  0 (default): FP64, the reference's
  * arithmetic and bits.  1: FP32-refined -- FP64 defect correction in which
  * the defect b - A u of the FP64 system is recomputed in FP64 on the GPU and
- * an FP32 red-black SOR solve of A d = b - A u (coefficients rounded to FP32)
+ * an FP32 red-black SOR solve of A d = b - A u (coefficients to 20 bits)
  * is applied; the stopping test stays the reference's on the FP64 residual
  * (R <= gsTol, or the relative bound), and maxNoGSIteration caps each FP32
  * solve.  Returns the previous mode, -1 before _allocateMem_. */
