@@ -382,7 +382,10 @@ int Batch::step(const StepParams& P) {
     if (!rc) rc = check(hipEventElapsedTime(&ms, e0, e1), "event");
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (rc) return -1;
+    if (rc) {
+        for (auto& e : E_) e->undo_rotation();
+        return -1;
+    }
     last_step_ms = ms;
     for (auto& e : E_) e->step_done();
     return 0;

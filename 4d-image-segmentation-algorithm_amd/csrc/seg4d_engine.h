@@ -147,6 +147,9 @@ public:
     int adopt_stream(hipStream_t s);
     // Completed time step: the history levels the setup filled become valid.
     void step_done();
+    // puts the buffer pointers back as they were before setup() rotated
+    // them (a step that failed after its set-up leaves u = u^n)
+    void undo_rotation();
     struct Geo geometry() const;
     // Times the red pass (FP64, no residual) over tile shapes of the tiled
     // order and the linear order, `reps` launches each, and keeps the
@@ -183,6 +186,8 @@ public:
     bool hist_ok_ = false;
     bool hist2_ok_ = false, hist3_ok_ = false;
     bool shifted2_ = false, shifted3_ = false;  // this step's setup filled prev2_ / prev3_
+    bool rotated_ = false;                       // setup() rotated; step_done() not yet called
+    double* rot_[4] = {};                        // u, prev_full, prev2_, prev3_ before it
     double* prev2_ = nullptr;      // u^(n-2) for the quadratic and cubic starts
     double* prev3_ = nullptr;      // u^(n-3) for the cubic start
     Net* net_ = nullptr;           // not owned

@@ -182,7 +182,17 @@ int Engine::adopt_stream(hipStream_t s) {
     return 0;
 }
 
+void Engine::undo_rotation() {
+    if (!rotated_) return;
+    full_[A_U] = rot_[0];
+    full_[A_PREV] = rot_[1];
+    prev2_ = rot_[2];
+    prev3_ = rot_[3];
+    rotated_ = false;
+}
+
 void Engine::step_done() {
+    rotated_ = false;
     hist3_ok_ = shifted3_;  // prev3 = u^(n-2)
     hist2_ok_ = shifted2_;  // prev2 = u^(n-1)
     hist_ok_ = true;        // prev_full = u^n and u = u^(n+1)
@@ -273,11 +283,7 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
     sb.dblack = mixed ? db_ : nullptr;
     const double csor = P.tau / (P.h * P.h);
     double* part2 = part_ + npart_;
-    double* prevf = write_prev ? full_[A_PREV] : nullptr;
-    if (!write_prev) {
-        prevf = scratch(A_TMP2);
-        if (!prevf) return -1;
-    }
+    double* prevf = write_prev ? full_[A_PREV] : nullptr;  // u^(n-1) (read by warm starts)
     // the extrapolated and the learned start need prev_full = u^(n-1) from a
     // completed step; without it the solve starts from u^n.  The quadratic
     // and cubic starts need one / two more levels (prev2_, prev3_); with
@@ -293,12 +299,11 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
     if (keep2 && hist_ok_ && hist2_ok_) warm = 3;
     if (warm_ == 4 && keep3 && hist_ok_ && hist2_ok_ && hist3_ok_) warm = 4;
     // History ring: prev_full (u^(n-1)), prev2_ (u^(n-2)), prev3_ (u^(n-3)).
-    // u^n is written over the oldest level the mode keeps, and the pointers
-    // rotate after the launch (prev_full = u^n, prev2_ = u^(n-1), ...), so no
-    // level is copied.
+    // No level is copied: after the launch the buffers rotate (prev_full =
+    // u's buffer holding u^n, prev2_ = u^(n-1), ...) and u's buffer becomes
+    // the oldest level the mode keeps, which k_unpack fills with u^(n+1).
     const double* h2 = nullptr;
     const double* h3 = nullptr;
-    double* pout = prevf;
     shifted2_ = shifted3_ = false;
     const bool h2_valid = hist2_ok_, h3_valid = hist3_ok_;  // before this step shifts them
     if (keep2) {
@@ -306,14 +311,12 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
                              "hipMalloc prev2"))
             return -1;
         h2 = prev2_;
-        pout = prev2_;
         shifted2_ = hist_ok_;  // prev2 becomes a valid u^(n-1)
         if (keep3) {
             if (!prev3_ && check(hipMalloc((void**)&prev3_, (size_t)n_ * sizeof(double)),
                                  "hipMalloc prev3"))
                 return -1;
             h3 = prev3_;
-            pout = prev3_;
             shifted3_ = hist_ok_ && hist2_ok_;  // prev3 becomes a valid u^(n-2)
         }
     }
@@ -348,29 +351,31 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
         (void)hipEventRecord(e0, st_);
     }
     if (mixed && with_g) {
-        if (p2) k_setup_step<true, true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm, csor, part2);
-        else k_setup_step<true, false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm, csor, part2);
+        if (p2) k_setup_step<true, true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, ls, part_, warm, csor, part2);
+        else k_setup_step<true, false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, ls, part_, warm, csor, part2);
     } else if (mixed) {
-        if (p2) k_setup_step<false, true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm, csor, part2);
-        else k_setup_step<false, false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm, csor, part2);
+        if (p2) k_setup_step<false, true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, ls, part_, warm, csor, part2);
+        else k_setup_step<false, false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, ls, part_, warm, csor, part2);
     } else if (with_g) {
-        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm);
-        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm);
+        if (p2) k_setup_step<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, ls, part_, warm);
+        else k_setup_step<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, ls, part_, warm);
     } else {
-        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm);
-        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, pout, ls, part_, warm);
+        if (p2) k_setup_step<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, ls, part_, warm);
+        else k_setup_step<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, P.eps2, h4, full_[A_U], g, sb, prevf, h2, h3, ls, part_, warm);
     }
     if (launch_check("k_setup_step")) return -1;
     split_ = mixed;
-    if (pout == prev3_ && prev3_) {
-        double* o1 = full_[A_PREV];
-        double* o2 = prev2_;
-        full_[A_PREV] = prev3_;
-        prev2_ = o1;
-        prev3_ = o2;
-    } else if (pout == prev2_ && prev2_) {
-        prev2_ = full_[A_PREV];
-        full_[A_PREV] = pout;
+    if (write_prev) {
+        rot_[0] = full_[A_U];
+        rot_[1] = full_[A_PREV];
+        rot_[2] = prev2_;
+        rot_[3] = prev3_;
+        rotated_ = true;
+        double* freed = keep3 ? prev3_ : (keep2 ? prev2_ : full_[A_PREV]);
+        if (keep3) prev3_ = prev2_;
+        if (keep2) prev2_ = full_[A_PREV];
+        full_[A_PREV] = full_[A_U];
+        full_[A_U] = freed;  // written whole by k_unpack at the end of the step
     }
     if (prof.on) {
         hipEventRecord(e1, st_);
@@ -865,7 +870,7 @@ int Engine::learned_preview(float* host_c, double* host_q) {
 
 int Engine::unpack(const double* dmask) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
-    dim3 grid(cdiv((long)(I_ - 2) * (J_ - 2), kBlock), K_ - 2, L_ - 2);
+    dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
     k_unpack<<<grid, kBlock, 0, st_>>>(G, ur_[red_final_], ub_, full_[A_U], dmask);
     return launch_check("k_unpack");
 }
@@ -1037,12 +1042,15 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
         hipEventRecord(e0, st_);
     }
     if (setup(P, true, true, nullptr, precision_ == 1)) return -1;
-    if (solve(P, st)) return -1;
+    if (solve(P, st)) {
+        undo_rotation();
+        return -1;
+    }
     const bool has_balls = balls && balls->total > 0;
-    if (has_balls) {
-        if (unpack(nullptr) || rescale(full_[A_U], *balls, true)) return -1;
-    } else {
-        if (unpack(full_[A_MASK])) return -1;
+    if (has_balls ? (unpack(nullptr) || rescale(full_[A_U], *balls, true))
+                  : unpack(full_[A_MASK])) {
+        undo_rotation();
+        return -1;
     }
     if (prof.on) {
         hipEventRecord(e1, st_);
@@ -1054,7 +1062,10 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
         hipEventDestroy(e0);
         hipEventDestroy(e1);
     }
-    if (check(hipStreamSynchronize(st_), "seg_step")) return -1;
+    if (check(hipStreamSynchronize(st_), "seg_step")) {
+        undo_rotation();
+        return -1;
+    }
     step_done();
     return 0;
 }
@@ -1062,14 +1073,22 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
 int Engine::mcf_step(const StepParams& P, double t, SolveStats* st) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     if (setup(P, false, true)) return -1;
+    // the boundary values at t go to the ghosts of u's new buffer (k_unpack
+    // copies them back from the colour arrays with the interior)
     dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
     k_mcf_exact<<<grid, kBlock, 0, st_>>>(G, P.h, t, full_[A_U], 1);
-    if (launch_check("k_mcf_exact")) return -1;
-    k_pack_ghosts<<<grid, kBlock, 0, st_>>>(G, full_[A_U], ur_[0], ur_[1], ub_);
-    if (launch_check("k_pack_ghosts")) return -1;
-    if (solve(P, st)) return -1;
-    if (unpack(nullptr)) return -1;
-    if (check(hipStreamSynchronize(st_), "mcf_step")) return -1;
+    int rc = launch_check("k_mcf_exact");
+    if (!rc) {
+        k_pack_ghosts<<<grid, kBlock, 0, st_>>>(G, full_[A_U], ur_[0], ur_[1], ub_);
+        rc = launch_check("k_pack_ghosts");
+    }
+    if (!rc) rc = solve(P, st);
+    if (!rc) rc = unpack(nullptr);
+    if (!rc) rc = check(hipStreamSynchronize(st_), "mcf_step");
+    if (rc) {
+        undo_rotation();
+        return -1;
+    }
     step_done();
     return 0;
 }

@@ -294,8 +294,9 @@ __device__ __forceinline__ bool map_plane_block(const PlaneMap& M, int b, int& b
 // 2 = from the learned start (interior points, LearnedStart); 3 = from the
 // quadratic extrapolation 3 u^n - 3 u^(n-1) + u^(n-2) (prev2 holds u^(n-2));
 // 4 = from the cubic one 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) (prev3
-// holds u^(n-3)).  Non-null history arrays shift by one level: prev3
-// receives u^(n-2), prev2 u^(n-1), prev_full u^n.
+// holds u^(n-3)).  u^n is not copied: the host rotates the full-layout
+// buffers after the launch (u's buffer becomes the history's newest level
+// and the oldest level's buffer receives u^(n+1) from k_unpack).
 // MIXED: the set-up of the FP32-refined form, fused with its first FP64
 // defect.  The colour arrays receive u^n and the FP32 correction arrays the
 // start's offset d0 = start - u^n (the correction solve starts from it); the
@@ -306,7 +307,7 @@ template <bool WITH_G, bool POW2, bool MIXED = false>
 static __global__ __launch_bounds__(kBlock) void k_setup_step(
     Geo G, PlaneMap PM, Scale S, double eps2, double h4, const double* __restrict__ u,
     CPtr8 g, SplitBufs sb, const double* hist1, const double* hist2, const double* hist3,
-    double* prev_out, LearnedStart ls, double* __restrict__ partials,
+    LearnedStart ls, double* __restrict__ partials,
     int warm, double cc = 0.0, double* __restrict__ partials2 = nullptr) {
     int bx, k, l;
     if (!map_plane_block(PM, blockIdx.x, bx, k, l)) return;
@@ -320,8 +321,7 @@ static __global__ __launch_bounds__(kBlock) void k_setup_step(
         const double uc = u[p];
         double start = uc;
         const bool in = is_interior(G, i, j, k, l);
-        // history u^(n-1), u^(n-2), u^(n-3) as the start needs it; u^n goes to
-        // prev_out, which may be one of them (read first, same point)
+        // history u^(n-1), u^(n-2), u^(n-3) as the start needs it
         const bool cub = warm == 4 || (warm == 2 && ls.order == 3);
         const double pf = warm >= 1 ? hist1[p] : 0.0;
         const double p2 = (warm == 3 || cub) ? hist2[p] : 0.0;
@@ -333,7 +333,6 @@ static __global__ __launch_bounds__(kBlock) void k_setup_step(
             const double base = ls.order == 3 ? start : uc + (uc - pf);
             start = base + sqrt(*ls.dsum * ls.inv_n) * (double)(ls.out[p] * ls.inv_st);
         }
-        prev_out[p] = uc;
         if constexpr (MIXED) {
             // (at ghosts start = u^n, so d0 = 0 there)
             if (c == 0) {
@@ -903,19 +902,21 @@ static __global__ __launch_bounds__(1024) void k_reduce_sum(const double* __rest
 // the reference's Mmask pass (segmentation4D.c:1093-1101) fused in when
 // there are no balls to rescale first.
 // ----------------------------------------------------------------------------
+// Every padded point: the ghosts come from the colour arrays too (the set-up
+// wrote u^n's ghosts there, the MCF step its boundary values), because u's
+// buffer is a recycled history level; the mask applies to the interior only.
 static __global__ __launch_bounds__(kBlock) void k_unpack(Geo G, const double* __restrict__ ur,
                                                   const double* __restrict__ ub,
                                                   double* __restrict__ u,
                                                   const double* __restrict__ mask) {
     const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int k = blockIdx.y + 1, l = blockIdx.z + 1;
-    const int w = G.I - 2;
-    if (t >= w * (G.J - 2)) return;
-    const int j = t / w + 1, i = t - (j - 1) * w + 1;
+    const int k = blockIdx.y, l = blockIdx.z;
+    if (t >= G.I * G.J) return;
+    const int j = t / G.I, i = t - j * G.I;
     const int p = ((l * G.K + k) * G.J + j) * G.I + i;
     const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
     double v = ((i + j + k + l) & 1) ? ub[pp] : ur[pp];
-    if (mask != nullptr && mask[p] == 0) v = 0;
+    if (mask != nullptr && mask[p] == 0 && is_interior(G, i, j, k, l)) v = 0;
     u[p] = v;
 }
 
