@@ -79,6 +79,57 @@ def test_fp32_refined_vs_oracle(seglib, oracle, dims, inner_omega):
         lib.deallocate()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_warm_mode_switches_keep_history(seglib, oracle, precision):
+    """The history levels and u live in buffers that rotate by pointer after
+    each set-up (the oldest level's buffer receives u^(n+1)); which buffers
+    rotate depends on the warm-start mode.  Switching modes between steps in
+    resident mode: after every step prev_u holds u^n bitwise (ghosts
+    included), and an FP64 step with mode 0 (start u^n) equals the oracle's
+    step from the device's u^n bitwise."""
+    lib = seglib
+    dims = (24, 20, 12, 6)
+    shape, img, u0 = _volume(lib, dims, 11)
+    lib.allocate(*dims)
+    try:
+        lib.set(gsTol=0.0, **KW)
+        lib.arr("i0")[...] = img
+        lib.arr("u")[...] = 0.0
+        lib.arr("Mmask")[...] = 1.0
+        lib.gf("i0", "u")
+        g = [lib.arr("g_" + f).copy() for f in ("e", "w", "n", "s", "t", "b", "fw", "bw")]
+        tol = REL * REL * oracle.sumsq_interior(params(shape, **KW), u0)
+        lib.set(gsTol=tol, **KW)
+        lib.set_resident(True)
+        lib.arr("u")[...] = u0
+        lib.upload("u")
+        lib.upload("Mmask")
+        lib.set_precision(precision)
+        modes = [4, 4, 4, 0, 1, 1, 0, 3, 4, 4, 0, 1, 0]
+        exact = 0
+        for st, warm in enumerate(modes):
+            lib.set_warm_start(warm)
+            lib.download("u")
+            before = lib.arr("u").copy()
+            z, R = lib.seg_time_step()
+            lib.download("u")
+            lib.download("prev_u")
+            assert np.array_equal(lib.arr("prev_u"), before), f"step {st} (warm {warm}): prev_u"
+            if precision == 0 and warm == 0:
+                uo, prev, coef = before.copy(), np.zeros(shape), zeros(shape, 8)
+                zo = oracle.seg_time_step(params(shape, gsTol=tol, **KW), uo, prev, g, coef,
+                                          np.ones(shape))[0]
+                assert z == zo, f"step {st}: sweeps {z} vs oracle {zo}"
+                assert np.array_equal(lib.arr("u"), uo), f"step {st}: u after a mode-0 step"
+                exact += 1
+        assert precision == 1 or exact == modes.count(0)
+    finally:
+        lib.set_warm_start(0)
+        lib.set_precision(0)
+        lib.set_resident(False)
+        lib.deallocate()
+
+
 @pytest.mark.parametrize("precision,warm", [(0, 1), (1, 1), (0, 3), (1, 3), (0, 4), (1, 4)])
 def test_extrapolated_start_vs_oracle(seglib, oracle, precision, warm):
     """SOR started from 2 u^n - u^(n-1) (warm 1) or 3 u^n - 3 u^(n-1) +
