@@ -185,7 +185,7 @@ public:
     int warm_ = 0;
     bool hist_ok_ = false;
     bool hist2_ok_ = false, hist3_ok_ = false;
-    bool shifted2_ = false, shifted3_ = false;  // this step's setup filled prev2_ / prev3_
+    bool shifted2_ = false, shifted3_ = false;  // after this step prev2_ / prev3_ hold valid levels
     bool rotated_ = false;                       // setup() rotated; step_done() not yet called
     double* rot_[4] = {};                        // u, prev_full, prev2_, prev3_ before it
     double* prev2_ = nullptr;      // u^(n-2) for the quadratic and cubic starts
