@@ -1041,7 +1041,10 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
         hipEventCreate(&e1);
         hipEventRecord(e0, st_);
     }
-    if (setup(P, true, true, nullptr, precision_ == 1)) return -1;
+    if (setup(P, true, true, nullptr, precision_ == 1)) {
+        undo_rotation();  // set-up may fail after it rotated (its reduction)
+        return -1;
+    }
     if (solve(P, st)) {
         undo_rotation();
         return -1;
@@ -1072,7 +1075,10 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
 
 int Engine::mcf_step(const StepParams& P, double t, SolveStats* st) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
-    if (setup(P, false, true)) return -1;
+    if (setup(P, false, true)) {
+        undo_rotation();  // set-up may fail after it rotated (its reduction)
+        return -1;
+    }
     // the boundary values at t go to the ghosts of u's new buffer (k_unpack
     // copies them back from the colour arrays with the interior)
     dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
