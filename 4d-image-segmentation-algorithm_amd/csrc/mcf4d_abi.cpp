@@ -110,8 +110,14 @@ API bool _allocateMem_mcf(void) {
         s4d::set_error("_allocateMem_mcf: padded dims must be >= 3");
         return false;
     }
-    free_host();
     const long N = nelem();
+    // before any allocation: int indices (mcf_4D.c:36-40) reach 2^31 - 1
+    if (N >= (1L << 31)) {
+        s4d::set_error("_allocateMem_mcf: padded volume exceeds 2^31 elements (32-bit index, as "
+                       "the reference)");
+        return false;
+    }
+    free_host();
     double** all[] = {&u, &prev_u, &exact_u, &e, &w, &n, &s, &t, &b, &fw, &bw};
     for (double** p : all) {
         *p = (double*)std::calloc((size_t)N, sizeof(double));

@@ -186,8 +186,16 @@ API bool _allocateMem_(void) {
         s4d::set_error("_allocateMem_: padded dims must be >= 3");
         return false;
     }
-    free_host();
     const long N = nelem();
+    // checked before any allocation: the reference's int index (ijkl,
+    // segmentation4D.c:61-65) and the device kernels' 32-bit indices cover
+    // at most 2^31 - 1 padded elements
+    if (N >= (1L << 31)) {
+        s4d::set_error("_allocateMem_: padded volume exceeds 2^31 elements (32-bit index, as the "
+                       "reference)");
+        return false;
+    }
+    free_host();
     double** all[] = {&u, &prev_u, &i0, &Mmask, &e, &w, &n, &s, &t, &b, &fw, &bw,
                       &g_e, &g_w, &g_n, &g_s, &g_t, &g_b, &g_fw, &g_bw};
     for (double** p : all) {

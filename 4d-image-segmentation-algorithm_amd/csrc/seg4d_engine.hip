@@ -189,6 +189,12 @@ void Engine::undo_rotation() {
     prev2_ = rot_[2];
     prev3_ = rot_[3];
     rotated_ = false;
+    // The buffer that became u's may already hold (part of) u^(n+1): k_unpack
+    // writes it at the end of the step, and a later failure (the host wait, a
+    // batch's later volume) cannot tell whether it ran.  That buffer is a
+    // history level again (prev_full, prev2_ or prev3_), so no level can be
+    // trusted: the next step starts from u^n until new history exists.
+    hist_ok_ = hist2_ok_ = hist3_ok_ = false;
 }
 
 void Engine::step_done() {

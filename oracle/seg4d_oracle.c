@@ -26,7 +26,45 @@
 #include <omp.h>
 
 #define NT(p) ((p)->nthreads > 1 ? (p)->nthreads : 1)
-#define NR(p) ((p)->par_reduce ? NT(p) : 1)
+#define NR(p) ((p)->par_reduce == 1 ? NT(p) : 1)
+/* par_reduce == 2 with threads: the terms of a residual sum are formed in
+ * parallel into a buffer and added up serially in lexicographic order, so
+ * the sum is bitwise the serial one (the threaded oracle then gives the
+ * serial oracle's sweep counts at any size). */
+#define TERMS(p) ((p)->par_reduce == 2 && NT(p) > 1)
+
+/* interior-sized term buffer of the TERMS mode (test infrastructure: one
+ * caller at a time, kept between calls) */
+static double *terms_buffer(size_t n)
+{
+    static double *buf = NULL;
+    static size_t cap = 0;
+    if (n > cap) {
+        free(buf);
+        buf = (double *)malloc(n * sizeof(double));
+        cap = buf ? n : 0;
+    }
+    return buf;
+}
+
+static size_t interior_index(const orc_params *p, int i, int j, int k, int l)
+{
+    return (((size_t)(l - 1) * (p->kMax - 2) + (k - 1)) * (p->jMax - 2) + (j - 1)) *
+               (p->iMax - 2) + (i - 1);
+}
+
+static size_t interior_size(const orc_params *p)
+{
+    return (size_t)(p->iMax - 2) * (p->jMax - 2) * (p->kMax - 2) * (p->lMax - 2);
+}
+
+/* serial left-to-right sum of n terms: acc = ((0 + t0) + t1) + ... */
+static double serial_sum(const double *t, size_t n)
+{
+    double acc = 0.0;
+    for (size_t q = 0; q < n; q++) acc += t[q];
+    return acc;
+}
 
 size_t orc_size(const orc_params *p)
 {
@@ -165,8 +203,10 @@ static double heat_residual(const orc_params *p, const double *u,
     long off[8];
     nb_offsets(p, off);
     const double h4 = p->h * p->h * p->h * p->h;
+    double *terms = TERMS(p) ? terms_buffer(interior_size(p)) : NULL;
     double acc = 0.0;
-    #pragma omp parallel for num_threads(NR(p)) if (NR(p) > 1) reduction(+:acc) schedule(static)
+    #pragma omp parallel for num_threads(terms ? NT(p) : NR(p)) if (terms || NR(p) > 1) \
+        reduction(+:acc) schedule(static)
     for (int l = 1; l <= p->lMax - 2; l++)
         for (int k = 1; k <= p->kMax - 2; k++)
             for (int j = 1; j <= p->jMax - 2; j++)
@@ -176,9 +216,10 @@ static double heat_residual(const orc_params *p, const double *u,
                                 u[q + off[3]] + u[q + off[4]] + u[q + off[5]] +
                                 u[q + off[6]] + u[q + off[7]];
                     double r = u[q] * (1 + c * 8) - c * nb - prev[q];
-                    acc += r * r * h4;
+                    if (terms) terms[interior_index(p, i, j, k, l)] = r * r * h4;
+                    else acc += r * r * h4;
                 }
-    return acc;
+    return terms ? serial_sum(terms, interior_size(p)) : acc;
 }
 
 /* segmentation4D.c:337-409 -- copy to prev, reflect once (ghosts then stay
@@ -324,8 +365,10 @@ double orc_residual(const orc_params *p, const double *u, const double *prev,
     nb_offsets(p, off);
     const double c = p->tau / (p->h * p->h);
     const double h4 = p->h * p->h * p->h * p->h;
+    double *terms = TERMS(p) ? terms_buffer(interior_size(p)) : NULL;
     double acc = 0.0;
-    #pragma omp parallel for num_threads(NR(p)) if (NR(p) > 1) reduction(+:acc) schedule(static)
+    #pragma omp parallel for num_threads(terms ? NT(p) : NR(p)) if (terms || NR(p) > 1) \
+        reduction(+:acc) schedule(static)
     for (int l = 1; l <= p->lMax - 2; l++)
         for (int k = 1; k <= p->kMax - 2; k++)
             for (int j = 1; j <= p->jMax - 2; j++)
@@ -339,9 +382,10 @@ double orc_residual(const orc_params *p, const double *u, const double *prev,
                     double diag = coef[0][q] + coef[1][q] + coef[2][q] + coef[3][q] +
                                   coef[4][q] + coef[5][q] + coef[6][q] + coef[7][q];
                     double r = u[q] * (1 + c * diag) - c * num - prev[q];
-                    acc += r * r * h4;
+                    if (terms) terms[interior_index(p, i, j, k, l)] = r * r * h4;
+                    else acc += r * r * h4;
                 }
-    return acc;
+    return terms ? serial_sum(terms, interior_size(p)) : acc;
 }
 
 /* segmentation4D.c:229-309 -- do { red; black; R } while (R > gsTol && z < cap) */

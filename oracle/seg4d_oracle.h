@@ -35,7 +35,11 @@ typedef struct {
     int par_reduce;                   /* 1 = OpenMP reduction for residual
                                          sums (the reference OpenMP program's
                                          order); 0 = lexicographic serial sum
-                                         (the serial reference's order)     */
+                                         (the serial reference's order);
+                                         2 = residual terms formed by the
+                                         threads, then summed serially in
+                                         lexicographic order (bitwise the
+                                         serial sum, most of the speed)     */
 } orc_params;
 
 typedef struct {                      /* ball list for the rescale epilogue */

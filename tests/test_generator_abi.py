@@ -133,3 +133,21 @@ def test_allocate_fails_loudly_without_gpu(lib):
     lib.deallocate()
     assert lib.lib._segTimeStep_() is False
     assert "not allocated" in lib.last_error()
+
+
+@pytest.mark.parametrize("which", ["seg", "mcf"])
+def test_oversize_volume_fails_before_any_allocation(which):
+    """A padded volume of 2^31 or more elements (the reference's int index,
+    segmentation4D.c:61-65, mcf_4D.c:36-40) is refused before any host or
+    device array exists: the global pointers stay NULL and the error names
+    the limit.  Runs with or without a GPU (nothing reaches the device)."""
+    L = seg4d.SegLib() if which == "seg" else seg4d.McfLib()
+    L.set_verbose(False)
+    alloc = L.lib._allocateMem_ if which == "seg" else L.lib._allocateMem_mcf
+    for dims in ((1291, 1291, 1291, 3), (2002, 2002, 1002, 5)):
+        # 1291^3 * 3 = 6,454,995,513 >= 2^31; the first fits a long, not an int
+        L.set(iMax=dims[0], jMax=dims[1], kMax=dims[2], lMax=dims[3])
+        assert alloc() is False
+        assert "2^31" in L.last_error()
+        assert not L.ptr("u") and not L.ptr("prev_u")
+    L.set(iMax=0, jMax=0, kMax=0, lMax=0)
