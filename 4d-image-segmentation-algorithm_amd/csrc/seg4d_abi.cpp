@@ -763,6 +763,12 @@ API int seg4dSetSweepTile(int runs, int planes) {
     return 0;
 }
 
+API int seg4dInjectStepFailure(int where) {
+    if (!g_eng) return -1;
+    g_eng->inject_ = (where == 1 || where == 2) ? where : 0;
+    return 0;
+}
+
 API bool seg4dSynchronize(void) {
     if (!g_eng) return true;
     return g_eng->sync() == 0;

@@ -198,6 +198,11 @@ public:
     double inner_frac_ = 0.9;
     double inner_omega_ = 1.2;     // FP32 correction solve's SOR factor (0: omega_c)
     double inner_floor2_ = 1e-10;  // (1e-5)^2: what an FP32 solve can resolve
+    // Test hook of the failure paths: 1 = the next seg_step fails right
+    // after its set-up (buffers rotated), 2 = it fails at its final wait,
+    // after k_unpack wrote u^(n+1) into the recycled buffer.  Forced from
+    // the host: every kernel still runs to completion.  Cleared when used.
+    int inject_ = 0;
     Profile prof;
     std::string err;
 

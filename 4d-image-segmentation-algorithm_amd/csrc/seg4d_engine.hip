@@ -1051,6 +1051,12 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
         undo_rotation();  // set-up may fail after it rotated (its reduction)
         return -1;
     }
+    if (inject_ == 1) {
+        inject_ = 0;
+        (void)hipStreamSynchronize(st_);
+        undo_rotation();
+        return check(hipErrorUnknown, "injected failure after the set-up");
+    }
     if (solve(P, st)) {
         undo_rotation();
         return -1;
@@ -1074,6 +1080,11 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
     if (check(hipStreamSynchronize(st_), "seg_step")) {
         undo_rotation();
         return -1;
+    }
+    if (inject_ == 2) {  // as a failed final wait: k_unpack has run
+        inject_ = 0;
+        undo_rotation();
+        return check(hipErrorUnknown, "injected failure at the final wait");
     }
     step_done();
     return 0;

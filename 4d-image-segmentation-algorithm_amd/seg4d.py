@@ -221,6 +221,8 @@ class SegLib(_Lib):
         L.seg4dBatchAutotune.argtypes = [c_int, POINTER(ctypes.c_float)]
         L.seg4dBatchTile.restype = c_int
         L.seg4dBatchTile.argtypes = [c_int, POINTER(c_int), POINTER(c_int)]
+        L.seg4dInjectStepFailure.restype = c_int
+        L.seg4dInjectStepFailure.argtypes = [c_int]
         L.seg4d_gen_preset.restype = c_int
         L.seg4d_gen_preset.argtypes = [c_int, c_uint64, POINTER(GenSpec)]
         L.seg4d_generate.restype = c_int
@@ -533,6 +535,12 @@ class SegLib(_Lib):
             self.lib.seg4dBatchTile(v, ctypes.byref(tb), ctypes.byref(tk))
             tiles.append((tb.value, tk.value))
         return tiles, list(ms)
+
+    def inject_step_failure(self, where):
+        """Test hook: the next _segTimeStep_ fails after its set-up (1) or
+        at its final wait (2), forced from the host."""
+        if self.lib.seg4dInjectStepFailure(int(where)) != 0:
+            raise RuntimeError("seg4dInjectStepFailure: no volume allocated")
 
     def synchronize(self):
         self.check(self.lib.seg4dSynchronize(), "seg4dSynchronize")

@@ -359,6 +359,15 @@ int seg4dBatchConfigure(int precision, int warm, int graph, int depth);
 int seg4dBatchAutotune(int reps, float *best_ms);
 int seg4dBatchTile(int v, int *runs, int *planes);
 
+/* Test hook of the failure paths (no counterpart in the reference): makes
+ * the next _segTimeStep_ fail, forced from the host, 1 = right after its
+ * set-up kernel, 2 = at its final wait (after the new u was written into
+ * the recycled device buffer).  Every kernel still runs to completion; the
+ * step then returns false as after a device error, with u = u^n on the
+ * device, the warm-start history dropped and seg4dLastError() naming the
+ * injection.  0 clears it.  Returns 0, or -1 before _allocateMem_. */
+int seg4dInjectStepFailure(int where);
+
 /* Blocks until all queued device work is done. */
 bool seg4dSynchronize(void);
 /* Device name / CU count / arch into buf; returns 0 on success. */
