@@ -604,13 +604,16 @@ __device__ __forceinline__ void update_point(const PointIn<T>& P, T ybw, T ypart
                 for (int f = 0; f < 8; f++)
                     defect[(f - 8) * 64] = __int_as_float(__double2hiint((double)P.c[f]));
             }
-            if constexpr (RESID >= 5) xout[P.pp] = P.uo;
+            if constexpr (RESID >= 5) __builtin_nontemporal_store(P.uo, xout + P.pp);
             return;
         }
     }
     const T gs = (P.pv + c * num) / D;
     const T un = P.uo + om * (gs - P.uo);
-    xout[P.pp] = un;
+    // streaming store: the new value is read again only by the next pass, so
+    // it need not displace the other colour's planes from L2 (measured 1%
+    // faster than a plain store at 256x256x128x16)
+    __builtin_nontemporal_store(un, xout + P.pp);
     if constexpr (RESID == 2) {
         const T r = un * D - c * num - P.pv;
         const double rd = (double)r;
