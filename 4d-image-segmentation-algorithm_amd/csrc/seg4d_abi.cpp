@@ -763,6 +763,12 @@ API int seg4dSetSweepTile(int runs, int planes) {
     return 0;
 }
 
+API int seg4dL2normNoAlloc(const double* a, const double* b, long n, double hh, double* result,
+                           void* stream) {
+    // no engine, host array or device allocation is involved
+    return s4d::l2norm_noalloc(a, b, n, hh, result, static_cast<hipStream_t>(stream));
+}
+
 API int seg4dInjectStepFailure(int where) {
     if (!g_eng) return -1;
     g_eng->inject_ = (where == 1 || where == 2) ? where : 0;

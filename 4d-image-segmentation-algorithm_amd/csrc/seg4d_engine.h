@@ -254,6 +254,13 @@ public:
     bool balls_valid_ = false;
 };
 
+// Allocation-free _l2norm_ on caller-owned device arrays a, b of n elements:
+// sqrt(sum (a-b)^2 h^4) into the device double *result, enqueued on stream st
+// (nothing waited for).  Block partials go to a fixed device scratch array;
+// n is limited to 65536 x 4096 elements.  Returns 0, or -1 with last_error().
+int l2norm_noalloc(const double* a, const double* b, long n, double h, double* result,
+                   hipStream_t st);
+
 // Chunking of the two-level residual reduction over nb partials per colour.
 void resid_chunking(int nb, int* nc, int* ch);
 

@@ -1124,7 +1124,7 @@ int Engine::reflect(double* d) {
 }
 
 int Engine::reduce_to_host(int np, double* out) {
-    k_reduce_sum<<<1, 1024, 0, st_>>>(part_, np, dscal_);
+    k_reduce_sum<false><<<1, 1024, 0, st_>>>(part_, np, dscal_);
     if (launch_check("k_reduce_sum")) return -1;
     if (check(hipMemcpyAsync(hscal_, dscal_, sizeof(double), hipMemcpyDeviceToHost, st_),
               "scalar copy"))
@@ -1215,12 +1215,79 @@ int Engine::gfun(const double* a, const double* b, double h, double K, double th
 
 int Engine::l2norm(const double* a, const double* b, double h, double* out) {
     const double h4 = h * h * h * h;
-    const int nblk = 2048;
+    const int nblk = l2_blocks(n_);  // <= the partial capacity (>= n_ / kBlock)
     k_sqdiff<<<nblk, kBlock, 0, st_>>>(a, b, n_, h4, part_);
     if (launch_check("k_sqdiff")) return -1;
     double S = 0.0;
     if (reduce_to_host(nblk, &S)) return -1;
     *out = std::sqrt(S);
+    return 0;
+}
+
+// ---- allocation-free _l2norm_ ------------------------------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, tight serving loops): the block partial sums live in `scratch`, a
+// file-scope device array of fixed size, instead of the engine's buffers.
+// Every GPU the library runs on holds its own copy until the process ends;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// for the current device (kept per device in g_scratch_dev), never through
+// the symbol itself.  Nothing in it needs clearing: k_sqdiff writes every
+// partial it uses before k_reduce_sum reads it.  Calls that overlap on one
+// GPU share the array and race.
+namespace {
+static __device__ __align__(256) double scratch[kL2MaxPartials];
+constexpr int kScratchDevices = 64;
+double* g_scratch_dev[kScratchDevices] = {};
+
+int scratch_address(double** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error(std::string("seg4dL2normNoAlloc: hipGetDevice: ") + hipGetErrorString(e));
+        return -1;
+    }
+    if (dev < 0 || dev >= kScratchDevices) {
+        set_error("seg4dL2normNoAlloc: device index beyond the per-device scratch table");
+        return -1;
+    }
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("seg4dL2normNoAlloc: hipGetSymbolAddress(scratch): ") +
+                      hipGetErrorString(e));
+            return -1;
+        }
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+}  // namespace
+
+int l2norm_noalloc(const double* a, const double* b, long n, double h, double* result,
+                   hipStream_t st) {
+    if (!a || !b || !result) {
+        set_error("seg4dL2normNoAlloc: null device pointer");
+        return -1;
+    }
+    if (n < 1 || n > kL2MaxPartials * kL2PerPartial) {
+        set_error("seg4dL2normNoAlloc: element count " + std::to_string(n) +
+                  " outside 1.." + std::to_string(kL2MaxPartials * kL2PerPartial) +
+                  " (the scratch array holds 65536 partial sums of 4096 elements)");
+        return -1;
+    }
+    double* sp = nullptr;
+    if (scratch_address(&sp)) return -1;
+    const int nblk = l2_blocks(n);
+    const double h4 = h * h * h * h;
+    k_sqdiff<<<nblk, kBlock, 0, st>>>(a, b, n, h4, sp);
+    k_reduce_sum<true><<<1, 1024, 0, st>>>(sp, nblk, result);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("seg4dL2normNoAlloc: launch: ") + hipGetErrorString(e));
+        return -1;
+    }
     return 0;
 }
 

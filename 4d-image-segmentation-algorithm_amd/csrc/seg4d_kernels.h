@@ -884,7 +884,9 @@ static __global__ __launch_bounds__(1024) void k_set_r0(const double* __restrict
     }
 }
 
-// Generic fixed-order sum of `np` partials into *out.
+// Generic fixed-order sum of `np` partials into *out (SQRT: its square root,
+// correctly rounded like the host's sqrt).
+template <bool SQRT = false>
 static __global__ __launch_bounds__(1024) void k_reduce_sum(const double* __restrict__ partials,
                                                     int np, double* out) {
     double s = 0.0;
@@ -896,8 +898,18 @@ static __global__ __launch_bounds__(1024) void k_reduce_sum(const double* __rest
     if (threadIdx.x == 0) {
         double S = sm[0];
         for (int q = 1; q < 16; q++) S += sm[q];
-        *out = S;
+        *out = SQRT ? sqrt(S) : S;
     }
+}
+
+// Blocks of the _l2norm_ reduction over n elements: one partial per 4096
+// elements (16 per thread, grid-stride), at most kL2MaxPartials; shared by
+// the engine's and the allocation-free entry point so both sum in one order.
+constexpr int kL2MaxPartials = 65536;
+constexpr long kL2PerPartial = 16L * kBlock;
+__host__ __device__ inline int l2_blocks(long n) {
+    const long b = (n + kL2PerPartial - 1) / kL2PerPartial;
+    return b < 1 ? 1 : (b > kL2MaxPartials ? kL2MaxPartials : (int)b);
 }
 
 // ----------------------------------------------------------------------------

@@ -359,6 +359,20 @@ int seg4dBatchConfigure(int precision, int warm, int graph, int depth);
 int seg4dBatchAutotune(int reps, float *best_ms);
 int seg4dBatchTile(int v, int *runs, int *planes);
 
+/* Allocation-free variant of _l2norm_ (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops).  a, b: device arrays of n doubles; result: a device
+ * double receiving sqrt(sum over the n elements of (a-b)^2 h^4), the value
+ * _l2norm_ returns for the same arrays, bit for bit.  The work is enqueued on
+ * `stream` (a hipStream_t; NULL = the default stream) and not waited for:
+ * synchronise before reading *result.  It calls no allocator: the partial
+ * sums go to a fixed device array of 65536 doubles (512 KB) per GPU, so
+ * 1 <= n <= 65536 * 4096 = 268,435,456 (larger n is refused), and calls that
+ * overlap on one GPU share that array and race.  Independent of
+ * _allocateMem_.  Returns 0, or -1 with seg4dLastError() set. */
+int seg4dL2normNoAlloc(const double *a, const double *b, long n, double h, double *result,
+                       void *stream);
+
 /* Test hook of the failure paths (no counterpart in the reference): makes
  * the next _segTimeStep_ fail, forced from the host, 1 = right after its
  * set-up kernel, 2 = at its final wait (after the new u was written into
