@@ -329,7 +329,9 @@ API bool _gf_inputImage_plus_thresholdedImage_(double* a, double* bb) {
     double* db = device_of(bb, s4d::A_TMP2, true);
     if (!da || !db || da == db) return fail("_gf_: cannot resolve arguments");
     if (!presmooth_device(da) || !presmooth_device(db)) return fail("_gf_: presmoothing");
-    if (g_eng->gfun(da, db, h, K, theta_img, rho_thr)) return fail("_gf_: edge indicator");
+    // a finite presmoothing residual of b means every value of b is finite
+    if (g_eng->gfun(da, db, h, K, theta_img, rho_thr, std::isfinite(g_last.R)))
+        return fail("_gf_: edge indicator");
     if (!g_resident) {
         if (!download_to(a, da) || !download_to(bb, db) ||
             g_eng->download(s4d::A_PREV, prev_u))

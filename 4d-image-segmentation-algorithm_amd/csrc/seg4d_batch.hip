@@ -1,6 +1,8 @@
 // seg4d_batch.hip -- the batched form (see seg4d_batch.h).
 #include "seg4d_batch.h"
 
+#include <cmath>
+
 #include <algorithm>
 #include <cstring>
 
@@ -97,7 +99,7 @@ int Batch::set_volume(int v, const double* img, const double* u0, const double* 
     // the edge indicator of the image alone: rho = 0, so the second
     // image's term vanishes exactly and the image serves as both arguments
     if (E.presmooth(E.full(A_I0), ep.sigma, ep.h, ep.omega, ep.gsTol, ep.rel2, ep.cap, &z, &R) ||
-        E.gfun(E.full(A_I0), E.full(A_I0), ep.h, ep.K, ep.theta, 0.0))
+        E.gfun(E.full(A_I0), E.full(A_I0), ep.h, ep.K, ep.theta, 0.0, std::isfinite(R)))
         return -1;
     return E.sync();
 }

@@ -134,8 +134,12 @@ public:
     // Implicit heat step in place on device array d (prev receives input).
     int presmooth(double* d, double sigma, double h, double omega, double gsTol,
                   double rel2, double cap, int* iters, double* R);
+    // 8 face g's from the presmoothed images a and b.  b_finite: every value
+    // of b is known finite (e.g. its presmoothing residual was finite: the
+    // ghosts are copies of interior values and a non-finite interior value
+    // makes the residual non-finite), which lets rho == 0 skip b.
     int gfun(const double* a, const double* b, double h, double K, double theta,
-             double rho);
+             double rho, bool b_finite = false);
 
     // --- helpers --------------------------------------------------------------
     int l2norm(const double* a, const double* b, double h, double* out);
@@ -188,6 +192,7 @@ public:
     bool shifted2_ = false, shifted3_ = false;  // after this step prev2_ / prev3_ hold valid levels
     bool rotated_ = false;                       // setup() rotated; step_done() not yet called
     double* rot_[4] = {};                        // u, prev_full, prev2_, prev3_ before it
+    double* hp_ = nullptr;         // presmoothing right-hand side, colour-packed (lazy)
     double* prev2_ = nullptr;      // u^(n-2) for the quadratic and cubic starts
     double* prev3_ = nullptr;      // u^(n-3) for the cubic start
     Net* net_ = nullptr;           // not owned

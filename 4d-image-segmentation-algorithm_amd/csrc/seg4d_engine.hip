@@ -29,6 +29,30 @@ Scale make_scale(double h) {
 }
 
 inline unsigned cdiv(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+// Block map of the full-layout stencil kernels (set-up, edge indicator): gx
+// runs of kBlock points per (j,i) plane, gk planes, gl frames, in tiles of 32
+// runs x 8 planes x 1 frame per XCD residency round (4% faster than 8 x 8 x 4
+// for the set-up at 256x256x128x16); planes of fewer than 32 runs keep
+// 8 x 8 x 4, which leaves fewer idle blocks.
+PlaneMap plane_map(int gx, int gk, int gl) {
+    PlaneMap PM;
+    PM.gx = gx;
+    PM.gk = gk;
+    PM.gl = gl;
+    const bool wide = PM.gx >= 32;
+    PM.tx = wide ? 32 : 8;
+    PM.tk = 8;
+    PM.tl = wide ? 1 : 4;
+    PM.ntx = (PM.gx + PM.tx - 1) / PM.tx;
+    PM.ntk = (PM.gk + PM.tk - 1) / PM.tk;
+    PM.ntiles = PM.ntx * PM.ntk * ((PM.gl + PM.tl - 1) / PM.tl);
+    return PM;
+}
+
+unsigned plane_grid(const PlaneMap& PM) {
+    return (unsigned)(((PM.ntiles + 7) / 8) * 8 * PM.tx * PM.tk * PM.tl);
+}
 }  // namespace
 
 // Chunking of the two-level residual reduction over nb partials per colour:
@@ -149,6 +173,8 @@ void Engine::release() {
         op32_[c] = dr_[c] = nullptr;
     }
     if (db_) hipFree(db_);
+    if (hp_) hipFree(hp_);
+    hp_ = nullptr;
     if (prev2_) hipFree(prev2_);
     if (prev3_) hipFree(prev3_);
     prev2_ = prev3_ = nullptr;
@@ -335,21 +361,8 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
             return check(hipErrorUnknown, net_->err.c_str());
         warm = 2;
     }
-    PlaneMap PM;
-    PM.gx = (int)cdiv((long)I_ * J_, kBlock);
-    PM.gk = K_;
-    PM.gl = L_;
-    // tiles of 32 runs x 8 planes x 1 frame per XCD residency round (4% faster
-    // than 8 x 8 x 4 at 256x256x128x16); planes of fewer than 32 runs keep
-    // 8 x 8 x 4, which leaves fewer idle blocks
-    const bool wide = PM.gx >= 32;
-    PM.tx = wide ? 32 : 8;
-    PM.tk = 8;
-    PM.tl = wide ? 1 : 4;
-    PM.ntx = (PM.gx + PM.tx - 1) / PM.tx;
-    PM.ntk = (PM.gk + PM.tk - 1) / PM.tk;
-    PM.ntiles = PM.ntx * PM.ntk * ((PM.gl + PM.tl - 1) / PM.tl);
-    const dim3 grid((unsigned)(((PM.ntiles + 7) / 8) * 8 * PM.tx * PM.tk * PM.tl));
+    const PlaneMap PM = plane_map((int)cdiv((long)I_ * J_, kBlock), K_, L_);
+    const dim3 grid(plane_grid(PM));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof.on) {
         (void)hipEventCreate(&e0);
@@ -1178,27 +1191,56 @@ int Engine::presmooth(double* d, double sigma, double h, double omega, double gs
         if (sumsq_interior(full_[A_PREV], h, &S)) return -1;
         tol = rel2 * S;
     }
-    const dim3 gp(cdiv((long)H_ * (J_ - 2), kBlock), K_ - 2, L_ - 2);
-    const dim3 gr(cdiv((long)(I_ - 2) * (J_ - 2), kBlock), K_ - 2, L_ - 2);
-    const int np = (int)(gr.x * gr.y * gr.z);
-    CPtr8 none{};
-    int z = 0;
+    // right-hand side packed per colour in the interior-plane order
+    const long nhp = 2L * (K_ - 2) * (L_ - 2) * G.RP;
+    if (!hp_ && check(hipMalloc((void**)&hp_, (size_t)nhp * sizeof(double)), "hipMalloc heat prev"))
+        return -1;
+    const dim3 gfull(cdiv((long)I_ * J_, kBlock), K_, L_);
+    k_heat_pack<<<gfull, kBlock, 0, st_>>>(G, d, ur_[0], ur_[1], ub_, hp_);
+    if (launch_check("k_heat_pack")) return -1;
+    // red-black passes on the colour layout with the residual fused: red(z+1)
+    // yields the red terms of iteration z (its update is discarded when z is
+    // the last iteration: red ping-pong), black(z) the black terms of z.  The
+    // stopping test is the reference's `while (R > gsTol && z < cap)`
+    // (segmentation4D.c:405) on the host, with cap a double as there.
+    const SweepMap M = sweep_map();
+    const int nb = M.gx * M.gk;
+    if ((long)nb > npart_) return check(hipErrorInvalidValue, "presmooth partials");
+    int rnc = 0, rch = 0;
+    resid_chunking(nb, &rnc, &rch);
+    const dim3 grid((unsigned)sweep_grid(M));
+    double* part_r = part_;
+    double* part_b = part_ + npart_;
+    int z = 1;
     double R = 0.0;
-    do {
-        z = z + 1;
-        k_full_pass<true><<<gp, kBlock, 0, st_>>>(G, c, omega, 0, d, full_[A_PREV], none);
-        k_full_pass<true><<<gp, kBlock, 0, st_>>>(G, c, omega, 1, d, full_[A_PREV], none);
-        k_full_resid<true><<<gr, kBlock, 0, st_>>>(G, c, h4, d, full_[A_PREV], none, part_);
+    k_heat_sweep<0><<<grid, kBlock, 0, st_>>>(G, M, c, omega, h4, 0, ur_[0], ur_[1], ub_, hp_, nullptr);
+    k_heat_sweep<2><<<grid, kBlock, 0, st_>>>(G, M, c, omega, h4, 1, ub_, ub_, ur_[1], hp_, part_b);
+    for (;;) {
+        k_heat_sweep<1><<<grid, kBlock, 0, st_>>>(G, M, c, omega, h4, 0, ur_[z & 1],
+                                                   ur_[(z + 1) & 1], ub_, hp_, part_r);
+        k_resid_chunks<<<2 * rnc, 1024, 0, st_>>>(part_r, part_b, nb, rnc, rch, red2_, nullptr);
+        k_reduce_sum<false><<<1, 1024, 0, st_>>>(red2_, 2 * rnc, dscal_);
         if (launch_check("presmooth kernels")) return -1;
-        if (reduce_to_host(np, &R)) return -1;
-    } while (R > tol && z < cap);
+        if (check(hipMemcpyAsync(hscal_, dscal_, sizeof(double), hipMemcpyDeviceToHost, st_),
+                  "scalar copy") ||
+            check(hipStreamSynchronize(st_), "presmooth sync"))
+            return -1;
+        R = hscal_[0];
+        if (!(R > tol && z < cap)) break;
+        z = z + 1;
+        k_heat_sweep<2><<<grid, kBlock, 0, st_>>>(G, M, c, omega, h4, 1, ub_, ub_, ur_[z & 1],
+                                                   hp_, part_b);
+    }
+    // the accepted red values are red(z)'s; every padded point back to d
+    k_unpack<<<gfull, kBlock, 0, st_>>>(G, ur_[z & 1], ub_, d, nullptr);
+    if (launch_check("k_unpack heat")) return -1;
     if (iters) *iters = z;
     if (Rout) *Rout = R;
     return 0;
 }
 
 int Engine::gfun(const double* a, const double* b, double h, double K, double theta,
-                 double rho) {
+                 double rho, bool b_finite) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     const Scale S = make_scale(h);
     g_version_++;
@@ -1207,9 +1249,20 @@ int Engine::gfun(const double* a, const double* b, double h, double K, double th
         g.p[f] = full_[A_G0 + f];
         if (!g.p[f]) return check(hipErrorInvalidValue, "gfun: no g arrays");
     }
-    dim3 grid(cdiv((long)(I_ - 2) * (J_ - 2), kBlock), K_ - 2, L_ - 2);
-    if (is_pow2(h)) k_gfun<true><<<grid, kBlock, 0, st_>>>(G, S, K, theta, rho, a, b, g);
-    else k_gfun<false><<<grid, kBlock, 0, st_>>>(G, S, K, theta, rho, a, b, g);
+    // XCD-resident tiles over the interior (runs of kBlock points of the
+    // interior (j,i) plane, k - 1, l - 1), shaped as the set-up kernel's
+    const PlaneMap PM = plane_map(cdiv((long)(I_ - 2) * (J_ - 2), kBlock), K_ - 2, L_ - 2);
+    const dim3 grid(plane_grid(PM));
+    // the second image's term can be dropped only where it is known to be a
+    // signed zero: rho == 0 and every value of b finite
+    const bool both = !(rho == 0.0 && b_finite);
+    if (is_pow2(h)) {
+        if (both) k_gfun<true, true><<<grid, kBlock, 0, st_>>>(G, PM, S, K, theta, rho, a, b, g);
+        else k_gfun<true, false><<<grid, kBlock, 0, st_>>>(G, PM, S, K, theta, rho, a, b, g);
+    } else {
+        if (both) k_gfun<false, true><<<grid, kBlock, 0, st_>>>(G, PM, S, K, theta, rho, a, b, g);
+        else k_gfun<false, false><<<grid, kBlock, 0, st_>>>(G, PM, S, K, theta, rho, a, b, g);
+    }
     return launch_check("k_gfun");
 }
 

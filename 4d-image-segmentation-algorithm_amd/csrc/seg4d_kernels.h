@@ -1047,6 +1047,148 @@ static __global__ __launch_bounds__(kBlock) void k_full_resid(Geo G, double c, d
 }
 
 // ----------------------------------------------------------------------------
+// Implicit heat step (presmoothing, segmentation4D.c:337-409) on the colour
+// layout.  k_heat_pack copies the reflected input into the colour arrays
+// (both red buffers, every padded point: the ghosts stay frozen) and its
+// interior into hp, the right-hand side prev in the interior-plane order of
+// the operand blocks: hp[colour][plane][t], t = (j-1) H + m.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ long heat_prev_index(const Geo& G, int c, int k, int l, int t) {
+    const long nplane = (long)(G.K - 2) * (G.L - 2);
+    return ((long)c * nplane + (long)(l - 1) * (G.K - 2) + (k - 1)) * G.RP + t;
+}
+
+static __global__ __launch_bounds__(kBlock) void k_heat_pack(Geo G, const double* __restrict__ d,
+                                                     double* __restrict__ ur0,
+                                                     double* __restrict__ ur1,
+                                                     double* __restrict__ ub,
+                                                     double* __restrict__ hp) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int k = blockIdx.y, l = blockIdx.z;
+    if (t >= G.I * G.J) return;
+    const int j = t / G.I, i = t - j * G.I;
+    const int c = (i + j + k + l) & 1;
+    const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
+    const double v = d[((l * G.K + k) * G.J + j) * G.I + i];
+    if (c == 0) {
+        ur0[pp] = v;
+        ur1[pp] = v;
+    } else {
+        ub[pp] = v;
+    }
+    if (is_interior(G, i, j, k, l)) hp[heat_prev_index(G, c, k, l, (j - 1) * G.H + (i >> 1))] = v;
+}
+
+// One colour pass of the heat step, one thread per packed column m of rows
+// (j, k), marching l with the other colour at l-1, l, l+1 in registers (the
+// launch map of the SOR passes).  Neighbour sum in the reference's order
+// +x, -x, -y, +y, -z, +z, -t, +t (segmentation4D.c:362-369); residual
+// u (1 + 8c) - c sum - prev (:392-401) before (RESID 1: red pass z+1 yields
+// the red terms of iteration z) or after (RESID 2: black pass of z) the
+// update, summed per block.
+// Loads of one heat-pass point (issued for two l-steps before their arithmetic).
+struct HeatIn {
+    double yo, uS, uN, uB, uT, pv, uo;
+    int pp, sft;
+    bool act;
+};
+
+template <int RESID>
+__device__ __forceinline__ void heat_update(const HeatIn& P, double wb, double wc, double wf,
+                                            double c, double om, double h4, double* xout,
+                                            double& acc) {
+    if (!P.act) return;
+    const double uW = P.sft ? wc : P.yo;
+    const double uE = P.sft ? P.yo : wc;
+    double nb = uE + uW;
+    nb = nb + P.uS;
+    nb = nb + P.uN;
+    nb = nb + P.uB;
+    nb = nb + P.uT;
+    nb = nb + wb;
+    nb = nb + wf;
+    const double D = 1 + c * 8;
+    if constexpr (RESID == 1) {
+        const double r = P.uo * D - c * nb - P.pv;
+        acc += r * r * h4;
+    }
+    const double gs = (P.pv + c * nb) / D;
+    const double un = P.uo + om * (gs - P.uo);
+    __builtin_nontemporal_store(un, xout + P.pp);
+    if constexpr (RESID == 2) {
+        const double r = un * D - c * nb - P.pv;
+        acc += r * r * h4;
+    }
+}
+
+template <int RESID>
+static __global__ __launch_bounds__(kBlock) void k_heat_sweep(
+    Geo G, SweepMap M, double c, double om, double h4, int colour, const double* xin,
+    double* xout, const double* __restrict__ y, const double* __restrict__ hp,
+    double* __restrict__ partials) {
+    int bx, kb;
+    if (!map_block(M, blockIdx.x, gridDim.x, bx, kb)) return;
+    const int lb = (kb - 1) * M.gx + bx;
+    const int kgrp = (kb - 1) % M.gk0, lch = (kb - 1) / M.gk0;
+    const int l0 = 1 + lch * M.lc;
+    const int per = kBlock / M.kp;
+    const int sub = threadIdx.x / per;
+    const int kk = kgrp * M.kp + sub + 1;
+    const bool kok = kk <= G.K - 2;
+    const int k = kok ? kk : G.K - 2;
+    const int t = bx * per + (threadIdx.x - sub * per);
+    const int jr = t / G.H;
+    const int m = t - jr * G.H;
+    const int j = jr + 1;
+    const bool rowok = kok && j <= G.J - 2;
+    const int base = (k * G.J + j) * G.H + m;
+    double acc = 0.0;
+    // w[q]: the other colour at column m, frame l - 1 + q
+    double w[4];
+    w[0] = rowok ? y[base + (l0 - 1) * G.pw] : 0.0;
+    w[1] = rowok ? y[base + l0 * G.pw] : 0.0;
+    auto load = [&](int l, HeatIn& P) {
+        const int pp = base + l * G.pw;
+        P.pp = pp;
+        P.sft = colour ^ ((j + k + l) & 1);
+        const int i = 2 * m + P.sft;
+        P.act = rowok && i >= 1 && i <= G.I - 2;
+        if (P.act) {
+            P.yo = y[pp + (P.sft ? 1 : -1)];
+            P.uS = y[pp - G.py];
+            P.uN = y[pp + G.py];
+            P.uB = y[pp - G.pz];
+            P.uT = y[pp + G.pz];
+            P.pv = hp[heat_prev_index(G, colour, k, l, t)];
+            P.uo = xin[pp];
+        }
+    };
+    const int lmax = min(l0 + M.lc - 1, G.L - 2);
+    int l = l0;
+    for (; l + 1 <= lmax; l += 2) {
+        HeatIn P0, P1;
+        w[2] = rowok ? y[base + (l + 1) * G.pw] : 0.0;
+        w[3] = rowok ? y[base + (l + 2) * G.pw] : 0.0;
+        load(l, P0);
+        load(l + 1, P1);
+        heat_update<RESID>(P0, w[0], w[1], w[2], c, om, h4, xout, acc);
+        heat_update<RESID>(P1, w[1], w[2], w[3], c, om, h4, xout, acc);
+        w[0] = w[2];
+        w[1] = w[3];
+    }
+    if (l <= lmax) {
+        HeatIn P0;
+        w[2] = rowok ? y[base + (l + 1) * G.pw] : 0.0;
+        load(l, P0);
+        heat_update<RESID>(P0, w[0], w[1], w[2], c, om, h4, xout, acc);
+    }
+    if constexpr (RESID != 0) {
+        const double bs = block_sum256(acc);
+        if (threadIdx.x == 0) partials[lb] = bs;
+    }
+}
+
+// ----------------------------------------------------------------------------
 // Ghost shell by clamped-index copy (closed form of the sequential X,Y,Z,W
 // passes of segmentation4D.c:826-864).  Grid x over padded (j,i), y=k, z=l.
 // ----------------------------------------------------------------------------
@@ -1066,25 +1208,110 @@ static __global__ __launch_bounds__(kBlock) void k_reflect(Geo G, double* u) {
 }
 
 // ----------------------------------------------------------------------------
+// Edge indicator
+// ----------------------------------------------------------------------------
+// The 27 points of the 33-point stencil that the four forward faces (+x, +y,
+// +z, +t) use: the centre, p + e_A, p - e_B, p + e_A + e_B, p + e_A - e_B.
+__device__ __forceinline__ void load_stencil_fwd(const double* __restrict__ u, int p,
+                                                 const Geo& G, Stencil33& T) {
+    const int st[4] = {1, G.sy, G.sz, G.sw};
+#pragma unroll
+    for (int w = 0; w < 3; w++)
+#pragma unroll
+        for (int z = 0; z < 3; z++)
+#pragma unroll
+            for (int y = 0; y < 3; y++)
+#pragma unroll
+                for (int x = 0; x < 3; x++) {
+                    const int o[4] = {x - 1, y - 1, z - 1, w - 1};
+                    const int nz = (o[0] != 0) + (o[1] != 0) + (o[2] != 0) + (o[3] != 0);
+                    const int neg = (o[0] < 0) + (o[1] < 0) + (o[2] < 0) + (o[3] < 0);
+                    // at most two nonzero offsets, and never two negative ones
+                    if (nz <= 2 && !(nz == 2 && neg == 2))
+                        T.s[w][z][y][x] = u[p + o[0] * st[0] + o[1] * st[1] + o[2] * st[2] +
+                                            o[3] * st[3]];
+                }
+}
+
 // Edge indicator g on 8 faces (segmentation4D.c:582-823 after presmoothing):
 // g_f = 1/(1 + K (theta |grad I0|_f^2 + rho |grad Ithr|_f^2)).
-// Grid x over interior (j,i), y = k-1, z = l-1.
-// ----------------------------------------------------------------------------
-template <bool POW2>
-static __global__ __launch_bounds__(kBlock) void k_gfun(Geo G, Scale S, double K_, double theta,
-                                                double rho, const double* __restrict__ a,
+//
+// A thread forms only the four forward faces of its point from registers
+// (27-point stencil) and writes each twice: as face +A of p and as face -A
+// of p + e_A, whose gradient components are the same sums with the addends
+// swapped or negated (squares and commuted additions are bitwise equal), so
+// g_w(p + x) = g_e(p) etc. bit for bit.  Points on the low boundary of an
+// axis form their backward face themselves; ghost points get no g (the
+// reference loops over the interior only).  BOTH = false drops the second
+// image: when rho == 0 and that image is finite, rho * vb is a signed zero
+// and theta * va + (+-0) gives g bit for bit (1 + K * (+-0) = 1).
+// Blocks are mapped to XCD-resident tiles (PlaneMap) over the interior
+// (run, k - 1, l - 1), so the neighbour rows and planes a tile re-reads stay
+// in one L2.
+template <bool POW2, bool BOTH>
+static __global__ __launch_bounds__(kBlock) void k_gfun(Geo G, PlaneMap PM, Scale S, double K_,
+                                                double theta, double rho,
+                                                const double* __restrict__ a,
                                                 const double* __restrict__ b, Ptr8 g) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int k = blockIdx.y + 1, l = blockIdx.z + 1;
+    int bx, kk, ll;
+    if (!map_plane_block(PM, blockIdx.x, bx, kk, ll)) return;
+    const int t = bx * kBlock + threadIdx.x;
     const int w = G.I - 2;
     if (t >= w * (G.J - 2)) return;
     const int j = t / w + 1, i = t - (j - 1) * w + 1;
+    const int k = kk + 1, l = ll + 1;
     const int p = ((l * G.K + k) * G.J + j) * G.I + i;
-    double va[8], vb[8];
-    all_face_sq<POW2>(a, p, G, S, va);
-    all_face_sq<POW2>(b, p, G, S, vb);
+    const int st[4] = {1, G.sy, G.sz, G.sw};
+    const int co[4] = {i, j, k, l};
+    const int hi[4] = {G.I - 2, G.J - 2, G.K - 2, G.L - 2};
+    double va[4], vb[4];
+    {
+        Stencil33 T;
+        load_stencil_fwd(a, p, G, T);
+        va[0] = face_sq_reg<0, POW2>(T, S);
+        va[1] = face_sq_reg<2, POW2>(T, S);
+        va[2] = face_sq_reg<4, POW2>(T, S);
+        va[3] = face_sq_reg<6, POW2>(T, S);
+    }
+    if constexpr (BOTH) {
+        Stencil33 T;
+        load_stencil_fwd(b, p, G, T);
+        vb[0] = face_sq_reg<0, POW2>(T, S);
+        vb[1] = face_sq_reg<2, POW2>(T, S);
+        vb[2] = face_sq_reg<4, POW2>(T, S);
+        vb[3] = face_sq_reg<6, POW2>(T, S);
+    }
+    // streaming stores (the 64 bytes of g per point are the bulk of the
+    // traffic; kept out of L2 they leave it to the stencil's re-reads:
+    // 4.38 -> 3.48 ms at 256x256x128x16)
 #pragma unroll
-    for (int f = 0; f < 8; f++) g.p[f][p] = 1.0 / (1 + K_ * (theta * va[f] + rho * vb[f]));
+    for (int A = 0; A < 4; A++) {
+        double gv;
+        if constexpr (BOTH) gv = 1.0 / (1 + K_ * (theta * va[A] + rho * vb[A]));
+        else gv = 1.0 / (1 + K_ * (theta * va[A]));
+        __builtin_nontemporal_store(gv, g.p[2 * A] + p);
+        if (co[A] < hi[A]) __builtin_nontemporal_store(gv, g.p[2 * A + 1] + p + st[A]);
+    }
+    // backward faces of the low boundary (their west-type neighbour is a ghost)
+#pragma unroll
+    for (int A = 0; A < 4; A++) {
+        if (co[A] != 1) continue;
+        double wa, wb = 0.0;
+        if (A == 0) wa = face_sq<1, POW2>(a, p, G, S);
+        else if (A == 1) wa = face_sq<3, POW2>(a, p, G, S);
+        else if (A == 2) wa = face_sq<5, POW2>(a, p, G, S);
+        else wa = face_sq<7, POW2>(a, p, G, S);
+        if constexpr (BOTH) {
+            if (A == 0) wb = face_sq<1, POW2>(b, p, G, S);
+            else if (A == 1) wb = face_sq<3, POW2>(b, p, G, S);
+            else if (A == 2) wb = face_sq<5, POW2>(b, p, G, S);
+            else wb = face_sq<7, POW2>(b, p, G, S);
+            __builtin_nontemporal_store(1.0 / (1 + K_ * (theta * wa + rho * wb)),
+                                        g.p[2 * A + 1] + p);
+        } else {
+            __builtin_nontemporal_store(1.0 / (1 + K_ * (theta * wa)), g.p[2 * A + 1] + p);
+        }
+    }
 }
 
 // ----------------------------------------------------------------------------
