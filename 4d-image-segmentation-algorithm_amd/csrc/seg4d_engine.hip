@@ -323,7 +323,8 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
     // corrects the extrapolation of its network's base order (linear or
     // cubic) and keeps the history that base needs; until that history
     // exists it starts like mode 4.
-    const int nbase = (warm_ == 2 && net_ && net_->ready()) ? net_->base() : 0;
+    const bool use_net = warm_ == 2 && net_ && net_->ready();
+    const int nbase = use_net ? net_->base() : -1;
     const bool keep2 = (warm_ >= 3 || nbase == 3) && write_prev;  // history 2 / 3 levels deep
     const bool keep3 = (warm_ == 4 || nbase == 3) && write_prev;
     int warm = 0;
@@ -353,11 +354,14 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
         }
     }
     LearnedStart ls{nullptr, nullptr, 0.0, 1.f, 1};
-    if (nbase > 0 && hist_ok_ && write_prev && (nbase == 1 || (h2_valid && h3_valid))) {
+    // order 0 needs no history (start u^n + c / s_t); orders 1 and 3 their
+    // levels u^(n-1) .. u^(n-base)
+    if (use_net && write_prev &&
+        (nbase == 0 || (hist_ok_ && (nbase == 1 || (h2_valid && h3_valid))))) {
         // the ring is read before the set-up kernel rotates it; the kernel
-        // forms base + sigma c / s_t from the network's output
-        if (net_->predict(*this, full_[A_PREV], nbase == 3 ? prev2_ : nullptr,
-                          nbase == 3 ? prev3_ : nullptr, &ls))
+        // forms base + q c / s_t from the network's output
+        if (net_->predict(*this, nbase >= 1 ? full_[A_PREV] : nullptr,
+                          nbase == 3 ? prev2_ : nullptr, nbase == 3 ? prev3_ : nullptr, &ls))
             return check(hipErrorUnknown, net_->err.c_str());
         warm = 2;
     }
@@ -870,11 +874,11 @@ int Engine::solve_mixed(const StepParams& P, SolveStats* out) {
 int Engine::learned_preview(float* host_c, double* host_q) {
     if (!net_ || !net_->ready()) return check(hipErrorInvalidValue, "learned preview: no network");
     const int nb = net_->base();
-    if (!hist_ok_ || (nb == 3 && !(hist2_ok_ && hist3_ok_) ))
+    if ((nb >= 1 && !hist_ok_) || (nb == 3 && !(hist2_ok_ && hist3_ok_)))
         return check(hipErrorInvalidValue, "learned preview: the network's history does not exist");
     LearnedStart ls{nullptr, nullptr, 0.0, 1.f, 1};
-    if (net_->predict(*this, full_[A_PREV], nb == 3 ? prev2_ : nullptr, nb == 3 ? prev3_ : nullptr,
-                      &ls))
+    if (net_->predict(*this, nb >= 1 ? full_[A_PREV] : nullptr, nb == 3 ? prev2_ : nullptr,
+                      nb == 3 ? prev3_ : nullptr, &ls))
         return check(hipErrorUnknown, net_->err.c_str());
     double sum = 0.0;
     if (check(hipMemcpyAsync(host_c, ls.out, (size_t)n_ * sizeof(float), hipMemcpyDeviceToHost, st_),

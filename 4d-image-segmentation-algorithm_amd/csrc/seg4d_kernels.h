@@ -291,7 +291,8 @@ __device__ __forceinline__ bool map_plane_block(const PlaneMap& M, int b, int& b
 
 // warm: 0 = SOR starts from u^n (the reference); 1 = it starts from the
 // linear extrapolation 2 u^n - u^(n-1) (prev_full holds u^(n-1) on entry);
-// 2 = from the learned start (interior points, LearnedStart); 3 = from the
+// 2 = from the learned start (interior points, LearnedStart: the network's
+// correction of u^n, of the linear or of the cubic extrapolation); 3 = from the
 // quadratic extrapolation 3 u^n - 3 u^(n-1) + u^(n-2) (prev2 holds u^(n-2));
 // 4 = from the cubic one 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) (prev3
 // holds u^(n-3)).  u^n is not copied: the host rotates the full-layout
@@ -330,7 +331,8 @@ static __global__ __launch_bounds__(kBlock) void k_setup_step(
         if (warm == 3 && in) start = (3.0 * uc - 3.0 * pf) + p2;
         if (cub && in) start = ((4.0 * uc - 6.0 * pf) + 4.0 * p2) - p3;
         if (warm == 2 && in) {
-            const double base = ls.order == 3 ? start : uc + (uc - pf);
+            // the extrapolation the network corrects: cubic, linear or none
+            const double base = ls.order == 3 ? start : ls.order == 1 ? uc + (uc - pf) : uc;
             start = base + sqrt(*ls.dsum * ls.inv_n) * (double)(ls.out[p] * ls.inv_st);
         }
         if constexpr (MIXED) {

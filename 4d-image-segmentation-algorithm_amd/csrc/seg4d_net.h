@@ -10,11 +10,13 @@
 //   d = u^n - u^(n-1), sigma = rms of d over the interior (this step's motion)
 //   t = u^n - 3u^(n-1) + 3u^(n-2) - u^(n-3) (third difference, the error the
 //       quadratic extrapolation made at the last step), kappa = its rms
-//   base = the extrapolation the network corrects: linear 2u^n - u^(n-1)
-//          (base order 1, scale q = sigma) or cubic 4u^n - 6u^(n-1) +
-//          4u^(n-2) - u^(n-3) (base order 3, scale q = kappa)
+//   base = what the network corrects: u^n itself (base order 0, scale q = 1:
+//          the inputs are u^n and g only), the linear extrapolation
+//          2u^n - u^(n-1) (order 1, scale q = sigma) or the cubic
+//          4u^n - 6u^(n-1) + 4u^(n-2) - u^(n-3) (order 3, scale q = kappa)
 //   features x0 = [u^n, s_d d / sigma, g_mean, s_e t / kappa, 0 x 4]
-//                 (8 channels; channel 3 is zero for order 1)
+//                 (8 channels; channel 3 is zero below order 3, channel 1
+//                 at order 0)
 //   `layers` x { 3x3x3 spatial conv -> 3-tap temporal conv + bias -> ReLU },
 //   16 channels (1 to 4 layers)
 //   1x1 projection -> c;  start = base + q c / s_t
@@ -63,7 +65,7 @@ public:
     Net& operator=(const Net&) = delete;
 
     // He-initialised weights from `seed` for `layers` (1..4) layers and a
-    // base extrapolation of order `base` (1 or 3); empties the training pool.
+    // base of order `base` (0, 1 or 3); empties the training pool.
     int init(uint64_t seed, int layers = 4, int base = 1);
     int layers() const { return nlayers_; }
     int base() const { return base_; }
@@ -126,7 +128,7 @@ public:
 
 private:
     int nlayers_ = 4;  // layers of the current network
-    int base_ = 1;     // order of the extrapolation it corrects (1 or 3)
+    int base_ = 1;     // order of the base it corrects (0: u^n, 1 linear, 3 cubic)
     int check(hipError_t e, const char* what);
     int layout();
     int ensure_acts(int I, int J, int K, int L, bool training);

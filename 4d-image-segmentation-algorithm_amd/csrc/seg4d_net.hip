@@ -141,7 +141,8 @@ __global__ __launch_bounds__(256) void k_features(NGeo G, int order, const doubl
 #pragma unroll
         for (int q = 0; q < 4; q++) f[q] = (bf16)0.0f;
         if (point_of(G, v, i, j, k, l)) {
-            const double uc = u[v], p1 = h1[v];
+            // order 0: no history, the increment feature is zero
+            const double uc = u[v], p1 = order >= 1 ? h1[v] : uc;
             const double d = uc - p1;
             f[0] = (bf16)(float)uc;
             f[1] = (bf16)(float)d;
@@ -287,11 +288,13 @@ __global__ __launch_bounds__(256) void k_gather_patch(
     unsigned char lm = 0;
     if (!ghost) {
         const long rv = (((long)pl * RK + (az + pk)) * RJ + (ay + pj)) * RI + (ax + pi);
-        const double uc = u0[rv], um = um1[rv], up = up1[rv];
+        const double uc = u0[rv], um = order >= 1 ? um1[rv] : uc, up = up1[rv];
         f[0] = (bf16)(float)uc;
         f[1] = (bf16)((float)((uc - um) * inv_sig) * sd);
         f[2] = gm[rv];
-        if (order == 3) {
+        if (order == 0) {
+            tg = (float)(up - uc) * st;  // the increment itself (scale q = 1)
+        } else if (order == 3) {
             const double p2 = um2[rv], p3 = um3[rv];
             const double base = ((4.0 * uc - 6.0 * um) + 4.0 * p2) - p3;
             f[3] = (bf16)((float)(third_diff(uc, um, p2, p3) * inv_kap) * se);
@@ -323,9 +326,9 @@ __global__ __launch_bounds__(256) void k_moments(NGeo G, int order, const double
     double a = 0, e2 = 0, c = 0;
     int i, j, k, l;
     if (v < (long)G.I * G.J * G.K * G.L && point_of(G, v, i, j, k, l)) {
-        const double uc = u0[v], um = um1[v];
+        const double uc = u0[v], um = order >= 1 ? um1[v] : uc;
         const double d = uc - um;
-        double base = uc + d, t3 = 0.0;
+        double base = order >= 1 ? uc + d : uc, t3 = 0.0;
         if (order == 3) {
             base = ((4.0 * uc - 6.0 * um) + 4.0 * um2[v]) - um3[v];
             t3 = third_diff(uc, um, um2[v], um3[v]);
@@ -482,8 +485,8 @@ void Net::release() {
 
 int Net::init(uint64_t seed, int layers, int base) {
     release();
-    if (layers < 1 || layers > kMaxLayers || (base != 1 && base != 3)) {
-        err = "net: layers must be 1..4 and the base order 1 or 3";
+    if (layers < 1 || layers > kMaxLayers || (base != 0 && base != 1 && base != 3)) {
+        err = "net: layers must be 1..4 and the base order 0, 1 or 3";
         return -1;
     }
     nlayers_ = layers;
@@ -533,6 +536,12 @@ int Net::init(uint64_t seed, int layers, int base) {
         check(hipMemset(wb_, 0, nb * 2), "net memset") ||
         check(hipMemset(wtr_, 0, nb * 2), "net memset"))
         return -1;
+    {
+        // dscal_[10] = 1: the target scale q of the base order 0
+        const double one = 1.0;
+        if (check(hipMemcpy(dscal_ + 10, &one, sizeof one, hipMemcpyHostToDevice), "net const"))
+            return -1;
+    }
     s_d = s_t = s_e = 1.f;
     adam_t_ = 0;
     return refresh();
@@ -793,7 +802,8 @@ int Net::compute_scales(hipStream_t st) {
         for (size_t n = (size_t)base_; n + 1 < r.snaps.size(); n++) {
             const double* um3 = base_ == 3 ? r.snaps[n - 3] : nullptr;
             const double* um2 = base_ == 3 ? r.snaps[n - 2] : nullptr;
-            k_moments<<<nb, 256, 0, st>>>(G, base_, um3, um2, r.snaps[n - 1], r.snaps[n],
+            k_moments<<<nb, 256, 0, st>>>(G, base_, um3, um2,
+                                          base_ >= 1 ? r.snaps[n - 1] : nullptr, r.snaps[n],
                                           r.snaps[n + 1], part);
             k_reduce_d<<<1, 256, 0, st>>>(part, nb, 3, dscal_);
             if (check(hipMemcpyAsync(hscal_, dscal_, 3 * sizeof(double), hipMemcpyDeviceToHost, st),
@@ -808,16 +818,16 @@ int Net::compute_scales(hipStream_t st) {
             const double k2 = r.kappa[n] > 0 ? r.kappa[n] * r.kappa[n] : 1.0;
             sd += hscal_[0] / s2;
             se += hscal_[1] / k2;
-            sc += hscal_[2] / (base_ == 3 ? k2 : s2);
+            sc += hscal_[2] / (base_ == 3 ? k2 : base_ == 1 ? s2 : 1.0);
             npts += (long)(r.I - 2) * (r.J - 2) * (r.K - 2) * (r.L - 2);
         }
         (void)hipFree(part);
     }
-    if (npts == 0 || !(sd > 0) || !(sc > 0)) {
+    if (npts == 0 || (base_ >= 1 && !(sd > 0)) || !(sc > 0)) {
         err = "net: training pool needs runs with motion and enough time levels for the base";
         return -1;
     }
-    s_d = (float)(1.0 / std::sqrt(sd / npts));
+    s_d = sd > 0 ? (float)(1.0 / std::sqrt(sd / npts)) : 1.f;
     s_e = se > 0 ? (float)(1.0 / std::sqrt(se / npts)) : 1.f;
     s_t = (float)(1.0 / std::sqrt(sc / npts));
     return 0;
@@ -869,7 +879,8 @@ int Net::train(int iters, int batch, float lr, uint64_t seed, int log_every, dou
             const double* um3 = base_ == 3 ? r.snaps[n - 3] : nullptr;
             const double* um2 = base_ == 3 ? r.snaps[n - 2] : nullptr;
             k_gather_patch<<<(unsigned)cdivl(per, 256), 256, 0, st>>>(
-                G, b, r.I, r.J, r.K, base_, um3, um2, r.snaps[n - 1], r.snaps[n], r.snaps[n + 1],
+                G, b, r.I, r.J, r.K, base_, um3, um2, base_ >= 1 ? r.snaps[n - 1] : nullptr,
+                r.snaps[n], r.snaps[n + 1],
                 r.gmean, ax, ay, az, s_d, s_e, s_t, r.sigma[n] > 0 ? 1.0 / r.sigma[n] : 0.0,
                 r.kappa[n] > 0 ? 1.0 / r.kappa[n] : 0.0, feat_, target_, lmask_);
             auto cnt = [&](int a, int rn) {
@@ -967,7 +978,7 @@ int Net::predict(Engine& E, const double* h1, const double* h2, const double* h3
         err = "net: not initialised";
         return -1;
     }
-    if (!h1 || (base_ == 3 && (!h2 || !h3))) {
+    if ((base_ >= 1 && !h1) || (base_ == 3 && (!h2 || !h3))) {
         err = "net: the prediction needs the base's history levels";
         return -1;
     }
@@ -1022,8 +1033,10 @@ int Net::predict(Engine& E, const double* h1, const double* h2, const double* h3
                                                               s_d, s_e, w0s_);
     if (forward(G, E.st_, false, w0s_)) return -1;
     ls->out = out_;
-    ls->dsum = base_ == 3 ? dsum + 1 : dsum;  // the target's scale: kappa or sigma
-    ls->inv_n = inv_n;
+    // the target's scale: kappa (order 3), sigma (order 1), or 1 (order 0:
+    // dscal_[10] holds 1.0, set at init)
+    ls->dsum = base_ == 3 ? dsum + 1 : base_ == 1 ? dsum : dscal_ + 10;
+    ls->inv_n = base_ == 0 ? 1.0 : inv_n;
     ls->inv_st = 1.f / s_t;
     ls->order = base_;
     (void)hipEventRecord(e1, E.st_);

@@ -371,11 +371,12 @@ class SegLib(_Lib):
     # -- learned warm start ---------------------------------------------------
     def net_init(self, seed=0, layers=None, base=None):
         """Fresh network weights; `layers` (1..4, default 4) layers correcting
-        the extrapolation of order `base` (1 linear, default; 3 cubic)."""
+        the base of order `base` (0: u^n itself; 1 linear, default; 3 cubic)."""
         if layers is None and base is None:
             rc = self.lib.seg4dNetInit(int(seed))
         else:
-            rc = self.lib.seg4dNetInitShape(int(seed), int(layers or 4), int(base or 1))
+            rc = self.lib.seg4dNetInitShape(int(seed), 4 if layers is None else int(layers),
+                                            1 if base is None else int(base))
         if rc != 0:
             raise RuntimeError(f"seg4dNetInit failed: {self.last_error()}")
 

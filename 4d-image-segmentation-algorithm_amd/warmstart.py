@@ -102,7 +102,7 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--layers", type=int, default=1)
-    ap.add_argument("--base", type=int, default=3, choices=(1, 3))
+    ap.add_argument("--base", type=int, default=3, choices=(0, 1, 3))
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     a = ap.parse_args()
     sys.path.insert(0, PKG_DIR)

@@ -23,10 +23,14 @@ Reported:
                  plus a fixed number of SOR iterations of step 1, extrapolated
                  with the GPU run's mean sweeps per step.
 
-The same 30 steps are then run in the FP32-refined form (FP32 SOR inside FP64
-defect correction, same FP64 stopping test) from u^n, from the linear and the
-cubic extrapolation of the previous solutions, and from the learned network's
-prediction, each reported with its time, sweeps, max|u - u_FP64| and mask IoU.
+The same time steps are then run in the FP32-refined form (FP32 SOR inside
+FP64 defect correction, same FP64 stopping test) from u^n, from the linear and
+the cubic extrapolation of the previous solutions, and from the learned
+network's prediction, each reported with its time, sweeps, max|u - u_FP64| and
+mask IoU; then three non-parity FP64 controls (SOR factor 1.2, the cubic
+start, both) that separate what the factor, the start and FP32 contribute;
+then a few steps in mirror mode (the drop-in default, PCIe-inclusive).  The
+edge indicator is timed both PCIe-inclusive (mirror mode) and resident.
 
 Usage: python bench.py [--steps 30] [--warmup 1] [--no-cpu] [--cpu-iters 10]
 """
@@ -77,16 +81,34 @@ def params(lib):
     lib.set_relative_tolerance(1e-6)
 
 
+def host_cores():
+    """(physical cores, logical CPUs) this process may run on, from the
+    kernel's CPU topology (distinct core_id per package)."""
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = list(range(os.cpu_count() or 1))
+    phys = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology"
+        try:
+            pk = open(f"{base}/physical_package_id").read().strip()
+            co = open(f"{base}/core_id").read().strip()
+            phys.add((pk, co))
+        except OSError:
+            phys.add(("?", str(c)))
+    return len(phys), len(cpus)
+
+
 def cpu_baseline(lib, shape, u0, iters, mean_sweeps, nint):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from harness import Oracle, params as oparams, zeros
 
     O = Oracle()
-    try:
-        avail = len(os.sched_getaffinity(0))
-    except AttributeError:
-        avail = os.cpu_count() or 1
-    threads = max(1, min(avail, 16, shape[0] - 2))
+    phys, logical = host_cores()
+    # the reference OpenMP program parallelises the t loop: at most lMax - 2
+    # threads do work; one thread per physical core, at most 16
+    threads = max(1, min(phys, 16, shape[0] - 2))
     p = oparams(shape, h=1.0, tau=0.5, eps2=1e-6, omega_c=1.8, gsTol=-1.0,
                 maxNoGSIteration=iters, nthreads=threads, par_reduce=1)
     g = [np.ascontiguousarray(lib.arr("g_" + f)) for f in ("e", "w", "n", "s", "t", "b", "fw", "bw")]
@@ -104,6 +126,8 @@ def cpu_baseline(lib, shape, u0, iters, mean_sweeps, nint):
         "value": nint * mean_sweeps / step_s,
         "unit": "voxel-updates/s",
         "cores": threads,
+        "physical_cores_available": phys,
+        "logical_cpus_available": logical,
         "kind": "port",
         "sample": (f"oracle OpenMP restatement of the reference path: 1 coefficient pass "
                    f"({t_coef:.2f} s) + {iters} SOR iterations ({t_iter:.3f} s each) of "
@@ -136,9 +160,9 @@ def main():
                     help="interior size nx ny nz nt of the training runs (default: warmstart.py's)")
     ap.add_argument("--net-layers", type=int, default=1,
                     help="layers of a network trained with --train-net")
-    ap.add_argument("--net-base", type=int, default=3, choices=(1, 3),
-                    help="order of the extrapolation a network trained with --train-net "
-                         "corrects (1 linear, 3 cubic)")
+    ap.add_argument("--net-base", type=int, default=3, choices=(0, 1, 3),
+                    help="base a network trained with --train-net corrects (0: u^n itself, "
+                         "inputs u^n and g; 1: linear, 3: cubic extrapolation)")
     ap.add_argument("--inner-omega", type=float, default=None,
                     help="SOR factor of the FP32 correction solves (default: omega_c)")
     ap.add_argument("--batch", action="store_true",
@@ -147,6 +171,11 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-sequential", action="store_true",
                     help="batched workload: skip the one-volume-at-a-time comparison")
+    ap.add_argument("--no-controls", action="store_true",
+                    help="skip the non-parity control runs (FP64 at omega 1.2 / from the "
+                         "cubic start) that attribute the refined forms' speed-up")
+    ap.add_argument("--mirror-steps", type=int, default=2,
+                    help="time steps timed in mirror mode (host arrays, PCIe) after the runs")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one device copy of u (padded volume, "
                          "FP64) as the known-size reference for rocprofv3 --pmc byte counts")
@@ -190,13 +219,27 @@ def main():
     lib.arr("Mmask")[...] = 1.0
     log(f"[bench] generated in {time.perf_counter() - t:.2f} s")
 
-    # edge indicator (mirror mode: includes the host<->device copies)
+    # edge indicator, as a reference driver gets it (mirror mode: the host
+    # arrays in and the presmoothed images, prev_u and the 8 g arrays out
+    # over PCIe), then again with the arrays resident (device time)
+    img0 = lib.arr("i0").copy()
     t = time.perf_counter()
     lib.gf("i0", "u")
     edge_s = time.perf_counter() - t
-    log(f"[bench] edge indicator {edge_s:.2f} s (PCIe-inclusive)")
-
+    edge_iters = lib.lib.seg4dLastSweeps()
     lib.set_resident(True)
+    lib.arr("i0")[...] = img0
+    lib.arr("u")[...] = 0.0
+    lib.upload("i0")
+    lib.upload("u")
+    lib.synchronize()
+    t = time.perf_counter()
+    lib.gf("i0", "u")
+    lib.synchronize()
+    edge_dev_s = time.perf_counter() - t
+    del img0
+    log(f"[bench] edge indicator {edge_s:.2f} s PCIe-inclusive, {1e3 * edge_dev_s:.1f} ms resident")
+
     lib.arr("u")[...] = u0
     lib.upload("u")
     lib.upload("Mmask")
@@ -235,22 +278,24 @@ def main():
     thr = 0.5 * (inner.max() + inner.min())
     mask64 = inner >= thr
     mask_frac = float(mask64.mean())
+    u64 = inner.copy()
 
-    def run_refined(warm):
-        """The same time steps in the FP32-refined form from the same u0;
-        warm = 1 starts every SOR solve from 2 u^n - u^(n-1), 4 from the cubic
-        extrapolation 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) (lower orders
-        while the history is shorter), 2 from the learned network's
-        prediction (from the second step on)."""
-        u64 = inner.copy()
-        lib.set_precision(1)
+    def run_form(precision, warm, omega=1.8, label=None):
+        """The same time steps from the same u0 in another form: precision 1 =
+        FP32-refined (FP32 SOR inside FP64 defect correction, FP64 stopping
+        test); warm 1 / 4 = linear / cubic extrapolation start, 2 = the
+        learned network's prediction; omega = the FP64 solve's SOR factor
+        (1.8 is the workload's; other values are non-parity controls).
+        Timed like the plain run; error and mask against its final u."""
+        lib.set_precision(precision)
         lib.set_warm_start(warm)
+        lib.set(omega_c=omega)
         om_keep = lib.lib.seg4dSetInnerOmega(
             args.inner_omega if args.inner_omega is not None else -1.0)
         lib.arr("u")[...] = u0
         lib.upload("u")
         lib.set(numberOfTimeStep=1)
-        lib.seg_time_step()  # warm-up of the FP32 buffers and kernels
+        lib.seg_time_step()  # warm-up of the form's buffers and kernels
         lib.arr("u")[...] = u0
         lib.upload("u")  # also discards the time history
         lib.synchronize()
@@ -271,54 +316,93 @@ def main():
         pms1, pn1 = lib.net_predict_stats()
         lib.set_precision(0)
         lib.set_warm_start(0)
+        lib.set(omega_c=1.8)
         inner_om = lib.lib.seg4dSetInnerOmega(om_keep)
         lib.download("u")
         um = lib.arr("u")[1:-1, 1:-1, 1:-1, 1:-1]
         thm = 0.5 * (um.max() + um.min())
         maskm = um >= thm
-        inter = np.logical_and(mask64, maskm).sum()
-        union = np.logical_or(mask64, maskm).sum()
+        inter = int(np.logical_and(mask64, maskm).sum())
+        union = int(np.logical_or(mask64, maskm).sum())
         r32 = mprof["red1_f32_ms"] / max(mprof["red1_f32_launches"], 1)
+        starts = {0: "u^n", 1: "2u^n - u^(n-1)", 2: "learned",
+                  4: "4u^n - 6u^(n-1) + 4u^(n-2) - u^(n-3)"}
         res = {
-            "warm_start": {0: "u^n", 1: "2u^n - u^(n-1)", 2: "learned",
-                           4: "4u^n - 6u^(n-1) + 4u^(n-2) - u^(n-3)"}[warm],
-            "inner_omega": inner_om if inner_om > 0 else "omega_c",
+            "precision": "fp32-refined" if precision else "f64",
+            "warm_start": starts[warm],
+            "sor_omega": (inner_om if inner_om > 0 else omega) if precision else omega,
+            "parity_form": precision == 0 and warm == 0 and omega == 1.8,
             "time_to_tolerance_s": t3 - t2,
             "ms_per_step": (t3 - t2) / args.steps * 1e3,
             "speedup_vs_fp64": total / (t3 - t2),
-            "fp32_sweeps_per_step": msweeps,
-            "rounds_per_step": mrounds,
-            "red_pass_f32_avg_ms": r32,
-            "red_pass_f32_gbps": nred * 48.0 / (r32 / 1e3) / 1e9 if r32 > 0 else None,
-            "defect_ms_per_step": mprof["defect_ms"] / args.steps,
+            "sweeps_per_step": msweeps,
+            "sweeps_total": int(sum(msweeps)),
+            "rounds_per_step": mrounds if precision else None,
+            "setup_ms_per_step": mprof["setup_ms"] / max(mprof["setup_launches"], 1),
+            "red_pass_f32_avg_ms": r32 if precision else None,
+            "red_pass_f32_gbps": (nred * 48.0 / (r32 / 1e3) / 1e9) if precision and r32 > 0
+            else None,
+            "defect_ms_per_step": mprof["defect_ms"] / args.steps if precision else None,
             "predict_ms_per_step": (pms1 - pms0) / max(pn1 - pn0, 1) if warm == 2 else None,
             "max_abs_du_vs_fp64_final": float(np.abs(um - u64).max()),
             "max_abs_u_fp64_final": float(np.abs(u64).max()),
-            "mask_iou_vs_fp64_final": float(inter / union) if union else 1.0,
-            "reference": ("final u of the FP64 GPU path, which is bitwise equal to the FP64 "
-                          "oracle on every size the tests run the oracle at"),
+            "mask_iou_vs_fp64_final": inter / union if union else 1.0,
+            "mask_voxels_differing": union - inter,
+            "reference": ("final u of the plain FP64 GPU path, bitwise equal to the FP64 oracle "
+                          "(tests: this workload's edge indicator and first steps, all 20 steps "
+                          "at 64x64x32x8)"),
         }
-        log(f"[bench] FP32-refined (start {res['warm_start']}): {t3 - t2:.3f} s "
-            f"({total / (t3 - t2):.2f}x), sweeps {msweeps}, rounds {mrounds}, "
+        name = label or f"{res['precision']} start {res['warm_start']} omega {res['sor_omega']}"
+        log(f"[bench] {name}: {t3 - t2:.3f} s ({total / (t3 - t2):.2f}x), sweeps {msweeps}, "
             f"IoU {res['mask_iou_vs_fp64_final']:.5f}")
         return res
 
     mixed = mixed_warm = mixed_cubic = learned = None
+    controls = None
     if not args.no_mixed:
-        mixed = run_refined(0)
-        mixed_warm = run_refined(1)
-        mixed_cubic = run_refined(4)
+        mixed = run_form(1, 0)
+        mixed_warm = run_form(1, 1)
+        mixed_cubic = run_form(1, 4)
         if os.path.exists(net_path):
             lib.net_load(net_path)
-            learned = run_refined(2)
+            learned = run_form(1, 2)
             layers, base = lib.net_shape()
             learned["network"] = {"layers": layers, "channels": 16,
-                                  "corrects": "2u^n - u^(n-1)" if base == 1 else
-                                  "4u^n - 6u^(n-1) + 4u^(n-2) - u^(n-3)"}
+                                  "corrects": {0: "u^n (inputs u^n, g)", 1: "2u^n - u^(n-1)",
+                                               3: "4u^n - 6u^(n-1) + 4u^(n-2) - u^(n-3)"}[base]}
             learned["weights"] = os.path.relpath(net_path, ROOT)
             learned["training"] = training
+            learned["sweeps_saved_vs_start_un"] = mixed["sweeps_total"] - learned["sweeps_total"]
+            learned["sweeps_saved_vs_cubic_start"] = (mixed_cubic["sweeps_total"] -
+                                                      learned["sweeps_total"])
         else:
             log(f"[bench] no learned warm start weights at {net_path}: skipped")
+        if not args.no_controls:
+            # attribution of the refined forms' speed-up (non-parity: the
+            # workload fixes omega 1.8 and the start u^n for the FP64 path)
+            controls = {
+                "fp64_omega_1.2": run_form(0, 0, 1.2),
+                "fp64_cubic_start": run_form(0, 4, 1.8),
+                "fp64_omega_1.2_cubic_start": run_form(0, 4, 1.2),
+            }
+
+    # mirror mode (the drop-in default: host arrays authoritative, u and Mmask
+    # up and u, prev_u down every step) on a few steps
+    mirror = None
+    if args.mirror_steps > 0:
+        lib.set_resident(False)
+        lib.arr("u")[...] = u0
+        ms = []
+        for s in range(args.mirror_steps):
+            lib.set(numberOfTimeStep=s + 1)
+            t = time.perf_counter()
+            lib.seg_time_step()
+            ms.append((time.perf_counter() - t) * 1e3)
+        mirror = {"ms_per_step": ms, "resident_ms_per_step_same_steps":
+                  total / args.steps * 1e3,
+                  "pcie_bytes_per_step": 4 * 8 * int(np.prod(shape))}
+        log(f"[bench] mirror mode: {[round(x, 1) for x in ms]} ms per step")
+        lib.set_resident(True)
 
     red1_avg_s = prof["red1_ms"] / max(prof["red1_launches"], 1) / 1e3
     achieved = nred * BYTES_PER_UPDATE / red1_avg_s / 1e9
@@ -332,7 +416,8 @@ def main():
     if not args.no_cpu:
         log("[bench] CPU baseline sample ...")
         cpu = cpu_baseline(lib, shape, u0, args.cpu_iters, float(np.mean(sweeps)), nint)
-        log(f"[bench] CPU baseline {cpu['value']:.3e} voxel-updates/s on {cpu['cores']} threads")
+        log(f"[bench] CPU baseline {cpu['value']:.3e} voxel-updates/s on {cpu['cores']} threads "
+            f"({cpu['physical_cores_available']} physical cores available)")
     lib.deallocate()
 
     out = {
@@ -373,11 +458,16 @@ def main():
         "sweeps_per_step": sweeps,
         "setup_ms_per_step": prof["setup_ms"] / max(prof["setup_launches"], 1),
         "edge_indicator_s_pcie_inclusive": edge_s,
+        "edge_indicator_ms_resident": edge_dev_s * 1e3,
+        "edge_indicator_presmoothing_sweeps": edge_iters,
+        "mirror_mode": mirror,
         "mask_fraction": mask_frac,
+        "mask_voxels": int(mask64.sum()),
         "fp32_refined": mixed,
         "fp32_refined_extrapolated_start": mixed_warm,
         "fp32_refined_cubic_start": mixed_cubic,
         "fp32_refined_learned_start": learned,
+        "controls_non_parity": controls,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

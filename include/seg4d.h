@@ -250,9 +250,9 @@ int seg4dLastRounds(void);
  * linear extrapolation 2 u^n - u^(n-1), used when the previous call was a
  * completed _segTimeStep_ whose state stayed on the device (resident mode).
  * 2: the prediction of the learned network (seg4dNet*): the network's
- * correction of the linear extrapolation (under the condition of 1) or of
- * the cubic one (under the condition of 4; before that, as 4), by the
- * network's base order.  3: the quadratic
+ * correction of u^n (base order 0, from the first step on), of the linear
+ * extrapolation (order 1, under the condition of 1) or of the cubic one
+ * (order 3, under the condition of 4; before that, as 4).  3: the quadratic
  * extrapolation 3 u^n - 3 u^(n-1) + u^(n-2) after two such steps (after
  * one, as 1; one more padded FP64 array on the device).  4: the cubic one
  * 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) after three (else as 3; two more
@@ -266,19 +266,20 @@ int seg4dSetWarmStart(int mode);
 /* ---- learned warm start (no counterpart in the reference) ---------------
  * A separable 4D convolutional network (1 to 4 x {3x3x3 spatial conv, 3-tap
  * temporal conv, bias, ReLU}, 16 channels, 1x1 projection; BF16 MFMA
- * kernels, FP32 accumulation) predicts the correction of an extrapolation
- * of the solution -- the linear 2 u^n - u^(n-1) (base order 1) or the cubic
- * 4 u^n - 6 u^(n-1) + 4 u^(n-2) - u^(n-3) (base order 3) -- from u^n, the
- * increment u^n - u^(n-1), the cubic's difference from the linear one
- * (order 3) and the mean g.  It is trained on solver runs: seg4dNetRecord
+ * kernels, FP32 accumulation) predicts the correction of a base -- u^n
+ * itself (base order 0: the network sees only u^n and the mean g and
+ * predicts the increment u^(n+1) - u^n), the linear extrapolation
+ * 2 u^n - u^(n-1) (order 1) or the cubic 4 u^n - 6 u^(n-1) + 4 u^(n-2) -
+ * u^(n-3) (order 3) -- from u^n, the increment u^n - u^(n-1) (orders 1, 3),
+ * the cubic's difference from the linear one (order 3) and the mean g.  It is trained on solver runs: seg4dNetRecord
  * after each _segTimeStep_ (and once before the first) fills the pool;
  * seg4dNetTrain fits it with Adam on patches of 64^3 points x all frames.
  * The network outlives _deallocateMem_, so runs of several volumes can be
  * pooled.  Functions return 0 (or the documented value) on success, -1 on
  * failure. */
 int seg4dNetInit(uint64_t seed);          /* fresh weights: 4 layers, order 1 */
-/* Fresh weights of `layers` (1..4) layers correcting the extrapolation of
- * order `base` (1 or 3); empties the pool. */
+/* Fresh weights of `layers` (1..4) layers correcting the base of order
+ * `base` (0, 1 or 3); empties the pool. */
 int seg4dNetInitShape(uint64_t seed, int layers, int base);
 int seg4dNetShape(int *layers, int *base);  /* shape of the current network */
 int seg4dNetLoad(const char *path);       /* weights file of seg4dNetSave   */
@@ -308,8 +309,9 @@ int seg4dNetGrad(int I, int J, int K, int L, int P, const uint16_t *feat_bf16,
                  double *loss);
 /* Test hook: the network's correction c of the next solve's start (FP32,
  * padded layout, interior points; the caller's array holds one float per
- * padded point) and its scale q (rms increment for base order 1, rms third
- * difference for order 3), so that start = base + q c / s_t, from the
+ * padded point) and its scale q (1 for base order 0, rms increment for
+ * order 1, rms third difference for order 3), so that start = base +
+ * q c / s_t, from the
  * device state after completed resident time steps; needs the history the
  * base uses.  The state is left unchanged. */
 int seg4dNetPreview(float *c, double *q);

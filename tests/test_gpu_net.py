@@ -194,7 +194,7 @@ def _volume(lib, dims, seed):
     return shape, img, u0
 
 
-@pytest.mark.parametrize("layers,base", [(4, 1), (1, 3)])
+@pytest.mark.parametrize("layers,base", [(4, 1), (1, 3), (4, 0)])
 def test_train_on_solver_runs(seglib, tmp_path, layers, base):
     """Pool two short runs, train, and check that the loss falls below that
     of the base extrapolation (the untrained network's prediction, loss 1 by
@@ -241,12 +241,14 @@ def test_train_on_solver_runs(seglib, tmp_path, layers, base):
     assert lib.net_scales() == (s_d, s_t)
 
 
-@pytest.mark.parametrize("precision,layers,base", [(0, 4, 1), (1, 4, 1), (0, 1, 3), (1, 2, 3)])
+@pytest.mark.parametrize("precision,layers,base", [(0, 4, 1), (1, 4, 1), (0, 1, 3), (1, 2, 3),
+                                                   (1, 4, 0)])
 def test_learned_start_vs_oracle(seglib, oracle, precision, layers, base):
     """SOR started from the network's prediction (random weights with a
     non-zero projection, so the start differs from every other mode): same
     stopping test, trajectory within the stated error of the FP64 oracle.
-    The cubic base predicts once three completed steps precede."""
+    The cubic base predicts once three completed steps precede, the base
+    u^n (inputs u^n, g) from the first step on."""
     print(f"[gpu] learned start precision={precision} layers={layers} base={base}", flush=True)
     lib = seglib
     rng = np.random.default_rng(7)
@@ -285,8 +287,9 @@ def test_learned_start_vs_oracle(seglib, oracle, precision, layers, base):
             assert R <= lib.lib.seg4dLastTolerance()
         ms, n = lib.net_predict_stats()
         ms, n = ms - ms0, n - n0
-        # from the step after the base's history exists: 2..4 (linear), 4..6 (cubic)
-        assert n == 3
+        # from the step after the base's history exists: 2..4 (linear), 4..6
+        # (cubic), every step (base u^n)
+        assert n == (steps if base == 0 else 3)
         lib.download("u")
         ug = lib.arr("u")
         assert np.abs(ug - u).max() <= 1e-4 * np.abs(u).max()
@@ -378,6 +381,59 @@ def test_prediction_vs_torch(seglib):
     err = np.abs(got - ref).max()
     print(f"  kappa {q:.3e} (sigma {sig:.3e}), max|c| {np.abs(ref).max():.4f}, max err {err:.2e}",
           flush=True)
+    assert np.abs(ref).max() > 0
+    assert err <= 5e-3 * np.abs(ref).max()
+    assert not np.any(c[ghost])
+
+
+def test_prediction_base_un_vs_torch(seglib):
+    """The configured network's shape: base u^n, inputs u^n and the mean g
+    only (the increment and third-difference channels stay zero), no history:
+    its correction on the initial state against the PyTorch reference with
+    the same BF16 rounding points, and the scale q = 1."""
+    print("[gpu] learned prediction, base u^n, vs torch", flush=True)
+    lib = seglib
+    rng = np.random.default_rng(12)
+    layers = 2
+    lib.net_init(3, layers=layers, base=0)
+    w = random_params(rng, 0.5, layers)
+    lib.net_set_params(w, 0.7, 3.0)
+    dims = (37, 30, 18, 6)
+    shape, img, u0 = _volume(lib, dims, 6)
+    lib.allocate(*dims)
+    try:
+        lib.set(gsTol=0.0, **KW)
+        lib.arr("i0")[...] = img
+        lib.arr("u")[...] = 0.0
+        lib.arr("Mmask")[...] = 1.0
+        lib.gf("i0", "u")
+        g = [lib.arr("g_" + f).copy() for f in ("e", "w", "n", "s", "t", "b", "fw", "bw")]
+        lib.set_resident(True)
+        lib.arr("u")[...] = u0
+        lib.upload("u")
+        assert lib.set_warm_start(2) == 0
+        c, q = lib.net_preview(shape)
+    finally:
+        lib.set_warm_start(0)
+        lib.set_resident(False)
+        lib.deallocate()
+    assert q == 1.0
+    inner = (slice(1, -1),) * 4
+    gm = g[0].copy()
+    for a in g[1:]:
+        gm = gm + a
+    gm = gm * 0.125
+    feat = np.zeros(shape + (8,), dtype=np.float32)
+    feat[..., 0] = u0
+    feat[..., 2] = gm
+    ghost = np.ones(shape, dtype=bool)
+    ghost[inner] = False
+    feat[ghost] = 0.0
+    fr = torch.from_numpy(feat).to(torch.bfloat16).to(torch.float32)
+    ref = reference(torch.from_numpy(w.copy()), fr, shape[0], layers).numpy()[0]
+    got = c[inner]
+    err = np.abs(got - ref).max()
+    print(f"  max|c| {np.abs(ref).max():.4f}, max err {err:.2e}", flush=True)
     assert np.abs(ref).max() > 0
     assert err <= 5e-3 * np.abs(ref).max()
     assert not np.any(c[ghost])
