@@ -3,10 +3,13 @@
 // and meaning (segmentation4D.c:13-60, segmentation4D.h:8-62).
 #include "seg4d.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "seg4d_batch.h"
 #include "seg4d_engine.h"
@@ -153,11 +156,54 @@ bool upload_g_if_dirty() {
     return true;
 }
 
+// Host arrays that cross PCIe in mirror mode -- u, prev_u and Mmask every
+// time step, i0 and the g arrays with the edge indicator -- are pinned
+// (page-locked), so the copies run as DMA at link speed instead of through
+// the runtime's staging buffers; the others, and any array whose pinning
+// fails, come from calloc.  Both are zero-filled, as the reference's
+// allocation (segmentation4D.c:106-133).
+std::vector<double*> g_pinned;
+
+void zero_fill(double* p, size_t n) {
+    const unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    const size_t per = (n + nt - 1) / nt;
+    for (unsigned q = 0; q < nt; q++) {
+        const size_t lo = q * per, hi = std::min(n, lo + per);
+        if (lo < hi) th.emplace_back([=] { std::memset(p + lo, 0, (hi - lo) * sizeof(double)); });
+    }
+    for (auto& x : th) x.join();
+}
+
+double* host_alloc(size_t n, bool pin) {
+    if (pin) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, n * sizeof(double), hipHostMallocDefault) == hipSuccess && p) {
+            zero_fill(static_cast<double*>(p), n);
+            g_pinned.push_back(static_cast<double*>(p));
+            return static_cast<double*>(p);
+        }
+        (void)hipGetLastError();  // pageable memory instead
+    }
+    return static_cast<double*>(std::calloc(n, sizeof(double)));
+}
+
+void host_free(double* p) {
+    if (!p) return;
+    auto it = std::find(g_pinned.begin(), g_pinned.end(), p);
+    if (it != g_pinned.end()) {
+        g_pinned.erase(it);
+        (void)hipHostFree(p);
+    } else {
+        std::free(p);
+    }
+}
+
 void free_host() {
     double** all[] = {&u, &prev_u, &i0, &Mmask, &e, &w, &n, &s, &t, &b, &fw, &bw,
                       &g_e, &g_w, &g_n, &g_s, &g_t, &g_b, &g_fw, &g_bw};
     for (double** p : all) {
-        std::free(*p);
+        host_free(*p);
         *p = nullptr;
     }
     double*** cs[] = {&center_x, &center_y, &center_z};
@@ -198,9 +244,14 @@ API bool _allocateMem_(void) {
     free_host();
     double** all[] = {&u, &prev_u, &i0, &Mmask, &e, &w, &n, &s, &t, &b, &fw, &bw,
                       &g_e, &g_w, &g_n, &g_s, &g_t, &g_b, &g_fw, &g_bw};
+    // pinned: u, prev_u, i0, Mmask always; the g arrays while each is at most
+    // 2 GB (larger volumes keep 8 x that much pageable: they cross PCIe once)
+    const bool pin_g = (size_t)N * sizeof(double) <= (size_t(2) << 30);
+    int q = 0;
     for (double** p : all) {
-        // calloc: zero-filled like the reference's explicit loop (:106-133)
-        *p = (double*)std::calloc((size_t)N, sizeof(double));
+        const bool pin = q < 4 || (q >= 12 && pin_g);
+        q++;
+        *p = host_alloc((size_t)N, pin);
         if (!*p) {
             free_host();
             return fail("_allocateMem_: host allocation failed");

@@ -141,7 +141,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--preset", type=int, default=1, help="synthetic volume preset (0 or 1)")
+    ap.add_argument("--preset", type=int, default=1, help="synthetic volume preset 0-4 (the generator's workloads, 64x64x32x8 to 512x512x256x8)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--variant", type=int, default=None, help="SOR pass kernel variant")
