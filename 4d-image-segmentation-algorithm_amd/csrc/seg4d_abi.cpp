@@ -945,6 +945,16 @@ API int seg4dBatchConfigure(int precision, int warm, int graph, int depth) {
     return 0;
 }
 
+API int seg4dBatchSetStreams(int nstreams) {
+    if (!need_batch("seg4dBatchSetStreams")) return -1;
+    if (nstreams < 0 || nstreams > 64) {
+        s4d::set_error("seg4dBatchSetStreams: 0 (one launch per pass) or 1..64 groups");
+        return -1;
+    }
+    g_batch->streams = nstreams;
+    return 0;
+}
+
 API int seg4dBatchAutotune(int reps, float* best_ms) {
     if (!need_batch("seg4dBatchAutotune")) return -1;
     std::vector<float> ms;

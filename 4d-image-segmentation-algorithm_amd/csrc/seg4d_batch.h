@@ -9,6 +9,13 @@
 // iterations of a solve can be replayed from a captured HIP graph (two
 // iterations per replay).  Each volume's result is bitwise that of the
 // same step on a single-volume engine with the same settings.
+//
+// The concurrent form (streams > 0) instead splits the volumes into groups
+// of about equal voxel count, one stream and one host thread per group, and
+// steps each volume on the single-volume path (its own launches and
+// pipelined stopping test); the groups' launches overlap on the GPU, which
+// hides each solve's launch gaps and pass tails behind the other groups'
+// passes.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -55,16 +62,19 @@ public:
     int warm = 0;           // SOR start (Engine::warm_ modes 0, 1, 3, 4)
     int graph = 0;          // 1: solve iterations replayed from a captured graph
     int depth = 2;          // iterations (graph: replays) queued ahead of the test
+    int streams = 0;        // 0: one launch per pass for all volumes; k: k concurrent groups
     int max_rounds = 8;
     double inner_frac = 0.9, inner_floor2 = 1e-10, inner_omega = 1.2;
     std::vector<SolveStats> stats;  // last step, per volume
-    double last_step_ms = 0.0;
+    double last_step_ms = 0.0;      // device time (concurrent form: host wall time)
     long last_iterations = 0;       // sum over volumes of SOR / FP32 sweeps
     std::string err;
 
 private:
     int check(hipError_t e, const char* what);
     int build_jobs(const StepParams& P);
+    int step_streams(const StepParams& P);
+    int use_streams(int k);  // engines onto k group streams (0: the batch stream)
     template <typename T>
     int sor_run(std::vector<BatchJob<T>>& hj, BatchJob<T>* dj, SorCtl* dctl, SorCtl* hctl,
                 int cap, hipGraphExec_t* gexec, int* gcap);
@@ -77,6 +87,9 @@ private:
     SorCtl* hctl2_ = nullptr;
     BatchJob<double>* dj64_ = nullptr;
     BatchJob<float>* dj32_ = nullptr;
+    int* vmap_ = nullptr;     // volume of each group of 8 blocks (FP64 launches, then FP32)
+    int* vmap32_ = nullptr;   // the FP32 launches' part of vmap_
+    long vmap_n_ = 0;
     std::vector<BatchJob<double>> j64_;
     std::vector<BatchJob<float>> j32_;
     int grid_ = 0, grid32_ = 0, cgrid_ = 0;  // FP64 / FP32 SOR launches, element-wise
@@ -87,6 +100,9 @@ private:
     hipGraphExec_t g64_ = nullptr, g32_ = nullptr;
     int g64_cap_ = -1, g32_cap_ = -1;
     std::vector<hipEvent_t> evq_;
+    std::vector<hipStream_t> gst_;           // group streams of the concurrent form
+    std::vector<std::vector<int>> groups_;   // volumes of each group
+    int adopted_ = 0;                        // groups the engines run on (0: st_)
 };
 
 }  // namespace s4d

@@ -4,7 +4,9 @@
 #include <cmath>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
+#include <thread>
 
 #include "seg4d_batch_kernels.h"
 
@@ -31,12 +33,19 @@ void Batch::release() {
     g64_ = g32_ = nullptr;
     g64_cap_ = g32_cap_ = -1;
     E_.clear();  // engines release their buffers (the stream is not theirs)
+    for (auto s : gst_) (void)hipStreamDestroy(s);
+    gst_.clear();
+    groups_.clear();
+    adopted_ = 0;
     if (ctl_) (void)hipFree(ctl_);
     if (ctl2_) (void)hipFree(ctl2_);
     if (hctl_) (void)hipHostFree(hctl_);
     if (hctl2_) (void)hipHostFree(hctl2_);
     if (dj64_) (void)hipFree(dj64_);
     if (dj32_) (void)hipFree(dj32_);
+    if (vmap_) (void)hipFree(vmap_);
+    vmap_ = nullptr;
+    vmap_n_ = 0;
     ctl_ = ctl2_ = hctl_ = hctl2_ = nullptr;
     dj64_ = nullptr;
     dj32_ = nullptr;
@@ -183,10 +192,30 @@ int Batch::build_jobs(const StepParams& P) {
     }
     grid_ = (int)b0;
     grid32_ = (int)b032;
+    // volume of each group of 8 blocks: FP64 launches, then FP32 launches
+    const long nm = b0 / 8 + b032 / 8;
+    std::vector<int> vm((size_t)nm);
+    for (int v = 0; v < n; v++) {
+        const long g0 = j64_[v].block0 / 8, g1 = (j64_[v].block0 + j64_[v].nblocks + 7) / 8;
+        for (long q = g0; q < g1; q++) vm[(size_t)q] = v;
+        const long f0 = j32_[v].block0 / 8, f1 = (j32_[v].block0 + j32_[v].nblocks + 7) / 8;
+        for (long q = f0; q < f1; q++) vm[(size_t)(b0 / 8 + q)] = v;
+    }
+    if (nm > vmap_n_) {
+        if (vmap_) (void)hipFree(vmap_);
+        vmap_ = nullptr;
+        vmap_n_ = 0;
+        if (check(hipMalloc((void**)&vmap_, (size_t)nm * sizeof(int)), "hipMalloc volume map"))
+            return -1;
+        vmap_n_ = nm;
+    }
+    vmap32_ = vmap_ + b0 / 8;
     rgrid_ = (int)r0;
     rgrid32_ = (int)r032;
     cgrid_ = (int)cb0;
-    if (check(hipMemcpyAsync(dj64_, j64_.data(), (size_t)n * sizeof(BatchJob<double>),
+    if (check(hipMemcpyAsync(vmap_, vm.data(), (size_t)nm * sizeof(int), hipMemcpyHostToDevice,
+                             st_), "volume map") ||
+        check(hipMemcpyAsync(dj64_, j64_.data(), (size_t)n * sizeof(BatchJob<double>),
                              hipMemcpyHostToDevice, st_), "jobs") ||
         check(hipMemcpyAsync(dj32_, j32_.data(), (size_t)n * sizeof(BatchJob<float>),
                              hipMemcpyHostToDevice, st_), "jobs") ||
@@ -216,12 +245,15 @@ int Batch::sor_run(std::vector<BatchJob<T>>& hj, BatchJob<T>* dj, SorCtl* dctl, 
     (void)hj;
     const int n = size();
     const dim3 grid((unsigned)(sizeof(T) == 4 ? grid32_ : grid_));
+    const int* vm = sizeof(T) == 4 ? vmap32_ : vmap_;
     int rc = 0;
     auto red = [&](int zr, int zw, bool resid) {
-        if (resid) k_sor_batch<T, 1, 2><<<grid, kBlock, 0, st_>>>(dj, n, 0, zr, zw);
-        else k_sor_batch<T, 0, 2><<<grid, kBlock, 0, st_>>>(dj, n, 0, zr, zw);
+        if (resid) k_sor_batch<T, 1, 2><<<grid, kBlock, 0, st_>>>(dj, vm, dctl, 0, zr, zw);
+        else k_sor_batch<T, 0, 2><<<grid, kBlock, 0, st_>>>(dj, vm, dctl, 0, zr, zw);
     };
-    auto black = [&](int zr) { k_sor_batch<T, 2, 2><<<grid, kBlock, 0, st_>>>(dj, n, 1, zr, 0); };
+    auto black = [&](int zr) {
+        k_sor_batch<T, 2, 2><<<grid, kBlock, 0, st_>>>(dj, vm, dctl, 1, zr, 0);
+    };
     // iteration z + 1 and the stopping test of iteration z
     const dim3 rgrid((unsigned)(sizeof(T) == 4 ? rgrid32_ : rgrid_));
     auto iter = [&](int z) {
@@ -287,6 +319,90 @@ int Batch::sor_run(std::vector<BatchJob<T>>& hj, BatchJob<T>* dj, SorCtl* dctl, 
     return check(hipStreamSynchronize(st_), "batch solve sync");
 }
 
+int Batch::use_streams(int k) {
+    const int n = size();
+    if (k > n) k = n;
+    if (k == adopted_) return 0;
+    if (k == 0) {
+        for (auto& e : E_)
+            if (e->adopt_stream(st_)) {
+                err = e->err;
+                set_error(err);
+                return -1;
+            }
+        adopted_ = 0;
+        return 0;
+    }
+    while ((int)gst_.size() < k) {
+        hipStream_t s = nullptr;
+        if (check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate")) return -1;
+        gst_.push_back(s);
+    }
+    // largest volumes first, each to the group with the fewest voxels so far
+    std::vector<int> order(n);
+    for (int v = 0; v < n; v++) order[v] = v;
+    auto vox = [&](int v) { return (long)E_[v]->I_ * E_[v]->J_ * E_[v]->K_ * E_[v]->L_; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return vox(a) > vox(b); });
+    groups_.assign(k, std::vector<int>());
+    std::vector<long> load(k, 0);
+    for (int v : order) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        groups_[g].push_back(v);
+        load[g] += vox(v);
+    }
+    for (int g = 0; g < k; g++)
+        for (int v : groups_[g])
+            if (E_[v]->adopt_stream(gst_[g])) {
+                err = E_[v]->err;
+                set_error(err);
+                return -1;
+            }
+    adopted_ = k;
+    return 0;
+}
+
+int Batch::step_streams(const StepParams& P) {
+    const int n = size();
+    if (use_streams(streams)) return -1;
+    for (auto& e : E_) {
+        e->max_rounds_ = max_rounds;
+        e->inner_frac_ = inner_frac;
+        e->inner_floor2_ = inner_floor2;
+        e->inner_omega_ = inner_omega;
+    }
+    int dev = 0;
+    if (check(hipGetDevice(&dev), "hipGetDevice")) return -1;
+    const int k = adopted_;
+    std::vector<std::string> errs(k);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> th;
+    th.reserve(k);
+    for (int g = 0; g < k; g++)
+        th.emplace_back([&, g] {
+            if (hipSetDevice(dev) != hipSuccess) {
+                errs[g] = "batch: hipSetDevice in a group thread";
+                return;
+            }
+            for (int v : groups_[g])
+                if (E_[v]->seg_step(P, nullptr, &stats[v])) {
+                    errs[g] = "batch volume " + std::to_string(v) + ": " + E_[v]->err;
+                    return;
+                }
+        });
+    for (auto& t : th) t.join();
+    last_step_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    last_iterations = 0;
+    for (int v = 0; v < n; v++) last_iterations += stats[v].z;
+    for (int g = 0; g < k; g++)
+        if (!errs[g].empty()) {
+            err = errs[g];
+            set_error(err);
+            return -1;
+        }
+    return 0;
+}
+
 int Batch::step(const StepParams& P) {
     const int n = size();
     if (n == 0) {
@@ -298,6 +414,8 @@ int Batch::step(const StepParams& P) {
         e->precision_ = precision;
         e->warm_ = warm;
     }
+    if (streams > 0) return step_streams(P);
+    if (use_streams(0)) return -1;
     if (build_jobs(P)) return -1;
     const int cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
     hipEvent_t e0, e1;
@@ -336,8 +454,8 @@ int Batch::step(const StepParams& P) {
             return r | check(hipStreamSynchronize(st_), "defect sync");
         };
         auto defect = [&]() {
-            k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, n, 0);
-            k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, n, 1);
+            k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, vmap_, ctl_, 0);
+            k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, vmap_, ctl_, 1);
             k_resid_chunks_batch<double><<<(unsigned)rgrid_, 1024, 0, st_>>>(dj64_, n);
             k_defect_decide<<<n, 1024, 0, st_>>>(dj64_, max_rounds);
             return sync_ctl();

@@ -9,7 +9,9 @@
 // partial index, same fixed-order reduction), so every volume's result is
 // bitwise that of its single-volume solve.  Volume ranges start at multiples
 // of 8, so a block's XCD (b mod 8) is the one the volume's own launch order
-// assumes.
+// assumes, and every group of 8 consecutive blocks belongs to one volume: a
+// table of one entry per group (vmap[b / 8]) names it with a single load
+// instead of a search over the ranges.
 #pragma once
 #include "seg4d_kernels.h"
 
@@ -37,11 +39,16 @@ struct BatchJob {
     int r1block0;           // this volume's first block of the chunk-sum launch
 };
 
-template <typename T>
-__device__ __forceinline__ int batch_volume(const BatchJob<T>* __restrict__ J, int nvol, int b) {
-    int v = 0;
-    while (v + 1 < nvol && b >= J[v + 1].block0) v++;
-    return v;
+// Pointers read from a descriptor in memory are generic to the compiler, and
+// generic accesses become FLAT instructions (aperture checks, and every
+// access counted on both the vector-memory and the LDS/scalar counters, which
+// serialises the waits).  The buffers are device memory: casting through the
+// global address space lets the compiler emit global loads and stores, as for
+// pointers passed as kernel arguments.
+template <typename P>
+__device__ __forceinline__ P* glob(P* p) {
+    // through an integer: a generic -> global -> generic cast pair folds away
+    return (P*)(__attribute__((address_space(1))) P*)(uintptr_t)p;
 }
 
 template <typename T>
@@ -53,23 +60,29 @@ __device__ __forceinline__ int batch_cvolume(const BatchJob<T>* __restrict__ J, 
 
 // One SOR colour pass of every running volume.  colour 0: red, reads
 // red[zr], writes red[zw]; colour 1: black in place, its red neighbours in
-// red[zr].  Volumes whose control block says stop return at once.
+// red[zr].  Volumes whose control block says stop return at once.  ctl is
+// the control-block array (volume v's is ctl[v], the one J[v].ctl names), so
+// that the stop flag and the descriptor load together once v is known.
 template <typename T, int RESID, int U>
 static __global__ __launch_bounds__(kBlock) void k_sor_batch(const BatchJob<T>* __restrict__ J,
-                                                      int nvol, int colour, int zr, int zw) {
-    const int v = batch_volume(J, nvol, blockIdx.x);
+                                                      const int* __restrict__ vmap,
+                                                      const SorCtl* __restrict__ ctl, int colour,
+                                                      int zr, int zw) {
+    const int v = vmap[blockIdx.x >> 3];
+    const int stop = ctl[v].stop;
     const BatchJob<T>& B = J[v];
-    if (B.ctl->stop) return;
     const int lb = blockIdx.x - B.block0;
-    if (lb >= B.nblocks) return;
+    if ((stop != 0) | (lb >= B.nblocks)) return;
     int bx, kb;
     if (!map_block(B.M, lb, B.nblocks, bx, kb)) return;
     if (colour == 0)
-        sor_block<T, RESID, U, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[zr], B.red[zw],
-                                      B.black, B.opnd[0], B.part_r, nullptr);
+        sor_block<T, RESID, U, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, glob(B.red[zr]),
+                                      glob(B.red[zw]), glob(B.black), glob(B.opnd[0]),
+                                      glob(B.part_r), nullptr);
     else
-        sor_block<T, RESID, U, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, B.black,
-                                      B.red[zr], B.opnd[1], B.part_b, nullptr);
+        sor_block<T, RESID, U, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, glob(B.black),
+                                      glob(B.black), glob(B.red[zr]), glob(B.opnd[1]),
+                                      glob(B.part_b), nullptr);
 }
 
 // FP64 defect of the corrected state u + d of every running volume (the
@@ -77,24 +90,26 @@ static __global__ __launch_bounds__(kBlock) void k_sor_batch(const BatchJob<T>* 
 // red stores u_r + d_r and reads black neighbours u_b + d_b; black stores
 // u_b + d_b with its red neighbours already corrected.
 static __global__ __launch_bounds__(kBlock) void k_defect_corr_batch(
-    const BatchJob<double>* __restrict__ J64, const BatchJob<float>* __restrict__ J32, int nvol,
-    int colour) {
-    const int v = batch_volume(J64, nvol, blockIdx.x);
+    const BatchJob<double>* __restrict__ J64, const BatchJob<float>* __restrict__ J32,
+    const int* __restrict__ vmap, const SorCtl* __restrict__ ctl, int colour) {
+    const int v = vmap[blockIdx.x >> 3];
+    const int stop = ctl[v].stop;
     const BatchJob<double>& B = J64[v];
-    if (B.ctl->stop) return;
     const int lb = blockIdx.x - B.block0;
-    if (lb >= B.nblocks) return;
+    if ((stop != 0) | (lb >= B.nblocks)) return;
     int bx, kb;
     if (!map_block(B.M, lb, B.nblocks, bx, kb)) return;
     const BatchJob<float>& D = J32[v];
     // split operands: the batch's set-up is the fused FP32-refined one
     if (colour == 0)
-        sor_block<double, 6, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, B.red[0], B.red[0],
-                                       B.black, B.opnd[0], B.part_r, B.defect_ob[0],
-                                       D.red[D.ctl->zfinal & 1], D.black);
+        sor_block<double, 6, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 0, glob(B.red[0]),
+                                       glob(B.red[0]), glob(B.black), glob(B.opnd[0]),
+                                       glob(B.part_r), glob(B.defect_ob[0]),
+                                       glob(D.red[D.ctl->zfinal & 1]), glob(D.black));
     else
-        sor_block<double, 6, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, B.black, B.black,
-                                       B.red[0], B.opnd[1], B.part_b, B.defect_ob[1], D.black,
+        sor_block<double, 6, 2, false>(B.G, B.M, bx, kb, B.c, B.om, B.h4, 1, glob(B.black),
+                                       glob(B.black), glob(B.red[0]), glob(B.opnd[1]),
+                                       glob(B.part_b), glob(B.defect_ob[1]), glob(D.black),
                                        nullptr);
 }
 
@@ -108,8 +123,8 @@ static __global__ __launch_bounds__(1024) void k_resid_chunks_batch(const BatchJ
     const BatchJob<T>& B = J[v];
     if (B.ctl->stop) return;
     const int q = blockIdx.x - B.r1block0;
-    const double S = chunk_sum(B.part_r, B.part_b, B.nb, B.nc, B.ch, q);
-    if (threadIdx.x == 0) B.red2[q] = S;
+    const double S = chunk_sum(glob(B.part_r), glob(B.part_b), B.nb, B.nc, B.ch, q);
+    if (threadIdx.x == 0) glob(B.red2)[q] = S;
 }
 
 // Residual reduction and stopping test of every running volume (one block
@@ -191,9 +206,11 @@ static __global__ __launch_bounds__(kBlock) void k_zero_batch(const BatchJob<dou
     if (J64[v].ctl->stop || J64[v].ctl->z == 0) return;
     const BatchJob<float>& B = J32[v];
     const long stride = (long)B.cnblocks * kBlock;
+    float* dr = glob(B.red[0]);
+    float* db = glob(B.black);
     for (long q = (long)(blockIdx.x - B.cblock0) * kBlock + threadIdx.x; q < B.np; q += stride) {
-        B.red[0][q] = 0.0f;
-        B.black[q] = 0.0f;
+        dr[q] = 0.0f;
+        db[q] = 0.0f;
     }
 }
 

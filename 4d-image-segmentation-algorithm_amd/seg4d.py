@@ -217,6 +217,8 @@ class SegLib(_Lib):
         L.seg4dBatchLastStepMs.restype = c_double
         L.seg4dBatchConfigure.restype = c_int
         L.seg4dBatchConfigure.argtypes = [c_int] * 4
+        L.seg4dBatchSetStreams.restype = c_int
+        L.seg4dBatchSetStreams.argtypes = [c_int]
         L.seg4dBatchAutotune.restype = c_int
         L.seg4dBatchAutotune.argtypes = [c_int, POINTER(ctypes.c_float)]
         L.seg4dBatchTile.restype = c_int
@@ -506,6 +508,12 @@ class SegLib(_Lib):
     def batch_configure(self, precision=0, warm=0, graph=0, depth=2):
         if self.lib.seg4dBatchConfigure(int(precision), int(warm), int(graph), int(depth)) != 0:
             raise RuntimeError(f"seg4dBatchConfigure failed: {self.last_error()}")
+
+    def batch_streams(self, nstreams):
+        """Concurrent form: the volumes in `nstreams` groups on their own streams
+        (0: one launch per pass for all volumes)."""
+        if self.lib.seg4dBatchSetStreams(int(nstreams)) != 0:
+            raise RuntimeError(f"seg4dBatchSetStreams failed: {self.last_error()}")
 
     def batch_step(self):
         if self.lib.seg4dBatchStep() != 0:

@@ -356,6 +356,16 @@ double seg4dBatchLastStepMs(void);   /* device time of the last step */
  * HIP graph (two iterations per replay); depth 1..16 iterations (replays)
  * queued ahead of the stopping test.  Defaults 0, 0, 0, 2. */
 int seg4dBatchConfigure(int precision, int warm, int graph, int depth);
+/* Concurrent form: nstreams >= 1 splits the volumes into that many groups of
+ * about equal voxel count (at most one per volume), each on its own stream
+ * driven by its own host thread, and steps every volume on the
+ * single-volume path (its own launches and pipelined stopping test), so
+ * that the groups' passes overlap on the GPU; results are bitwise those of
+ * the one-launch form.  seg4dBatchLastStepMs then reports the host wall time
+ * of the step.  A failed step leaves the volumes of other groups stepped
+ * (seg4dBatchStats tells which).  0 (the default) restores one launch per
+ * pass for all volumes. */
+int seg4dBatchSetStreams(int nstreams);
 /* Per-volume launch-order tiles (seg4dAutotune on each volume); best_ms[v]
  * (may be NULL) receives each volume's red-pass time. */
 int seg4dBatchAutotune(int reps, float *best_ms);

@@ -3,7 +3,9 @@ in one launch per colour pass.  Every volume's u after every step and its
 sweep count must be bitwise those of the same steps on the single-volume
 library (whose FP64 path is bitwise the oracle's, checked directly for the
 smallest volume below), in FP64 and FP32-refined arithmetic, with eager or
-graph-replayed iterations, any queue depth and autotuned launch tiles."""
+graph-replayed iterations, any queue depth and autotuned launch tiles, and
+in the concurrent form (volume groups on their own streams and host threads,
+switched to and from within one batch)."""
 import numpy as np
 import pytest
 
@@ -70,7 +72,9 @@ def volumes(seglib):
     return [_volume(seglib, d, 11 + q) for q, d in enumerate(DIMS)]
 
 
-def _run_batch(lib, volumes, precision, warm, graph, depth, tune, steps):
+def _run_batch(lib, volumes, precision, warm, graph, depth, tune, steps, streams=(0,)):
+    """streams[s]: the concurrent form's group count at step s (the last entry
+    for later steps; 0 is one launch per pass)."""
     lib.set(gsTol=0.0, **KW)
     lib.set_relative_tolerance(REL)
     lib.batch_create(DIMS)
@@ -84,6 +88,7 @@ def _run_batch(lib, volumes, precision, warm, graph, depth, tune, steps):
             print(f"  tuned tiles {tiles}", flush=True)
         for s in range(steps):
             lib.set(numberOfTimeStep=s + 1)
+            lib.batch_streams(streams[min(s, len(streams) - 1)])
             lib.batch_step()
             res.append([(lib.batch_u(v),) + lib.batch_stats(v)[:2] for v in range(len(DIMS))])
     finally:
@@ -92,15 +97,20 @@ def _run_batch(lib, volumes, precision, warm, graph, depth, tune, steps):
     return res
 
 
-@pytest.mark.parametrize("precision,warm,graph,depth,tune",
-                         [(0, 0, 0, 2, False), (0, 0, 1, 2, False), (0, 3, 0, 1, True),
-                          (1, 0, 0, 2, False), (1, 4, 1, 3, True)])
-def test_batch_matches_single_volume(seglib, volumes, precision, warm, graph, depth, tune):
+@pytest.mark.parametrize("precision,warm,graph,depth,tune,streams",
+                         [(0, 0, 0, 2, False, (0,)), (0, 0, 1, 2, False, (0,)),
+                          (0, 3, 0, 1, True, (0,)), (1, 0, 0, 2, False, (0,)),
+                          (1, 4, 1, 3, True, (0,)), (0, 0, 0, 2, False, (2,)),
+                          (0, 3, 0, 2, True, (3, 3, 0, 5)), (1, 4, 1, 2, True, (0, 2, 4))])
+def test_batch_matches_single_volume(seglib, volumes, precision, warm, graph, depth, tune,
+                                     streams):
     print(f"[gpu] batch precision {precision} warm {warm} graph {graph} depth {depth} "
-          f"tune {tune}", flush=True)
+          f"tune {tune} streams {streams}", flush=True)
     lib = seglib
     steps = STEPS + (2 if warm == 4 else 0)  # reach the cubic start
-    got = _run_batch(lib, volumes, precision, warm, graph, depth, tune, steps)
+    if len(streams) > 1:
+        steps = max(steps, len(streams))
+    got = _run_batch(lib, volumes, precision, warm, graph, depth, tune, steps, streams)
     for v, (img, u0) in enumerate(volumes):
         ref = _single(lib, DIMS[v], img, u0, precision, warm, steps)
         for s in range(steps):
