@@ -570,7 +570,7 @@ API void seg4dProfileReset(void) {
 API int seg4dProfileRead(double* out, int nout) {
     if (!g_eng || !out) return 0;
     const s4d::Profile& p = g_eng->prof;
-    const double v[23] = {p.sweep_ms[0], (double)p.sweep_launches[0],
+    const double v[25] = {p.sweep_ms[0], (double)p.sweep_launches[0],
                           p.sweep_ms[1], (double)p.sweep_launches[1],
                           p.sweep_ms[2], (double)p.sweep_launches[2],
                           p.setup_ms, (double)p.setup_launches, p.step_ms, (double)p.steps,
@@ -579,8 +579,9 @@ API int seg4dProfileRead(double* out, int nout) {
                           p.sweep_ms[3], (double)p.sweep_launches[3],
                           p.sweep_ms[4], (double)p.sweep_launches[4],
                           p.sweep_ms[5], (double)p.sweep_launches[5],
-                          p.defect_ms, (double)p.defect_launches, (double)p.rounds};
-    int k = nout < 23 ? nout : 23;
+                          p.defect_ms, (double)p.defect_launches, (double)p.rounds,
+                          p.copy_ms, (double)p.copy_launches};
+    int k = nout < 25 ? nout : 25;
     for (int q = 0; q < k; q++) out[q] = v[q];
     return k;
 }

@@ -60,6 +60,8 @@ struct Profile {            // HIP-event timing (events on the engine's stream)
     long voxel_updates = 0; // interior voxels x reference iterations
     long iterations = 0;
     long nred = 0, nblack = 0;  // interior points per colour
+    double copy_ms = 0.0;   // k_copy_f64 (a plain stream of known size)
+    long copy_launches = 0;
 };
 
 enum ArrayId {

@@ -276,8 +276,25 @@ int Engine::copy(ArrayId src, ArrayId dst) {
     if (dst == A_U || dst == A_PREV) hist_ok_ = hist2_ok_ = hist3_ok_ = false;
     if (dst >= A_G0 && dst <= A_G7) g_version_++;
     const long nb = (n_ + kBlock - 1) / kBlock;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof.on) {
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, st_);
+    }
     k_copy_f64<<<(unsigned)(nb < 65536 ? nb : 65536), kBlock, 0, st_>>>(full_[src], full_[dst], n_);
-    return launch_check("k_copy_f64");
+    const int rc = launch_check("k_copy_f64");
+    if (prof.on) {
+        (void)hipEventRecord(e1, st_);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        prof.copy_ms += ms;
+        prof.copy_launches++;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    return rc;
 }
 
 int Engine::download_coef(int f, double* host) {

@@ -938,117 +938,6 @@ static __global__ __launch_bounds__(kBlock) void k_unpack(Geo G, const double* _
 }
 
 // ----------------------------------------------------------------------------
-// Full-layout in-place SOR passes (reference layout).  Used by the implicit
-// heat step (constant coefficients, segmentation4D.c:355-403) and by the
-// reference-layout solver variant (8 coefficient arrays, :233-308).
-// Thread per (packed column m, row j) of one (k,l) row plane.
-// ----------------------------------------------------------------------------
-template <bool CONST_COEF>
-static __global__ __launch_bounds__(kBlock) void k_full_pass(Geo G, double c, double om,
-                                                     int colour, double* u,
-                                                     const double* __restrict__ prev,
-                                                     CPtr8 cf) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int k = blockIdx.y + 1, l = blockIdx.z + 1;
-    const int jr = t / G.H;
-    const int m = t - jr * G.H;
-    const int j = jr + 1;
-    if (j > G.J - 2) return;
-    const int sft = colour ^ ((j + k + l) & 1);
-    const int i = 2 * m + sft;
-    if (i < 1 || i > G.I - 2) return;
-    const int p = ((l * G.K + k) * G.J + j) * G.I + i;
-    const double uE = u[p + 1], uW = u[p - 1], uS = u[p - G.sy], uN = u[p + G.sy];
-    const double uB = u[p - G.sz], uT = u[p + G.sz], uBW = u[p - G.sw], uFW = u[p + G.sw];
-    double gs;
-    if constexpr (CONST_COEF) {
-        double nb = uE + uW;
-        nb = nb + uS;
-        nb = nb + uN;
-        nb = nb + uB;
-        nb = nb + uT;
-        nb = nb + uBW;
-        nb = nb + uFW;
-        gs = (prev[p] + c * nb) / (1 + c * 8);
-    } else {
-        const double ce = cf.p[0][p], cw = cf.p[1][p], cn = cf.p[2][p], cs = cf.p[3][p];
-        const double ct = cf.p[4][p], cb = cf.p[5][p], cfw = cf.p[6][p], cbw = cf.p[7][p];
-        double num = ce * uE;
-        num = num + cw * uW;
-        num = num + cs * uS;
-        num = num + cn * uN;
-        num = num + cb * uB;
-        num = num + ct * uT;
-        num = num + cbw * uBW;
-        num = num + cfw * uFW;
-        double dg = ce + cw;
-        dg = dg + cn;
-        dg = dg + cs;
-        dg = dg + ct;
-        dg = dg + cb;
-        dg = dg + cfw;
-        dg = dg + cbw;
-        gs = (prev[p] + c * num) / (1 + c * dg);
-    }
-    const double uo = u[p];
-    u[p] = uo + om * (gs - uo);
-}
-
-// Residual partials of the full-layout system; grid x over the interior
-// (j,i) plane, y = k-1, z = l-1.
-template <bool CONST_COEF>
-static __global__ __launch_bounds__(kBlock) void k_full_resid(Geo G, double c, double h4,
-                                                      const double* __restrict__ u,
-                                                      const double* __restrict__ prev,
-                                                      CPtr8 cf,
-                                                      double* __restrict__ partials) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int k = blockIdx.y + 1, l = blockIdx.z + 1;
-    const int w = G.I - 2;
-    double acc = 0.0;
-    if (t < w * (G.J - 2)) {
-        const int j = t / w + 1, i = t - (j - 1) * w + 1;
-        const int p = ((l * G.K + k) * G.J + j) * G.I + i;
-        const double uE = u[p + 1], uW = u[p - 1], uS = u[p - G.sy], uN = u[p + G.sy];
-        const double uB = u[p - G.sz], uT = u[p + G.sz], uBW = u[p - G.sw], uFW = u[p + G.sw];
-        double r;
-        if constexpr (CONST_COEF) {
-            double nb = uE + uW;
-            nb = nb + uS;
-            nb = nb + uN;
-            nb = nb + uB;
-            nb = nb + uT;
-            nb = nb + uBW;
-            nb = nb + uFW;
-            r = u[p] * (1 + c * 8) - c * nb - prev[p];
-        } else {
-            const double ce = cf.p[0][p], cw = cf.p[1][p], cn = cf.p[2][p], cs = cf.p[3][p];
-            const double ct = cf.p[4][p], cb = cf.p[5][p], cfw = cf.p[6][p], cbw = cf.p[7][p];
-            double num = ce * uE;
-            num = num + cw * uW;
-            num = num + cs * uS;
-            num = num + cn * uN;
-            num = num + cb * uB;
-            num = num + ct * uT;
-            num = num + cbw * uBW;
-            num = num + cfw * uFW;
-            double dg = ce + cw;
-            dg = dg + cn;
-            dg = dg + cs;
-            dg = dg + ct;
-            dg = dg + cb;
-            dg = dg + cfw;
-            dg = dg + cbw;
-            r = u[p] * (1 + c * dg) - c * num - prev[p];
-        }
-        acc = r * r * h4;
-    }
-    const double bs = block_sum256(acc);
-    if (threadIdx.x == 0)
-        partials[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = bs;
-}
-
-// ----------------------------------------------------------------------------
 // Implicit heat step (presmoothing, segmentation4D.c:337-409) on the colour
 // layout.  k_heat_pack copies the reflected input into the colour arrays
 // (both red buffers, every padded point: the ghosts stay frozen) and its

@@ -345,14 +345,14 @@ class SegLib(_Lib):
         self.lib.seg4dProfileReset()
 
     def profile_read(self):
-        out = (c_double * 23)()
-        self.lib.seg4dProfileRead(out, 23)
+        out = (c_double * 25)()
+        self.lib.seg4dProfileRead(out, 25)
         keys = ("red0_ms", "red0_launches", "red1_ms", "red1_launches",
                 "black_ms", "black_launches", "setup_ms", "setup_launches",
                 "step_ms", "steps", "voxel_updates", "iterations", "nred", "nblack",
                 "red0_f32_ms", "red0_f32_launches", "red1_f32_ms", "red1_f32_launches",
                 "black_f32_ms", "black_f32_launches", "defect_ms", "defect_launches",
-                "rounds")
+                "rounds", "copy_ms", "copy_launches")
         return dict(zip(keys, list(out)))
 
     def set_precision(self, mode):

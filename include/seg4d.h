@@ -179,8 +179,9 @@ double seg4dLastTolerance(void);
  * then setup_ms, setup_launches, step_ms, steps, voxel_updates (interior
  * voxels x SOR iterations), iterations, red points, black points;
  * out[14..19] the same three {ms, launches} pairs for the FP32 correction
- * sweeps, out[20..22] = FP64 defect ms, defect launches, correction rounds.
- * Returns the number of values written (at most 23). */
+ * sweeps, out[20..22] = FP64 defect ms, defect launches, correction rounds,
+ * out[23..24] = ms and launches of resident-mode copies (a plain stream of
+ * known size).  Returns the number of values written (at most 25). */
 void seg4dProfile(int on);
 int seg4dProfileRead(double *out, int nout);
 void seg4dProfileReset(void);
