@@ -102,7 +102,10 @@ int Engine::init(int I, int J, int K, int L, bool seg, bool mcf) {
     I_ = I; J_ = J; K_ = K; L_ = L;
     H_ = (I + 1) / 2;
     n_ = n;
-    np_ = (long)H_ * J * K * L;
+    {
+        const long pz = (((long)H_ * J + 63) / 64) * 64;
+        np_ = (64 - H_ % 64) % 64 + pz * K * L;  // cidx's layout (make_geo)
+    }
     if (check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate")) return -1;
     if (check(hipEventCreateWithFlags(&ev_, hipEventDisableTiming), "hipEventCreate")) return -1;
     auto alloc = [&](double** p, long cnt) -> int {
@@ -235,7 +238,10 @@ static Geo make_geo(int I, int J, int K, int L, int H) {
     G.I = I; G.J = J; G.K = K; G.L = L;
     G.sy = I; G.sz = I * J; G.sw = I * J * K;
     G.H = H;
-    G.py = H; G.pz = H * J; G.pw = H * J * K;
+    G.py = H;
+    G.pz = ((H * J + 63) / 64) * 64;
+    G.pw = G.pz * K;
+    G.c0 = (64 - H % 64) % 64;
     G.RP = (((J - 2) * H + 63) / 64) * 64;
     return G;
 }

@@ -31,8 +31,18 @@ struct Geo {
     int sy, sz, sw;   // full-layout strides
     int H;            // colour row length
     int py, pz, pw;   // colour-layout strides
+    int c0;           // colour-layout offset of plane (k, l) = (0, 0)
     int RP;           // operand-block plane length: (J-2)*H rounded up to 64
 };
+
+// Colour-array index of packed column m of row (j, k, l).  The plane pitch
+// pz is a multiple of 64 elements and c0 = -H mod 64, so the first interior
+// row (j = 1) of every (k, l) plane starts on a 64-element boundary: the
+// wavefronts of the passes, which walk the interior rows of a plane as one
+// flat run of points, then load whole 128-byte lines.
+__host__ __device__ __forceinline__ int cidx(const Geo& G, int j, int k, int l, int m) {
+    return G.c0 + l * G.pw + k * G.pz + j * G.py + m;
+}
 
 // Operand blocks.  The 8 coefficients and prev of every interior point of
 // one colour are stored interleaved per group of 64 consecutive points of an
@@ -318,7 +328,7 @@ static __global__ __launch_bounds__(kBlock) void k_setup_step(
         const int j = t / G.I, i = t - j * G.I;
         const int p = ((l * G.K + k) * G.J + j) * G.I + i;
         const int c = (i + j + k + l) & 1;
-        const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
+        const int pp = cidx(G, j, k, l, i >> 1);
         const double uc = u[p];
         double start = uc;
         const bool in = is_interior(G, i, j, k, l);
@@ -670,7 +680,7 @@ __device__ __forceinline__ void sor_block(
     const int j = jr + 1;
     const bool rowok = kok && j <= G.J - 2;
     double acc = 0.0;
-    const int base = (k * G.J + j) * G.H + m;  // l = 0
+    const int base = cidx(G, j, k, 0, m);  // l = 0
     // operand block of (k, l = 1, t); the next l plane is ostride further
     const long ob_off = opnd_index(G, k, 1, t);
     const T* ob0 = opnd + ob_off;
@@ -931,7 +941,7 @@ static __global__ __launch_bounds__(kBlock) void k_unpack(Geo G, const double* _
     if (t >= G.I * G.J) return;
     const int j = t / G.I, i = t - j * G.I;
     const int p = ((l * G.K + k) * G.J + j) * G.I + i;
-    const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
+    const int pp = cidx(G, j, k, l, i >> 1);
     double v = ((i + j + k + l) & 1) ? ub[pp] : ur[pp];
     if (mask != nullptr && mask[p] == 0 && is_interior(G, i, j, k, l)) v = 0;
     u[p] = v;
@@ -959,7 +969,7 @@ static __global__ __launch_bounds__(kBlock) void k_heat_pack(Geo G, const double
     if (t >= G.I * G.J) return;
     const int j = t / G.I, i = t - j * G.I;
     const int c = (i + j + k + l) & 1;
-    const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
+    const int pp = cidx(G, j, k, l, i >> 1);
     const double v = d[((l * G.K + k) * G.J + j) * G.I + i];
     if (c == 0) {
         ur0[pp] = v;
@@ -1032,7 +1042,7 @@ static __global__ __launch_bounds__(kBlock) void k_heat_sweep(
     const int m = t - jr * G.H;
     const int j = jr + 1;
     const bool rowok = kok && j <= G.J - 2;
-    const int base = (k * G.J + j) * G.H + m;
+    const int base = cidx(G, j, k, 0, m);
     double acc = 0.0;
     // w[q]: the other colour at column m, frame l - 1 + q
     double w[4];
@@ -1264,7 +1274,7 @@ static __global__ __launch_bounds__(kBlock) void k_pack_ghosts(Geo G, const doub
     const int j = t / G.I, i = t - j * G.I;
     if (is_interior(G, i, j, k, l)) return;
     const int p = ((l * G.K + k) * G.J + j) * G.I + i;
-    const int pp = ((l * G.K + k) * G.J + j) * G.H + (i >> 1);
+    const int pp = cidx(G, j, k, l, i >> 1);
     const double v = u[p];
     if ((i + j + k + l) & 1) {
         ub[pp] = v;
