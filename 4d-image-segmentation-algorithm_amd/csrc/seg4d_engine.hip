@@ -1276,9 +1276,9 @@ int Engine::gfun(const double* a, const double* b, double h, double K, double th
         g.p[f] = full_[A_G0 + f];
         if (!g.p[f]) return check(hipErrorInvalidValue, "gfun: no g arrays");
     }
-    // XCD-resident tiles over the interior (runs of kBlock points of the
-    // interior (j,i) plane, k - 1, l - 1), shaped as the set-up kernel's
-    const PlaneMap PM = plane_map(cdiv((long)(I_ - 2) * (J_ - 2), kBlock), K_ - 2, L_ - 2);
+    // XCD-resident tiles over the interior planes (runs of kBlock points of
+    // the padded (j,i) plane, k - 1, l - 1), shaped as the set-up kernel's
+    const PlaneMap PM = plane_map(cdiv((long)I_ * J_, kBlock), K_ - 2, L_ - 2);
     const dim3 grid(plane_grid(PM));
     // the second image's term can be dropped only where it is known to be a
     // signed zero: rho == 0 and every value of b finite

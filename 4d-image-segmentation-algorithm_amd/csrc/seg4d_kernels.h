@@ -1156,10 +1156,13 @@ static __global__ __launch_bounds__(kBlock) void k_gfun(Geo G, PlaneMap PM, Scal
                                                 const double* __restrict__ b, Ptr8 g) {
     int bx, kk, ll;
     if (!map_plane_block(PM, blockIdx.x, bx, kk, ll)) return;
+    // lanes walk whole padded rows (lanes on ghost columns and rows idle), so
+    // that a wavefront's stores cover whole 128-byte lines of the g arrays
+    // (3.47 -> 3.14 ms against lanes on the interior points only, 256x256x128x16)
     const int t = bx * kBlock + threadIdx.x;
-    const int w = G.I - 2;
-    if (t >= w * (G.J - 2)) return;
-    const int j = t / w + 1, i = t - (j - 1) * w + 1;
+    if (t >= G.I * G.J) return;
+    const int j = t / G.I, i = t - j * G.I;
+    if (i < 1 || i > G.I - 2 || j < 1 || j > G.J - 2) return;
     const int k = kk + 1, l = ll + 1;
     const int p = ((l * G.K + k) * G.J + j) * G.I + i;
     const int st[4] = {1, G.sy, G.sz, G.sw};
