@@ -132,7 +132,11 @@ int Engine::init(int I, int J, int K, int L, bool seg, bool mcf) {
     // (a SOR pass has at most cdiv(points per row plane, kBlock / 4) runs
     // per k plane and one frame per block)
     const long g_sweep = (long)cdiv((long)H_ * (J - 2), kBlock / 4) * (K - 2) * (L - 2);
+    // (the set-up kernel: blocks of 128 packed columns of a plane's rows)
+    const long g_setup =
+        cdiv(cdiv((long)(J - 1) * H_, 32) + (H_ + 31) / 32, 4) * K * L;
     long np = g_full > g_sweep ? g_full : g_sweep;
+    if (g_setup > np) np = g_setup;
     if (np < 4096) np = 4096;
     npart_ = np;
     if (alloc(&part_, 2 * np)) return -1;
@@ -388,8 +392,13 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
             return check(hipErrorUnknown, net_->err.c_str());
         warm = 2;
     }
-    const PlaneMap PM = plane_map((int)cdiv((long)I_ * J_, kBlock), K_, L_);
+    // blocks of 4 wavefronts, each on 32 packed columns of the plane's rows
+    // (k_setup_step): chunks q from -ceil(H/32) to the last row's end
+    const long nq = cdiv((long)(J_ - 1) * H_, 32) + (H_ + 31) / 32;
+    const PlaneMap PM = plane_map((int)cdiv(nq, 4), K_, L_);
     const dim3 grid(plane_grid(PM));
+    if ((long)PM.gx * PM.gk * PM.gl > npart_)
+        return check(hipErrorInvalidValue, "setup: partials exceed their buffer");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof.on) {
         (void)hipEventCreate(&e0);
