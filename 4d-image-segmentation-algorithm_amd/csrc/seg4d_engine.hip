@@ -925,8 +925,8 @@ int Engine::learned_preview(float* host_c, double* host_q) {
 
 int Engine::unpack(const double* dmask) {
     const Geo G = make_geo(I_, J_, K_, L_, H_);
-    dim3 grid(cdiv((long)I_ * J_, kBlock), K_, L_);
-    k_unpack<<<grid, kBlock, 0, st_>>>(G, ur_[red_final_], ub_, full_[A_U], dmask);
+    k_unpack<<<(unsigned)cdiv(n_, kBlock), kBlock, 0, st_>>>(G, ur_[red_final_], ub_, full_[A_U],
+                                                            dmask);
     return launch_check("k_unpack");
 }
 
@@ -1268,7 +1268,7 @@ int Engine::presmooth(double* d, double sigma, double h, double omega, double gs
                                                    hp_, part_b);
     }
     // the accepted red values are red(z)'s; every padded point back to d
-    k_unpack<<<gfull, kBlock, 0, st_>>>(G, ur_[z & 1], ub_, d, nullptr);
+    k_unpack<<<(unsigned)cdiv(n_, kBlock), kBlock, 0, st_>>>(G, ur_[z & 1], ub_, d, nullptr);
     if (launch_check("k_unpack heat")) return -1;
     if (iters) *iters = z;
     if (Rout) *Rout = R;

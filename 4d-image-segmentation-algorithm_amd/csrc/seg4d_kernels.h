@@ -951,11 +951,17 @@ static __global__ __launch_bounds__(kBlock) void k_unpack(Geo G, const double* _
                                                   const double* __restrict__ ub,
                                                   double* __restrict__ u,
                                                   const double* __restrict__ mask) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    const int k = blockIdx.y, l = blockIdx.z;
-    if (t >= G.I * G.J) return;
-    const int j = t / G.I, i = t - j * G.I;
-    const int p = ((l * G.K + k) * G.J + j) * G.I + i;
+    // one thread per padded point in array order (a 1-D grid of
+    // cdiv(I J K L, kBlock) blocks), so that a wavefront's stores of u cover
+    // whole lines whatever the row length
+    const long q = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (q >= (long)G.I * G.J * G.K * G.L) return;
+    const int p = (int)q;
+    int r = p / G.I;
+    const int i = p - r * G.I;
+    const int j = r % G.J;
+    r /= G.J;
+    const int k = r % G.K, l = r / G.K;
     const int pp = cidx(G, j, k, l, i >> 1);
     double v = ((i + j + k + l) & 1) ? ub[pp] : ur[pp];
     if (mask != nullptr && mask[p] == 0 && is_interior(G, i, j, k, l)) v = 0;
