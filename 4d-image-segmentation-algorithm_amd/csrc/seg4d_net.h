@@ -3,8 +3,8 @@
 // A separable 4D convolutional network predicts the next time step's
 // solution from u^n, its recent increments and g, and the solver starts SOR
 // from that prediction instead of u^n; the stopping test is unchanged, so
-// the plain method stays the judge and a better start only shows up as fewer
-// sweeps.
+// the plain method still decides convergence and a better start only shows
+// up as fewer sweeps.
 //
 // Network (parameters FP32, kernels BF16 with FP32 accumulation):
 //   d = u^n - u^(n-1), sigma = rms of d over the interior (this step's motion)

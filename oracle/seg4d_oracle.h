@@ -84,8 +84,9 @@ double orc_residual(const orc_params *p, const double *u, const double *prev,
 /* a10 */ void orc_rescale(const orc_params *p, double *u, const double *mask,
                           const orc_balls *balls);
 
-/* Segmentation initialisation (survey next-row 1), ball boxes clamped to the
- * array like orc_rescale. */
+/* Segmentation initialisation (segmentation4D.c:317-335 mass, :876-1018
+ * maxima, ball minima / maxima and threshold, :1106-1160 seed function),
+ * ball boxes clamped to the array like orc_rescale. */
 double orc_get4dmax(const orc_params *p, const double *img);
 double orc_ball_minmax(const orc_params *p, const double *img, int l, double cx,
                        double cy, double cz, double R, int want_max);
