@@ -85,7 +85,7 @@ double orc_residual(const orc_params *p, const double *u, const double *prev,
                           const orc_balls *balls);
 
 /* Segmentation initialisation (segmentation4D.c:317-335 mass, :876-1018
- * maxima, ball minima / maxima and threshold, :1106-1160 seed function),
+ * maxima, ball minima / maxima and threshold, :1106-1147 seed function),
  * ball boxes clamped to the array like orc_rescale. */
 double orc_get4dmax(const orc_params *p, const double *img);
 double orc_ball_minmax(const orc_params *p, const double *img, int l, double cx,
