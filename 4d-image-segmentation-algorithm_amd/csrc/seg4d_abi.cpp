@@ -175,6 +175,19 @@ void zero_fill(double* p, size_t n) {
     for (auto& x : th) x.join();
 }
 
+// dst <- src on up to 8 host threads
+void host_copy(double* dst, const double* src, size_t n) {
+    const unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    const size_t per = (n + nt - 1) / nt;
+    for (unsigned q = 0; q < nt; q++) {
+        const size_t lo = q * per, hi = std::min(n, lo + per);
+        if (lo < hi)
+            th.emplace_back([=] { std::memcpy(dst + lo, src + lo, (hi - lo) * sizeof(double)); });
+    }
+    for (auto& x : th) x.join();
+}
+
 double* host_alloc(size_t n, bool pin) {
     if (pin) {
         void* p = nullptr;
@@ -322,18 +335,33 @@ API bool _gs_coefficients_(void) {
 // ---- segmentation4D.c:215-315 ------------------------------------------------
 API bool _segTimeStep_(void) {
     if (!need_engine("_segTimeStep_")) return false;
-    if (!g_resident) {
-        if (g_eng->upload(s4d::A_U, u) || g_eng->upload(s4d::A_MASK, Mmask))
-            return fail("_segTimeStep_: upload");
-    }
     if (!upload_g_if_dirty()) return fail("_segTimeStep_: upload g");
+    // Mirror mode: u goes up before the step; Mmask, read only by the step's
+    // epilogue, goes up beside it on a second stream; the reference's
+    // prev_u <- u (segmentation4D.c:224) is done on the host while the device
+    // computes; after the step only u comes down.
+    std::thread prev_copy;
+    if (!g_resident) {
+        if (g_eng->upload(s4d::A_U, u) || g_eng->upload_side(s4d::A_MASK, Mmask)) {
+            g_eng->finish_side();
+            return fail("_segTimeStep_: upload");
+        }
+        prev_copy = std::thread([] { host_copy(prev_u, u, (size_t)nelem()); });
+    }
     const s4d::Balls balls = collect_balls();
     s4d::SolveStats st;
-    if (g_eng->seg_step(step_params(), &balls, &st)) return fail("_segTimeStep_");
+    const int rc = g_eng->seg_step(step_params(), &balls, &st);
+    if (prev_copy.joinable()) prev_copy.join();
+    const int rs = g_eng->finish_side();
+    if (rc || rs) {
+        // the device put its history back (u^n in u, the level before in
+        // prev); prev_u receives that level again
+        if (!g_resident) (void)g_eng->download(s4d::A_PREV, prev_u);
+        return fail("_segTimeStep_");
+    }
     g_last = st;
     if (!g_resident) {
-        if (g_eng->download(s4d::A_U, u) || g_eng->download(s4d::A_PREV, prev_u))
-            return fail("_segTimeStep_: download");
+        if (g_eng->download(s4d::A_U, u)) return fail("_segTimeStep_: download");
     }
     if (g_verbose) {
         std::printf("Step is %d\n", numberOfTimeStep);

@@ -90,6 +90,12 @@ public:
 
     int upload(ArrayId a, const double* host);
     int download(ArrayId a, double* host);
+    // Upload on a second stream (default priority) that overlaps the work
+    // queued on the engine's stream; the next consumer (the step's epilogue,
+    // any other transfer, sync) waits for it.  `host` must stay unchanged
+    // until finish_side() has returned.
+    int upload_side(ArrayId a, const double* host);
+    int finish_side();  // the side copy has completed (host buffer free again)
     int copy(ArrayId src, ArrayId dst);
     // Coefficient face f (0..7, order e w n s t b fw bw) in full layout.
     int download_coef(int f, double* host);
@@ -244,6 +250,10 @@ public:
     SorCtl* ctl_ = nullptr;      // device control block
     SorCtl* hctl_ = nullptr;     // pinned host mirror
     hipEvent_t ev_ = nullptr;
+    hipStream_t side_ = nullptr;   // upload_side's stream
+    hipEvent_t side_ev_ = nullptr;
+    bool side_pending_ = false;    // the engine's stream has not waited for it yet
+    int wait_side();
     std::vector<hipEvent_t> pev_; // profiling events (pairs)
     int red_final_ = 0;          // which red buffer holds the accepted state
     bool split_ = false;         // op_ holds the split operands of the FP32-refined set-up

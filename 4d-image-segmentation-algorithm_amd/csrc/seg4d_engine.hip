@@ -201,6 +201,14 @@ void Engine::release() {
     pev_.clear();
     if (ev_) hipEventDestroy(ev_);
     ev_ = nullptr;
+    if (side_) {
+        (void)hipStreamSynchronize(side_);
+        (void)hipStreamDestroy(side_);
+    }
+    if (side_ev_) (void)hipEventDestroy(side_ev_);
+    side_ = nullptr;
+    side_ev_ = nullptr;
+    side_pending_ = false;
     if (st_ && own_st_) hipStreamDestroy(st_);
     st_ = nullptr;
     own_st_ = true;
@@ -250,8 +258,42 @@ static Geo make_geo(int I, int J, int K, int L, int H) {
     return G;
 }
 
+int Engine::upload_side(ArrayId a, const double* host) {
+    if (!full_[a]) return check(hipErrorInvalidValue, "upload: array not allocated");
+    if (!side_ && check(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking), "side stream"))
+        return -1;
+    if (!side_ev_ &&
+        check(hipEventCreateWithFlags(&side_ev_, hipEventDisableTiming), "side event"))
+        return -1;
+    if (wait_side()) return -1;
+    // the array may still be read by work queued on the engine's stream
+    if (check(hipEventRecord(ev_, st_), "event") || check(hipStreamWaitEvent(side_, ev_, 0), "wait"))
+        return -1;
+    if (a == A_U || a == A_PREV) hist_ok_ = hist2_ok_ = hist3_ok_ = false;
+    if (a >= A_G0 && a <= A_G7) g_version_++;
+    if (check(hipMemcpyAsync(full_[a], host, (size_t)n_ * sizeof(double), hipMemcpyHostToDevice,
+                             side_), "side upload") ||
+        check(hipEventRecord(side_ev_, side_), "side event"))
+        return -1;
+    side_pending_ = true;
+    return 0;
+}
+
+int Engine::wait_side() {
+    if (!side_pending_) return 0;
+    side_pending_ = false;
+    return check(hipStreamWaitEvent(st_, side_ev_, 0), "wait for the side copy");
+}
+
+int Engine::finish_side() {
+    if (!side_) return 0;
+    const int rc = wait_side();
+    return check(hipStreamSynchronize(side_), "side copy") | rc;
+}
+
 int Engine::upload(ArrayId a, const double* host) {
     if (!full_[a]) return check(hipErrorInvalidValue, "upload: array not allocated");
+    if (wait_side()) return -1;
     if (a == A_U || a == A_PREV) hist_ok_ = hist2_ok_ = hist3_ok_ = false;
     if (a >= A_G0 && a <= A_G7) g_version_++;
     if (check(hipMemcpyAsync(full_[a], host, (size_t)n_ * sizeof(double),
@@ -262,6 +304,7 @@ int Engine::upload(ArrayId a, const double* host) {
 
 int Engine::download(ArrayId a, double* host) {
     if (!full_[a]) return check(hipErrorInvalidValue, "download: array not allocated");
+    if (wait_side()) return -1;
     if (check(hipMemcpyAsync(host, full_[a], (size_t)n_ * sizeof(double),
                              hipMemcpyDeviceToHost, st_), "download"))
         return -1;
@@ -924,6 +967,7 @@ int Engine::learned_preview(float* host_c, double* host_q) {
 }
 
 int Engine::unpack(const double* dmask) {
+    if (wait_side()) return -1;  // e.g. the mask uploaded beside the step
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     k_unpack<<<(unsigned)cdiv(n_, kBlock), kBlock, 0, st_>>>(G, ur_[red_final_], ub_, full_[A_U],
                                                             dmask);
@@ -995,6 +1039,7 @@ int Engine::put_balls(const Balls& b, BallList* out) {
 }
 
 int Engine::rescale(double* d, const Balls& balls, bool apply_mask) {
+    if (wait_side()) return -1;
     const Geo G = make_geo(I_, J_, K_, L_, H_);
     hist_ok_ = hist2_ok_ = hist3_ok_ = false;  // u changed outside a time step
     if (balls.total > 0) {
@@ -1387,6 +1432,9 @@ int Engine::mcf_exact(double* d, double h, double t, bool ghosts_only) {
     return launch_check("k_mcf_exact");
 }
 
-int Engine::sync() { return check(hipStreamSynchronize(st_), "sync"); }
+int Engine::sync() {
+    if (wait_side()) return -1;
+    return check(hipStreamSynchronize(st_), "sync");
+}
 
 }  // namespace s4d
