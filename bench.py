@@ -412,7 +412,7 @@ def main():
             ms.append((time.perf_counter() - t) * 1e3)
         mirror = {"ms_per_step": ms, "resident_ms_per_step_same_steps":
                   total / args.steps * 1e3,
-                  "pcie_bytes_per_step": 4 * 8 * int(np.prod(shape))}
+                  "pcie_bytes_per_step": 3 * 8 * int(np.prod(shape))}  # u up, Mmask up, u down
         log(f"[bench] mirror mode: {[round(x, 1) for x in ms]} ms per step")
         lib.set_resident(True)
 
