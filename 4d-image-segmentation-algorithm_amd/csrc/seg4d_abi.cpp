@@ -257,12 +257,11 @@ API bool _allocateMem_(void) {
     free_host();
     double** all[] = {&u, &prev_u, &i0, &Mmask, &e, &w, &n, &s, &t, &b, &fw, &bw,
                       &g_e, &g_w, &g_n, &g_s, &g_t, &g_b, &g_fw, &g_bw};
-    // pinned: u, prev_u, i0, Mmask always; the g arrays while each is at most
-    // 2 GB (larger volumes keep 8 x that much pageable: they cross PCIe once)
-    const bool pin_g = (size_t)N * sizeof(double) <= (size_t(2) << 30);
+    // pinned: u, prev_u, i0, Mmask and the g arrays (the coefficient arrays
+    // cross PCIe only on request)
     int q = 0;
     for (double** p : all) {
-        const bool pin = q < 4 || (q >= 12 && pin_g);
+        const bool pin = q < 4 || q >= 12;
         q++;
         *p = host_alloc((size_t)N, pin);
         if (!*p) {

@@ -209,11 +209,13 @@ def main():
     nx, ny, nz, nt = spec.nx, spec.ny, spec.nz, spec.nt
     shape = (nt + 2, nz + 2, ny + 2, nx + 2)
     nint = nx * ny * nz * nt
-    lib.allocate(nx, ny, nz, nt)
+    ta = time.perf_counter()
+    lib.allocate(nx, ny, nz, nt)  # includes page-locking and zero-filling the host arrays
+    alloc_s = time.perf_counter() - ta
     params(lib)
     if args.variant is not None:
         lib.set_sweep_variant(args.variant)
-    log(f"[bench] volume {nx}x{ny}x{nz}x{nt} on {lib.device_info()}")
+    log(f"[bench] volume {nx}x{ny}x{nz}x{nt} on {lib.device_info()}; allocated in {alloc_s:.2f} s")
     t = time.perf_counter()
     u0 = np.zeros(shape)
     lib.generate(spec, lib.arr("i0"), u0)
@@ -471,6 +473,7 @@ def main():
         "time_to_tolerance_s": total,
         "sweeps_per_step": sweeps,
         "setup_ms_per_step": prof["setup_ms"] / max(prof["setup_launches"], 1),
+        "allocate_s": alloc_s,
         "edge_indicator_s_pcie_inclusive": edge_s,
         "edge_indicator_ms_resident": edge_dev_s * 1e3,
         "edge_indicator_presmoothing_sweeps": edge_iters,
