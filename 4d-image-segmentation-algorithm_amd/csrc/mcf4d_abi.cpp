@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
+#include "host_mem.h"
 #include "seg4d_engine.h"
 
 #define API extern "C" __attribute__((visibility("default")))
@@ -89,10 +91,14 @@ s4d::StepParams step_params() {
     return P;
 }
 
+// u, prev_u and exact_u cross PCIe at every step in mirror mode: pinned
+// (page-locked); the coefficient arrays only on request: calloc.
+s4d::HostArrays g_host;
+
 void free_host() {
     double** all[] = {&u, &prev_u, &exact_u, &e, &w, &n, &s, &t, &b, &fw, &bw};
     for (double** p : all) {
-        std::free(*p);
+        g_host.free(*p);
         *p = nullptr;
     }
 }
@@ -119,8 +125,11 @@ API bool _allocateMem_mcf(void) {
     }
     free_host();
     double** all[] = {&u, &prev_u, &exact_u, &e, &w, &n, &s, &t, &b, &fw, &bw};
+    int q = 0;
     for (double** p : all) {
-        *p = (double*)std::calloc((size_t)N, sizeof(double));
+        // zero-filled like the reference's (mcf_4D.c:42-63 mallocs; its
+        // main fills every array before reading it)
+        *p = g_host.alloc((size_t)N, q++ < 3);
         if (!*p) {
             free_host();
             return fail("_allocateMem_mcf: host allocation failed");
@@ -164,13 +173,22 @@ API double _l2norm_mcf(double* a, double* bb) {
 API bool _mcfTimeStep(void) {
     if (!need_engine("_mcfTimeStep")) return false;
     const double time = tau * numberOfTimeStep;
+    // mirror mode: u up; the reference's prev_u <- u (mcf_4D.c:113) on host
+    // threads while the device computes; u down
     if (!g_resident && g_eng->upload(s4d::A_U, u)) return fail("_mcfTimeStep: upload");
+    std::thread prev_copy;
+    if (!g_resident) prev_copy = std::thread([] { s4d::host_copy(prev_u, u, (size_t)nelem()); });
     s4d::SolveStats st;
-    if (g_eng->mcf_step(step_params(), time, &st)) return fail("_mcfTimeStep");
+    const int rc = g_eng->mcf_step(step_params(), time, &st);
+    if (prev_copy.joinable()) prev_copy.join();
+    if (rc) {
+        // the device put its history back; prev_u receives that level again
+        if (!g_resident) (void)g_eng->download(s4d::A_PREV, prev_u);
+        return fail("_mcfTimeStep");
+    }
     g_last = st;
     if (!g_resident) {
-        if (g_eng->download(s4d::A_U, u) || g_eng->download(s4d::A_PREV, prev_u))
-            return fail("_mcfTimeStep: download");
+        if (g_eng->download(s4d::A_U, u)) return fail("_mcfTimeStep: download");
     }
     if (g_verbose) {
         std::printf("Residuum is %e\n", st.R);

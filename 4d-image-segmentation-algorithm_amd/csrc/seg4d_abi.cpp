@@ -11,6 +11,7 @@
 #include <thread>
 #include <vector>
 
+#include "host_mem.h"
 #include "seg4d_batch.h"
 #include "seg4d_engine.h"
 #include "seg4d_net.h"
@@ -162,61 +163,13 @@ bool upload_g_if_dirty() {
 // the runtime's staging buffers; the others, and any array whose pinning
 // fails, come from calloc.  Both are zero-filled, as the reference's
 // allocation (segmentation4D.c:106-133).
-std::vector<double*> g_pinned;
-
-void zero_fill(double* p, size_t n) {
-    const unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    const size_t per = (n + nt - 1) / nt;
-    for (unsigned q = 0; q < nt; q++) {
-        const size_t lo = q * per, hi = std::min(n, lo + per);
-        if (lo < hi) th.emplace_back([=] { std::memset(p + lo, 0, (hi - lo) * sizeof(double)); });
-    }
-    for (auto& x : th) x.join();
-}
-
-// dst <- src on up to 8 host threads
-void host_copy(double* dst, const double* src, size_t n) {
-    const unsigned nt = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    const size_t per = (n + nt - 1) / nt;
-    for (unsigned q = 0; q < nt; q++) {
-        const size_t lo = q * per, hi = std::min(n, lo + per);
-        if (lo < hi)
-            th.emplace_back([=] { std::memcpy(dst + lo, src + lo, (hi - lo) * sizeof(double)); });
-    }
-    for (auto& x : th) x.join();
-}
-
-double* host_alloc(size_t n, bool pin) {
-    if (pin) {
-        void* p = nullptr;
-        if (hipHostMalloc(&p, n * sizeof(double), hipHostMallocDefault) == hipSuccess && p) {
-            zero_fill(static_cast<double*>(p), n);
-            g_pinned.push_back(static_cast<double*>(p));
-            return static_cast<double*>(p);
-        }
-        (void)hipGetLastError();  // pageable memory instead
-    }
-    return static_cast<double*>(std::calloc(n, sizeof(double)));
-}
-
-void host_free(double* p) {
-    if (!p) return;
-    auto it = std::find(g_pinned.begin(), g_pinned.end(), p);
-    if (it != g_pinned.end()) {
-        g_pinned.erase(it);
-        (void)hipHostFree(p);
-    } else {
-        std::free(p);
-    }
-}
+s4d::HostArrays g_host;
 
 void free_host() {
     double** all[] = {&u, &prev_u, &i0, &Mmask, &e, &w, &n, &s, &t, &b, &fw, &bw,
                       &g_e, &g_w, &g_n, &g_s, &g_t, &g_b, &g_fw, &g_bw};
     for (double** p : all) {
-        host_free(*p);
+        g_host.free(*p);
         *p = nullptr;
     }
     double*** cs[] = {&center_x, &center_y, &center_z};
@@ -263,7 +216,7 @@ API bool _allocateMem_(void) {
     for (double** p : all) {
         const bool pin = q < 4 || q >= 12;
         q++;
-        *p = host_alloc((size_t)N, pin);
+        *p = g_host.alloc((size_t)N, pin);
         if (!*p) {
             free_host();
             return fail("_allocateMem_: host allocation failed");
@@ -345,7 +298,7 @@ API bool _segTimeStep_(void) {
             g_eng->finish_side();
             return fail("_segTimeStep_: upload");
         }
-        prev_copy = std::thread([] { host_copy(prev_u, u, (size_t)nelem()); });
+        prev_copy = std::thread([] { s4d::host_copy(prev_u, u, (size_t)nelem()); });
     }
     const s4d::Balls balls = collect_balls();
     s4d::SolveStats st;
