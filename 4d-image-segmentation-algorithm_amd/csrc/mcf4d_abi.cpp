@@ -181,11 +181,9 @@ API bool _mcfTimeStep(void) {
     s4d::SolveStats st;
     const int rc = g_eng->mcf_step(step_params(), time, &st);
     if (prev_copy.joinable()) prev_copy.join();
-    if (rc) {
-        // the device put its history back; prev_u receives that level again
-        if (!g_resident) (void)g_eng->download(s4d::A_PREV, prev_u);
-        return fail("_mcfTimeStep");
-    }
+    // a failed step leaves u as it was; in mirror mode prev_u then holds u^n,
+    // as after the reference's first statement
+    if (rc) return fail("_mcfTimeStep");
     g_last = st;
     if (!g_resident) {
         if (g_eng->download(s4d::A_U, u)) return fail("_mcfTimeStep: download");

@@ -305,12 +305,9 @@ API bool _segTimeStep_(void) {
     const int rc = g_eng->seg_step(step_params(), &balls, &st);
     if (prev_copy.joinable()) prev_copy.join();
     const int rs = g_eng->finish_side();
-    if (rc || rs) {
-        // the device put its history back (u^n in u, the level before in
-        // prev); prev_u receives that level again
-        if (!g_resident) (void)g_eng->download(s4d::A_PREV, prev_u);
-        return fail("_segTimeStep_");
-    }
+    // a failed step leaves u as it was; in mirror mode prev_u then holds u^n,
+    // as after the reference's first statement
+    if (rc || rs) return fail("_segTimeStep_");
     g_last = st;
     if (!g_resident) {
         if (g_eng->download(s4d::A_U, u)) return fail("_segTimeStep_: download");

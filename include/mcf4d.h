@@ -38,7 +38,8 @@ double _l2norm_mcf(double *dataArray3DPtr1, double *dataArray3DPtr2);
 /* mcf_4D.h:14 / mcf_4D.c:208-342: coefficients with g == 1 */
 bool _gs_coefficients_mcf(void);
 /* mcf_4D.h:16 / mcf_4D.c:102-205: copy, coefficients, exact boundary at
- * t = tau * numberOfTimeStep, red-black SOR */
+ * t = tau * numberOfTimeStep, red-black SOR.  On failure (false) u is as
+ * before the call; prev_u too in resident mode, prev_u = u in mirror mode. */
 bool _mcfTimeStep(void);
 /* mcf_4D.h:18 / mcf_4D.c:344-443 */
 bool _set4DBoundaryToExactValues_mcf(double t);

@@ -83,7 +83,8 @@ double _l2norm_(double *dataArray3DPtr1, double *dataArray3DPtr2);
 bool _gs_coefficients_(void);
 /* segmentation4D.h:16 / segmentation4D.c:215-315: prev_u <- u,
  * coefficients, red-black SOR to (R <= gsTol or maxNoGSIteration sweeps),
- * rescale epilogue. */
+ * rescale epilogue.  On failure (false) u is as before the call; prev_u
+ * too in resident mode, prev_u = u in mirror mode. */
 bool _segTimeStep_(void);
 /* segmentation4D.h:18 / segmentation4D.c:337-409: implicit heat step in
  * place (prev_u receives the input; ghosts reflected once). */

@@ -179,3 +179,44 @@ def test_failed_step_keeps_u_and_drops_history(seglib, oracle, where):
         lib.set_warm_start(0)
         lib.set_resident(False)
         lib.deallocate()
+
+
+@pytest.mark.parametrize("where", [1, 2])
+def test_failed_mirror_step_keeps_host_arrays(seglib, oracle, where):
+    """Mirror mode (the default), where the step copies u into prev_u on the
+    host while the device computes and uploads Mmask beside it: a step made to
+    fail after its set-up (1) or at its final wait (2) leaves the host u as it
+    was, prev_u holds u^n (the reference's first statement, prev_u <- u, has
+    run), and the next step equals the oracle's step from u^n."""
+    print(f"[gpu] failure: injected mirror-mode step failure {where}", flush=True)
+    lib = seglib
+    dims = (24, 20, 12, 6)
+    shape, img, u0 = _generated(lib, dims, 19)
+    lib.allocate(*dims)
+    try:
+        lib.set(gsTol=0.0, maxNoGSIteration=100000, **KW)
+        lib.arr("i0")[...] = img
+        lib.arr("u")[...] = 0.0
+        lib.arr("Mmask")[...] = 1.0
+        lib.gf("i0", "u")
+        g = [lib.arr("g_" + f).copy() for f in FACES]
+        tol = 1e-12 * oracle.sumsq_interior(params(shape, **KW), u0)
+        lib.set(gsTol=tol)
+        lib.arr("u")[...] = u0
+        z1, _ = lib.seg_time_step()
+        u1, p1 = lib.arr("u").copy(), lib.arr("prev_u").copy()
+        assert np.array_equal(p1, u0)  # the reference's prev_u <- u
+        lib.inject_step_failure(where)
+        assert lib.lib._segTimeStep_() is False
+        assert "injected" in lib.last_error()
+        assert np.array_equal(lib.arr("u"), u1)
+        assert np.array_equal(lib.arr("prev_u"), u1)
+        z2, _ = lib.seg_time_step()
+        u, prev = u1.copy(), np.zeros(shape)
+        zo, _ = oracle.seg_time_step(params(shape, gsTol=tol, maxNoGSIteration=100000, **KW),
+                                     u, prev, g, zeros(shape, 8), np.ones(shape))
+        assert z2 == zo
+        assert np.array_equal(lib.arr("u"), u)
+        assert np.array_equal(lib.arr("prev_u"), u1)
+    finally:
+        lib.deallocate()
