@@ -912,14 +912,36 @@ API double seg4dBatchLastStepMs(void) { return g_batch ? g_batch->last_step_ms :
 API int seg4dBatchConfigure(int precision, int warm, int graph, int depth) {
     if (!need_batch("seg4dBatchConfigure")) return -1;
     if (precision < 0 || precision > 1 || !(warm == 0 || warm == 1 || warm == 3 || warm == 4) ||
-        graph < 0 || graph > 1 || depth < 1 || depth > 16) {
-        s4d::set_error("seg4dBatchConfigure: precision 0/1, warm 0/1/3/4, graph 0/1, depth 1..16");
+        graph < 0 || graph > 2 || depth < 1 || depth > 16) {
+        s4d::set_error("seg4dBatchConfigure: precision 0/1, warm 0/1/3/4, graph 0/1/2, depth 1..16");
         return -1;
     }
     g_batch->precision = precision;
     g_batch->warm = warm;
     g_batch->graph = graph;
     g_batch->depth = depth;
+    return 0;
+}
+
+API int seg4dBatchGraphInfo(int* iters, int* rounds, long* steps, long* redone,
+                            double* host_ms) {
+    if (!need_batch("seg4dBatchGraphInfo")) return -1;
+    if (host_ms) *host_ms = g_batch->graph_host_ms;
+    if (iters) *iters = g_batch->graph_iters;
+    if (rounds) *rounds = g_batch->graph_rounds;
+    if (steps) *steps = g_batch->graph_steps;
+    if (redone) *redone = g_batch->graph_redone;
+    return 0;
+}
+
+API int seg4dBatchSetGraphPlan(int iters, int rounds) {
+    if (!need_batch("seg4dBatchSetGraphPlan")) return -1;
+    if (iters < 0 || rounds < 0 || (iters > 0) != (rounds > 0)) {
+        s4d::set_error("seg4dBatchSetGraphPlan: iters and rounds both >= 1, or both 0");
+        return -1;
+    }
+    g_batch->graph_iters = iters;
+    g_batch->graph_rounds = rounds;
     return 0;
 }
 

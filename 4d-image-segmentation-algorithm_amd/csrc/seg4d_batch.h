@@ -7,8 +7,13 @@
 // batched launches over per-volume descriptors; every decision is taken on
 // the device per volume, the host only waits for "all volumes stopped".  The
 // iterations of a solve can be replayed from a captured HIP graph (two
-// iterations per replay).  Each volume's result is bitwise that of the
-// same step on a single-volume engine with the same settings.
+// iterations per replay), or the whole time step can run as one graph
+// (graph mode 2: set-up, the solve's iterations, the correction rounds and
+// the epilogue, re-captured every step because the history buffers rotate;
+// the iterations it holds follow the previous step's sweep counts, and a
+// step whose solve needs more is redone eagerly).  Each volume's result is
+// bitwise that of the same step on a single-volume engine with the same
+// settings.
 //
 // The concurrent form (streams > 0) instead splits the volumes into groups
 // of about equal voxel count, one stream and one host thread per group, and
@@ -60,7 +65,8 @@ public:
 
     int precision = 0;      // 0 FP64 (the reference's arithmetic), 1 FP32-refined
     int warm = 0;           // SOR start (Engine::warm_ modes 0, 1, 3, 4)
-    int graph = 0;          // 1: solve iterations replayed from a captured graph
+    int graph = 0;          // 1: solve iterations replayed from a captured graph; 2: the
+                            // whole step as one graph
     int depth = 2;          // iterations (graph: replays) queued ahead of the test
     int streams = 0;        // 0: one launch per pass for all volumes; k: k concurrent groups
     int max_rounds = 8;
@@ -68,12 +74,23 @@ public:
     std::vector<SolveStats> stats;  // last step, per volume
     double last_step_ms = 0.0;      // device time (concurrent form: host wall time)
     long last_iterations = 0;       // sum over volumes of SOR / FP32 sweeps
+    // graph mode 2: iterations per solve and correction rounds the next
+    // captured step holds (from the last step: its largest sweep count plus a
+    // margin), steps run as one graph and steps redone eagerly after an overrun
+    int graph_iters = 0, graph_rounds = 0;
+    long graph_steps = 0, graph_redone = 0;
+    double graph_host_ms = 0.0;     // host time spent capturing and instantiating, all steps
     std::string err;
 
 private:
     int check(hipError_t e, const char* what);
     int build_jobs(const StepParams& P);
     int step_streams(const StepParams& P);
+    int step_eager(const StepParams& P, int cap);
+    int step_graph(const StepParams& P, int cap);  // 1: overran, redo eagerly
+    template <typename T>
+    void enqueue_iters(BatchJob<T>* dj, SorCtl* dctl, int cap, int nit);
+    void plan_graph(int max_sweeps, int max_rounds_used);
     int use_streams(int k);  // engines onto k group streams (0: the batch stream)
     template <typename T>
     int sor_run(std::vector<BatchJob<T>>& hj, BatchJob<T>* dj, SorCtl* dctl, SorCtl* hctl,
@@ -98,6 +115,10 @@ private:
     double jobs_c_ = 0.0, jobs_om_ = 0.0, jobs_om32_ = 0.0, jobs_h4_ = 0.0;
     int jobs_prec_ = -1;
     hipGraphExec_t g64_ = nullptr, g32_ = nullptr;
+    hipGraphExec_t gstep_ = nullptr;  // the captured time step (updated every step)
+    int* gate_ = nullptr;     // [2 + n] device gate of the captured step (k_gate_*)
+    int* hgate_ = nullptr;    // pinned mirror
+    int rounds_seen_ = 0;     // most correction rounds of any step so far
     int g64_cap_ = -1, g32_cap_ = -1;
     std::vector<hipEvent_t> evq_;
     std::vector<hipStream_t> gst_;           // group streams of the concurrent form

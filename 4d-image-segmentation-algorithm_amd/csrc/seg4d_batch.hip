@@ -30,8 +30,15 @@ void Batch::release() {
     if (st_) (void)hipStreamSynchronize(st_);
     if (g64_) (void)hipGraphExecDestroy(g64_);
     if (g32_) (void)hipGraphExecDestroy(g32_);
-    g64_ = g32_ = nullptr;
+    if (gstep_) (void)hipGraphExecDestroy(gstep_);
+    g64_ = g32_ = gstep_ = nullptr;
     g64_cap_ = g32_cap_ = -1;
+    graph_iters = graph_rounds = rounds_seen_ = 0;
+    graph_steps = graph_redone = 0;
+    graph_host_ms = 0.0;
+    if (gate_) (void)hipFree(gate_);
+    if (hgate_) (void)hipHostFree(hgate_);
+    gate_ = hgate_ = nullptr;
     E_.clear();  // engines release their buffers (the stream is not theirs)
     for (auto s : gst_) (void)hipStreamDestroy(s);
     gst_.clear();
@@ -83,7 +90,10 @@ int Batch::init(int n, const int* dims) {
         check(hipHostMalloc((void**)&hctl_, cb, hipHostMallocDefault), "hipHostMalloc") ||
         check(hipHostMalloc((void**)&hctl2_, cb, hipHostMallocDefault), "hipHostMalloc") ||
         check(hipMalloc((void**)&dj64_, (size_t)n * sizeof(BatchJob<double>)), "hipMalloc jobs") ||
-        check(hipMalloc((void**)&dj32_, (size_t)n * sizeof(BatchJob<float>)), "hipMalloc jobs"))
+        check(hipMalloc((void**)&dj32_, (size_t)n * sizeof(BatchJob<float>)), "hipMalloc jobs") ||
+        check(hipMalloc((void**)&gate_, (size_t)(n + 2) * sizeof(int)), "hipMalloc gate") ||
+        check(hipHostMalloc((void**)&hgate_, (size_t)(n + 2) * sizeof(int), hipHostMallocDefault),
+              "hipHostMalloc"))
         return -1;
     std::memset(hctl_, 0, cb);
     std::memset(hctl2_, 0, cb);
@@ -224,7 +234,8 @@ int Batch::build_jobs(const StepParams& P) {
     // captured graphs hold the old grid sizes
     if (g64_) (void)hipGraphExecDestroy(g64_);
     if (g32_) (void)hipGraphExecDestroy(g32_);
-    g64_ = g32_ = nullptr;
+    if (gstep_) (void)hipGraphExecDestroy(gstep_);
+    g64_ = g32_ = gstep_ = nullptr;
     g64_cap_ = g32_cap_ = -1;
     jobs_ok_ = true;
     jobs_c_ = c;
@@ -264,7 +275,8 @@ int Batch::sor_run(std::vector<BatchJob<T>>& hj, BatchJob<T>* dj, SorCtl* dctl, 
                                    st_), "ctl copy");
         black((z + 1) & 1);
     };
-    if (graph && (*gexec == nullptr || *gcap != cap)) {
+    const bool replay = graph == 1;
+    if (replay && (*gexec == nullptr || *gcap != cap)) {
         if (*gexec) (void)hipGraphExecDestroy(*gexec);
         *gexec = nullptr;
         // two iterations from an odd z: every replay starts at an odd
@@ -286,7 +298,7 @@ int Batch::sor_run(std::vector<BatchJob<T>>& hj, BatchJob<T>* dj, SorCtl* dctl, 
     red(0, 1, false);
     black(1);
     if (check(hipGetLastError(), "batched pass")) return -1;
-    const int per = graph ? 2 : 1;  // iterations per queued unit
+    const int per = replay ? 2 : 1;  // iterations per queued unit
     const int dq = depth > 0 ? depth : 1;
     while ((int)evq_.size() < dq) {
         hipEvent_t e;
@@ -298,7 +310,7 @@ int Batch::sor_run(std::vector<BatchJob<T>>& hj, BatchJob<T>* dj, SorCtl* dctl, 
     int queued = 0;
     for (;;) {
         while (queued < dq) {
-            if (graph) rc |= check(hipGraphLaunch(*gexec, st_), "graph launch");
+            if (replay) rc |= check(hipGraphLaunch(*gexec, st_), "graph launch");
             else iter(z);
             rc |= check(hipGetLastError(), "batched iteration");
             rc |= check(hipEventRecord(evq_[unit % dq], st_), "event");
@@ -411,6 +423,7 @@ int Batch::step(const StepParams& P) {
         return -1;
     }
     for (auto& e : E_) {
+        if (e->precision_ != precision || e->warm_ != warm) graph_iters = rounds_seen_ = 0;
         e->precision_ = precision;
         e->warm_ = warm;
     }
@@ -418,6 +431,168 @@ int Batch::step(const StepParams& P) {
     if (use_streams(0)) return -1;
     if (build_jobs(P)) return -1;
     const int cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
+    bool profiled = false;
+    for (auto& e : E_) profiled = profiled || e->prof.on;
+    if (graph == 2 && graph_iters > 0 && !profiled) {
+        const int r = step_graph(P, cap);
+        if (r <= 0) return r;
+        graph_redone++;  // a solve needed more than the graph held: the step again, eagerly
+    }
+    return step_eager(P, cap);
+}
+
+// Iterations per solve and correction rounds of the next captured step: the
+// last step's largest sweep count plus a margin (sweep counts change slowly
+// from step to step; an overrun costs one step redone eagerly, a spare
+// iteration four launches that return at once), and the most rounds any
+// step of this batch has needed (a spare round costs about as much as its
+// spare iterations; a second round, once needed, tends to recur).
+void Batch::plan_graph(int max_sweeps, int max_rounds_used) {
+    const int s = max_sweeps > 1 ? max_sweeps : 1;
+    graph_iters = s + std::max(4, s / 4);
+    rounds_seen_ = std::max(rounds_seen_, max_rounds_used);
+    graph_rounds = std::max(1, rounds_seen_);
+}
+
+// Iterations z = 1 .. nit of a solve after its first red and black pass,
+// enqueued without host copies, so that they can be captured: the stopping
+// test of each volume stays on the device and passes after it return at once.
+template <typename T>
+void Batch::enqueue_iters(BatchJob<T>* dj, SorCtl* dctl, int cap, int nit) {
+    const int n = size();
+    const dim3 grid((unsigned)(sizeof(T) == 4 ? grid32_ : grid_));
+    const dim3 rgrid((unsigned)(sizeof(T) == 4 ? rgrid32_ : rgrid_));
+    const int* vm = sizeof(T) == 4 ? vmap32_ : vmap_;
+    k_sor_batch<T, 0, 2><<<grid, kBlock, 0, st_>>>(dj, vm, dctl, 0, 0, 1);
+    k_sor_batch<T, 2, 2><<<grid, kBlock, 0, st_>>>(dj, vm, dctl, 1, 1, 0);
+    for (int z = 1; z <= nit; z++) {
+        k_sor_batch<T, 1, 2><<<grid, kBlock, 0, st_>>>(dj, vm, dctl, 0, z & 1, (z + 1) & 1);
+        k_resid_chunks_batch<T><<<rgrid, 1024, 0, st_>>>(dj, n);
+        k_reduce_batch<T><<<n, 1024, 0, st_>>>(dj, cap);
+        k_sor_batch<T, 2, 2><<<grid, kBlock, 0, st_>>>(dj, vm, dctl, 1, (z + 1) & 1, 0);
+    }
+}
+
+// The whole time step of every volume as one graph: the set-ups, the solve
+// (FP64: graph_iters iterations; FP32-refined: the first decision and
+// graph_rounds correction rounds of graph_iters FP32 iterations each), the
+// gate and the gated epilogue, and the copies of the control blocks to the
+// host.  The history buffers rotate every step, so the step is captured
+// every step and the instantiated graph updated.  Returns 0 (done), 1 (a
+// solve overran: nothing was written to u and the rotation is undone, so the
+// step can be redone) or -1 (error).
+int Batch::step_graph(const StepParams& P, int cap) {
+    const int n = size();
+    const int nit = std::min(graph_iters, cap);
+    const auto h0 = std::chrono::steady_clock::now();
+    if (check(hipStreamSynchronize(st_), "capture sync") ||
+        check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "capture step"))
+        return -1;
+    int rc = 0;
+    k_gate_begin<<<1, 64, 0, st_>>>(gate_, n);
+    for (int v = 0; v < n && !rc; v++)
+        if (E_[v]->setup(P, true, true, ctl_ + v, precision == 1)) {
+            err = E_[v]->err;
+            rc = -1;
+        }
+    if (!rc && precision == 0) {
+        enqueue_iters<double>(dj64_, ctl_, cap, nit);
+    } else if (!rc) {
+        const dim3 grid((unsigned)grid_), cgrid((unsigned)cgrid_);
+        const unsigned vb = (unsigned)((n + 63) / 64);
+        k_decide_batch<<<vb, 64, 0, st_>>>(dj64_, n);
+        for (int r = 0; r < graph_rounds; r++) {
+            k_zero_batch<<<cgrid, kBlock, 0, st_>>>(dj64_, dj32_, n);
+            k_inner_ctl_batch<<<vb, 64, 0, st_>>>(dj64_, dj32_, n, inner_frac, inner_floor2);
+            enqueue_iters<float>(dj32_, ctl2_, cap, nit);
+            k_gate_inner<<<vb, 64, 0, st_>>>(dj64_, dj32_, n, gate_);
+            k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, vmap_, ctl_, 0);
+            k_defect_corr_batch<<<grid, kBlock, 0, st_>>>(dj64_, dj32_, vmap_, ctl_, 1);
+            k_resid_chunks_batch<double><<<(unsigned)rgrid_, 1024, 0, st_>>>(dj64_, n);
+            k_defect_decide<<<n, 1024, 0, st_>>>(dj64_, max_rounds);
+        }
+    }
+    if (!rc) {
+        k_gate_done<<<1, 256, 0, st_>>>(dj64_, n, gate_);
+        for (int v = 0; v < n && !rc; v++)
+            if (E_[v]->unpack_gated(E_[v]->full(A_MASK), precision == 0 ? ctl_ + v : nullptr,
+                                    gate_)) {
+                err = E_[v]->err;
+                rc = -1;
+            }
+    }
+    if (!rc)
+        rc = check(hipMemcpyAsync(hctl_, ctl_, (size_t)n * sizeof(SorCtl), hipMemcpyDeviceToHost,
+                                  st_), "ctl copy") |
+             check(hipMemcpyAsync(hgate_, gate_, (size_t)(n + 2) * sizeof(int),
+                                  hipMemcpyDeviceToHost, st_), "gate copy") |
+             check(hipGetLastError(), "captured step");
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st_, &g);
+    if (rc || check(ec, "end capture")) {
+        if (g) (void)hipGraphDestroy(g);
+        for (auto& e : E_) e->undo_rotation(true);  // nothing ran
+        return -1;
+    }
+    if (gstep_) {
+        hipGraphNode_t en = nullptr;
+        hipGraphExecUpdateResult res;
+        if (hipGraphExecUpdate(gstep_, g, &en, &res) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(gstep_);
+            gstep_ = nullptr;
+        }
+    }
+    if (!gstep_) {
+        const hipError_t ei = hipGraphInstantiateWithFlags(&gstep_, g, 0);
+        if (ei != hipSuccess) gstep_ = nullptr;
+        rc = check(ei, "step graph instantiate");
+    }
+    (void)hipGraphDestroy(g);
+    graph_host_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc) rc = check(hipEventCreate(&e0), "event") | check(hipEventCreate(&e1), "event");
+    if (!rc) rc = check(hipEventRecord(e0, st_), "event");
+    if (!rc) rc = check(hipGraphLaunch(gstep_, st_), "step graph launch");
+    if (!rc) rc = check(hipEventRecord(e1, st_), "event");
+    if (!rc) rc = check(hipEventSynchronize(e1), "step graph wait");
+    float ms = 0.f;
+    if (!rc) rc = check(hipEventElapsedTime(&ms, e0, e1), "event");
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) {
+        for (auto& e : E_) e->undo_rotation();  // the epilogue may have run
+        return -1;
+    }
+    if (hgate_[1] == 0) {
+        for (auto& e : E_) e->undo_rotation(true);  // the gated epilogue wrote nothing
+        return 1;
+    }
+    last_step_ms = ms;
+    last_iterations = 0;
+    int ms_max = 0, mr_max = 0;
+    for (int v = 0; v < n; v++) {
+        const int z = precision == 0 ? hctl_[v].zfinal : hgate_[2 + v];
+        stats[v].z = z;
+        stats[v].R = hctl_[v].R;
+        stats[v].tol = hctl_[v].tol;
+        stats[v].sumsq = hctl_[v].sumsq;
+        stats[v].rounds = precision == 0 ? 0 : hctl_[v].z;
+        E_[v]->red_final_ = precision == 0 ? (hctl_[v].zfinal & 1) : 0;
+        last_iterations += z;
+        ms_max = std::max(ms_max, z);
+        mr_max = std::max(mr_max, stats[v].rounds);
+    }
+    for (auto& e : E_) e->step_done();
+    graph_steps++;
+    plan_graph(ms_max, mr_max);
+    return 0;
+}
+
+int Batch::step_eager(const StepParams& P, int cap) {
+    const int n = size();
+    int max_round_sweeps = 0, max_rounds_used = 0;
     hipEvent_t e0, e1;
     if (check(hipEventCreate(&e0), "event")) return -1;
     if (check(hipEventCreate(&e1), "event")) {
@@ -441,6 +616,7 @@ int Batch::step(const StepParams& P) {
             stats[v].rounds = 0;
             E_[v]->red_final_ = hctl_[v].zfinal & 1;
             last_iterations += hctl_[v].zfinal;
+            max_round_sweeps = std::max(max_round_sweeps, hctl_[v].zfinal);
         }
     } else if (!rc) {
         const dim3 grid((unsigned)grid_), cgrid((unsigned)cgrid_);
@@ -478,7 +654,10 @@ int Batch::step(const StepParams& P) {
             if (!rc) rc = sor_run<float>(j32_, dj32_, ctl2_, hctl2_, cap, &g32_, &g32_cap_);
             if (rc) break;
             for (int v = 0; v < n; v++)
-                if (active[v]) sweeps[v] += hctl2_[v].zfinal;
+                if (active[v]) {
+                    sweeps[v] += hctl2_[v].zfinal;
+                    max_round_sweeps = std::max(max_round_sweeps, hctl2_[v].zfinal);
+                }
             rc = defect();
         }
         for (int v = 0; v < n && !rc; v++) {
@@ -489,6 +668,7 @@ int Batch::step(const StepParams& P) {
             stats[v].rounds = hctl_[v].z;
             E_[v]->red_final_ = 0;
             last_iterations += sweeps[v];
+            max_rounds_used = std::max(max_rounds_used, hctl_[v].z);
         }
     }
     for (int v = 0; v < n && !rc; v++)
@@ -508,6 +688,7 @@ int Batch::step(const StepParams& P) {
     }
     last_step_ms = ms;
     for (auto& e : E_) e->step_done();
+    if (graph == 2) plan_graph(max_round_sweeps, max_rounds_used);
     return 0;
 }
 

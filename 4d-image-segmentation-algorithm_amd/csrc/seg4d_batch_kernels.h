@@ -214,4 +214,45 @@ static __global__ __launch_bounds__(kBlock) void k_zero_batch(const BatchJob<dou
     }
 }
 
+// ---- the captured time step (graph mode 2) ---------------------------------
+// A step captured as one graph holds a fixed number of iterations per solve
+// and of correction rounds.  gate[0] is set when a volume's solve needs more
+// than the graph holds; gate[1] = 1 when every volume finished inside it, and
+// only then does the epilogue (k_unpack_gated) write u.  gate[2 + v]
+// accumulates volume v's FP32 sweeps over the captured rounds.
+
+static __global__ void k_gate_begin(int* gate, int nvol) {
+    for (int q = (int)threadIdx.x; q < nvol + 2; q += (int)blockDim.x) gate[q] = 0;
+}
+
+// After the captured FP32 iterations of a round: a running volume whose
+// correction solve has not stopped overran.  It is marked stopped, so that
+// the rest of the graph skips it (its defect, later rounds), and the overrun
+// is flagged; the step is then redone eagerly.  Volumes that finished add
+// their sweeps (0 for volumes that stopped before the round).
+static __global__ void k_gate_inner(const BatchJob<double>* __restrict__ J64,
+                                    const BatchJob<float>* __restrict__ J32, int nvol, int* gate) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvol) return;
+    SorCtl* outer = J64[v].ctl;
+    const SorCtl* inner = J32[v].ctl;
+    if (!outer->stop && !inner->stop) {
+        outer->stop = 1;
+        gate[0] = 1;
+    } else {
+        gate[2 + v] += inner->zfinal;
+    }
+}
+
+// gate[1] = 1 iff no volume overran and every volume's solve stopped.
+static __global__ void k_gate_done(const BatchJob<double>* __restrict__ J64, int nvol, int* gate) {
+    __shared__ int any_running;
+    if (threadIdx.x == 0) any_running = 0;
+    __syncthreads();
+    for (int v = (int)threadIdx.x; v < nvol; v += (int)blockDim.x)
+        if (!J64[v].ctl->stop) any_running = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) gate[1] = (gate[0] == 0 && any_running == 0) ? 1 : 0;
+}
+
 }  // namespace s4d

@@ -160,8 +160,14 @@ public:
     // Completed time step: the history levels the setup filled become valid.
     void step_done();
     // puts the buffer pointers back as they were before setup() rotated
-    // them (a step that failed after its set-up leaves u = u^n)
-    void undo_rotation();
+    // them (a step that failed after its set-up leaves u = u^n).
+    // history_intact: nothing wrote the recycled buffer (no k_unpack ran), so
+    // the history levels stay valid; otherwise they are dropped.
+    void undo_rotation(bool history_intact = false);
+    // k_unpack behind a device gate (gate[1] != 0) with the red buffer chosen
+    // on the device from sel->zfinal's parity (sel null: the first buffer):
+    // the epilogue of a captured step, whose solve ends on the device.
+    int unpack_gated(const double* dmask, const SorCtl* sel, const int* gate);
     struct Geo geometry() const;
     // Times the red pass (FP64, no residual) over tile shapes of the tiled
     // order and the linear order, `reps` launches each, and keeps the

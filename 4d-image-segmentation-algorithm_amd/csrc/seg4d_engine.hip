@@ -223,13 +223,14 @@ int Engine::adopt_stream(hipStream_t s) {
     return 0;
 }
 
-void Engine::undo_rotation() {
+void Engine::undo_rotation(bool history_intact) {
     if (!rotated_) return;
     full_[A_U] = rot_[0];
     full_[A_PREV] = rot_[1];
     prev2_ = rot_[2];
     prev3_ = rot_[3];
     rotated_ = false;
+    if (history_intact) return;
     // The buffer that became u's may already hold (part of) u^(n+1): k_unpack
     // writes it at the end of the step, and a later failure (the host wait, a
     // batch's later volume) cannot tell whether it ran.  That buffer is a
@@ -972,6 +973,13 @@ int Engine::unpack(const double* dmask) {
     k_unpack<<<(unsigned)cdiv(n_, kBlock), kBlock, 0, st_>>>(G, ur_[red_final_], ub_, full_[A_U],
                                                             dmask);
     return launch_check("k_unpack");
+}
+
+int Engine::unpack_gated(const double* dmask, const SorCtl* sel, const int* gate) {
+    const Geo G = make_geo(I_, J_, K_, L_, H_);
+    k_unpack_gated<<<(unsigned)cdiv(n_, kBlock), kBlock, 0, st_>>>(G, ur_[0], ur_[1], ub_,
+                                                                  full_[A_U], dmask, sel, gate);
+    return launch_check("k_unpack_gated");
 }
 
 double* Engine::scratch(ArrayId a) {

@@ -947,14 +947,10 @@ __host__ __device__ inline int l2_blocks(long n) {
 // Every padded point: the ghosts come from the colour arrays too (the set-up
 // wrote u^n's ghosts there, the MCF step its boundary values), because u's
 // buffer is a recycled history level; the mask applies to the interior only.
-static __global__ __launch_bounds__(kBlock) void k_unpack(Geo G, const double* __restrict__ ur,
-                                                  const double* __restrict__ ub,
-                                                  double* __restrict__ u,
-                                                  const double* __restrict__ mask) {
-    // one thread per padded point in array order (a 1-D grid of
-    // cdiv(I J K L, kBlock) blocks), so that a wavefront's stores of u cover
-    // whole lines whatever the row length
-    const long q = (long)blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void unpack_point(const Geo& G, const double* __restrict__ ur,
+                                             const double* __restrict__ ub,
+                                             double* __restrict__ u,
+                                             const double* __restrict__ mask, long q) {
     if (q >= (long)G.I * G.J * G.K * G.L) return;
     const int p = (int)q;
     int r = p / G.I;
@@ -966,6 +962,30 @@ static __global__ __launch_bounds__(kBlock) void k_unpack(Geo G, const double* _
     double v = ((i + j + k + l) & 1) ? ub[pp] : ur[pp];
     if (mask != nullptr && mask[p] == 0 && is_interior(G, i, j, k, l)) v = 0;
     u[p] = v;
+}
+
+static __global__ __launch_bounds__(kBlock) void k_unpack(Geo G, const double* __restrict__ ur,
+                                                  const double* __restrict__ ub,
+                                                  double* __restrict__ u,
+                                                  const double* __restrict__ mask) {
+    // one thread per padded point in array order (a 1-D grid of
+    // cdiv(I J K L, kBlock) blocks), so that a wavefront's stores of u cover
+    // whole lines whatever the row length
+    unpack_point(G, ur, ub, u, mask, (long)blockIdx.x * kBlock + threadIdx.x);
+}
+
+// k_unpack behind a device gate: nothing is written unless gate[1] is set
+// (the epilogue of a step captured as one graph, whose solve ends on the
+// device).  sel: the FP64 solve's control block, whose final iteration's
+// parity names the red buffer holding the accepted values (null: ur0, as
+// after a correction).
+static __global__ __launch_bounds__(kBlock) void k_unpack_gated(
+    Geo G, const double* __restrict__ ur0, const double* __restrict__ ur1,
+    const double* __restrict__ ub, double* __restrict__ u, const double* __restrict__ mask,
+    const SorCtl* __restrict__ sel, const int* __restrict__ gate) {
+    if (gate[1] == 0) return;
+    const double* ur = (sel != nullptr && (sel->zfinal & 1)) ? ur1 : ur0;
+    unpack_point(G, ur, ub, u, mask, (long)blockIdx.x * kBlock + threadIdx.x);
 }
 
 // ----------------------------------------------------------------------------

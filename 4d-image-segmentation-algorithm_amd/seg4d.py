@@ -223,6 +223,12 @@ class SegLib(_Lib):
         L.seg4dBatchAutotune.argtypes = [c_int, POINTER(ctypes.c_float)]
         L.seg4dBatchTile.restype = c_int
         L.seg4dBatchTile.argtypes = [c_int, POINTER(c_int), POINTER(c_int)]
+        L.seg4dBatchSetGraphPlan.restype = c_int
+        L.seg4dBatchSetGraphPlan.argtypes = [c_int, c_int]
+        L.seg4dBatchGraphInfo.restype = c_int
+        L.seg4dBatchGraphInfo.argtypes = [POINTER(c_int), POINTER(c_int),
+                                          POINTER(ctypes.c_long), POINTER(ctypes.c_long),
+                                          POINTER(c_double)]
         L.seg4dInjectStepFailure.restype = c_int
         L.seg4dInjectStepFailure.argtypes = [c_int]
         L.seg4d_gen_preset.restype = c_int
@@ -532,6 +538,20 @@ class SegLib(_Lib):
                                     ctypes.byref(tol)) != 0:
             raise RuntimeError(f"seg4dBatchStats failed: {self.last_error()}")
         return z.value, r.value, R.value, tol.value
+
+    def batch_graph_info(self):
+        """Graph mode 2: (iterations per solve and rounds the next captured step
+        holds, steps run as one graph, steps redone eagerly, host ms spent
+        capturing and instantiating)."""
+        it, rd, st, re, hm = c_int(), c_int(), ctypes.c_long(), ctypes.c_long(), c_double()
+        if self.lib.seg4dBatchGraphInfo(ctypes.byref(it), ctypes.byref(rd), ctypes.byref(st),
+                                        ctypes.byref(re), ctypes.byref(hm)) != 0:
+            raise RuntimeError(f"seg4dBatchGraphInfo failed: {self.last_error()}")
+        return it.value, rd.value, st.value, re.value, hm.value
+
+    def batch_set_graph_plan(self, iters, rounds):
+        if self.lib.seg4dBatchSetGraphPlan(int(iters), int(rounds)) != 0:
+            raise RuntimeError(f"seg4dBatchSetGraphPlan failed: {self.last_error()}")
 
     def batch_autotune(self, reps=3):
         n = len(self._batch_dims)

@@ -497,7 +497,9 @@ def run_batch_bench(args, lib):
     1e-6, launch tiles from the autotuner.  Times the batch in FP64 (one
     launch per pass with eager and graph-replayed iterations, and the
     concurrent form with `--batch-streams` volume groups on their own
-    streams) and FP32-refined (start u^n and cubic start), and the same FP64
+    streams, and every step after the first as one captured graph) and
+    FP32-refined (start u^n and cubic start, with graph-replayed iterations
+    and with captured steps), and the same FP64
     steps one volume at a time on the single-volume path; `value` is the
     fastest FP64 batch form.  Prints one JSON line."""
     nv, steps = args.batch_volumes, args.batch_steps
@@ -532,6 +534,7 @@ def run_batch_bench(args, lib):
                 sweeps.append([lib.batch_stats(v)[0] for v in range(nv)])
             t1 = time.perf_counter()
             us = [lib.batch_u(v)[1:-1, 1:-1, 1:-1, 1:-1].copy() for v in range(nv)]
+            git, grd, gst, gre, ghm = lib.batch_graph_info()
         finally:
             lib.batch_destroy()
         total = t1 - t0
@@ -541,7 +544,11 @@ def run_batch_bench(args, lib):
             tl[str(t_)] = tl.get(str(t_), 0) + 1
         return {"time_s": total, "device_ms": dev_ms, "iterations": int(np.sum(sweeps)),
                 "voxel_updates_per_s": upd / total, "edge_indicator_s_pcie_inclusive": edge_s,
-                "tiles": tl}, us
+                "tiles": tl, "max_sweeps_per_step": [int(max(sw)) for sw in sweeps],
+                "graph": ({"captured_steps": gst, "redone_eagerly": gre,
+                           "last_plan_iterations": git, "last_plan_rounds": grd,
+                           "host_capture_ms": ghm}
+                          if graph == 2 else None)}, us
 
     log("[batch] FP64, eager iterations ...")
     fp64, u64 = batch_run(0, 0, 0)
@@ -549,6 +556,12 @@ def run_batch_bench(args, lib):
     log("[batch] FP64, graph-replayed iterations ...")
     fp64g, _ = batch_run(0, 0, 1)
     log(f"[batch] FP64 graph: {fp64g['time_s']:.3f} s")
+    log("[batch] FP64, every step after the first as one captured graph ...")
+    fp64s, us_ = batch_run(0, 0, 2)
+    fp64s["bitwise_equal_to_eager_form"] = all(
+        np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(u64, us_))
+    log(f"[batch] FP64 step graph: {fp64s['time_s']:.3f} s, {fp64s['graph']}, "
+        f"bitwise equal {fp64s['bitwise_equal_to_eager_form']}")
     conc = {}
     for k in [int(x) for x in args.batch_streams.split(",") if x.strip()]:
         log(f"[batch] FP64, concurrent form, {k} groups ...")
@@ -559,11 +572,14 @@ def run_batch_bench(args, lib):
         conc[f"fp64_batch_streams_{k}"] = r
         log(f"[batch] FP64 {k} groups: {r['time_s']:.3f} s, "
             f"bitwise equal {r['bitwise_equal_to_one_launch_form']}")
-    forms = {"fp64_batch": fp64, "fp64_batch_graph": fp64g, **conc}
+    forms = {"fp64_batch": fp64, "fp64_batch_graph": fp64g, "fp64_batch_step_graph": fp64s,
+             **conc}
     best = min(forms, key=lambda f: forms[f]["time_s"])
     ref = {}
     for name, (prec, warm, graph) in (("fp32_refined", (1, 0, 1)),
-                                      ("fp32_refined_cubic_start", (1, 4, 1))):
+                                      ("fp32_refined_cubic_start", (1, 4, 1)),
+                                      ("fp32_refined_step_graph", (1, 0, 2)),
+                                      ("fp32_refined_cubic_start_step_graph", (1, 4, 2))):
         log(f"[batch] {name} ...")
         r, um = batch_run(prec, warm, graph)
         ious, dus = [], []
@@ -628,9 +644,12 @@ def run_batch_bench(args, lib):
                    "form": best},
         "fp64_batch": fp64,
         "fp64_batch_graph": fp64g,
+        "fp64_batch_step_graph": fp64s,
         **conc,
         "fp32_refined_batch": ref["fp32_refined"],
         "fp32_refined_cubic_start_batch": ref["fp32_refined_cubic_start"],
+        "fp32_refined_step_graph_batch": ref["fp32_refined_step_graph"],
+        "fp32_refined_cubic_start_step_graph_batch": ref["fp32_refined_cubic_start_step_graph"],
         "fp64_sequential": seq,
     }
     print(json.dumps(out))

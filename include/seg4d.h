@@ -355,9 +355,23 @@ double seg4dBatchLastStepMs(void);   /* device time of the last step */
 /* precision 0 (FP64) / 1 (FP32-refined, correction solves at the factor of
  * seg4dSetInnerOmega's default 1.2); warm start 0, 1, 3 or 4 as
  * seg4dSetWarmStart; graph 1 replays the solve iterations from a captured
- * HIP graph (two iterations per replay); depth 1..16 iterations (replays)
- * queued ahead of the stopping test.  Defaults 0, 0, 0, 2. */
+ * HIP graph (two iterations per replay); graph 2 runs every time step after
+ * the first as one captured HIP graph (set-ups, solve, correction rounds,
+ * epilogue; every decision on the device), holding as many iterations per
+ * solve and rounds as the previous step needed plus a margin; a step whose
+ * solve needs more writes nothing and is redone eagerly; depth 1..16
+ * iterations (replays) queued ahead of the stopping test of eager solves.
+ * Defaults 0, 0, 0, 2. */
 int seg4dBatchConfigure(int precision, int warm, int graph, int depth);
+/* Graph mode 2: iterations per solve and rounds the next captured step
+ * holds, steps run as one graph and steps redone eagerly so far, and the
+ * host time spent capturing and instantiating them (any output may be
+ * NULL). */
+int seg4dBatchGraphInfo(int *iters, int *rounds, long *steps, long *redone, double *host_ms);
+/* Graph mode 2: the iterations per solve and rounds the next captured step
+ * holds, in place of the plan from the last step (both 0: run the next step
+ * eagerly).  Later steps plan from their predecessor again. */
+int seg4dBatchSetGraphPlan(int iters, int rounds);
 /* Concurrent form: nstreams >= 1 splits the volumes into that many groups of
  * about equal voxel count (at most one per volume), each on its own stream
  * driven by its own host thread, and steps every volume on the

@@ -5,7 +5,8 @@ library (whose FP64 path is bitwise the oracle's, checked directly for the
 smallest volume below), in FP64 and FP32-refined arithmetic, with eager or
 graph-replayed iterations, any queue depth and autotuned launch tiles, and
 in the concurrent form (volume groups on their own streams and host threads,
-switched to and from within one batch)."""
+switched to and from within one batch), and with each time step captured as
+one graph."""
 import numpy as np
 import pytest
 
@@ -72,9 +73,12 @@ def volumes(seglib):
     return [_volume(seglib, d, 11 + q) for q, d in enumerate(DIMS)]
 
 
-def _run_batch(lib, volumes, precision, warm, graph, depth, tune, steps, streams=(0,)):
+def _run_batch(lib, volumes, precision, warm, graph, depth, tune, steps, streams=(0,),
+               plans=None, info=None):
     """streams[s]: the concurrent form's group count at step s (the last entry
-    for later steps; 0 is one launch per pass)."""
+    for later steps; 0 is one launch per pass).  plans[s]: (iterations,
+    rounds) forced on the captured step s (graph mode 2); info (a list)
+    receives seg4dBatchGraphInfo after the last step."""
     lib.set(gsTol=0.0, **KW)
     lib.set_relative_tolerance(REL)
     lib.batch_create(DIMS)
@@ -89,8 +93,12 @@ def _run_batch(lib, volumes, precision, warm, graph, depth, tune, steps, streams
         for s in range(steps):
             lib.set(numberOfTimeStep=s + 1)
             lib.batch_streams(streams[min(s, len(streams) - 1)])
+            if plans and s in plans:
+                lib.batch_set_graph_plan(*plans[s])
             lib.batch_step()
             res.append([(lib.batch_u(v),) + lib.batch_stats(v)[:2] for v in range(len(DIMS))])
+        if info is not None:
+            info.extend(lib.batch_graph_info())
     finally:
         lib.batch_destroy()
         lib.set_relative_tolerance(0.0)
@@ -122,6 +130,36 @@ def test_batch_matches_single_volume(seglib, volumes, precision, warm, graph, de
                 f"volume {v} step {s + 1}: u differs (max {np.abs(ub - us).max():.3e})"
     print(f"  {len(DIMS)} volumes x {steps} steps bitwise equal; sweeps "
           f"{[got[-1][v][1] for v in range(len(DIMS))]}", flush=True)
+
+
+@pytest.mark.parametrize("precision,warm,depth", [(0, 0, 2), (0, 3, 1), (1, 0, 2), (1, 4, 3)])
+def test_batch_captured_step(seglib, volumes, precision, warm, depth):
+    """Graph mode 2: every step after the first runs as one captured graph
+    (set-ups, solve or correction rounds, gated epilogue), bitwise the
+    single-volume steps.  Step 2 is forced to hold one iteration per solve:
+    its solves overrun, the graph writes nothing and the step is redone
+    eagerly with the same result; the later steps plan from it again."""
+    print(f"[gpu] captured step precision {precision} warm {warm} depth {depth}", flush=True)
+    lib = seglib
+    steps = STEPS + 2
+    info = []
+    got = _run_batch(lib, volumes, precision, warm, 2, depth, False, steps, plans={1: (1, 1)},
+                     info=info)
+    iters, rounds, gsteps, redone, _ = info
+    assert redone == 1, f"forced overrun: {redone} steps redone"
+    assert gsteps == steps - 2, f"{gsteps} steps ran as one graph, expected {steps - 2}"
+    assert iters >= 5 and rounds >= 1
+    for v, (img, u0) in enumerate(volumes):
+        ref = _single(lib, DIMS[v], img, u0, precision, warm, steps)
+        for s in range(steps):
+            ub, zb, rb = got[s][v]
+            us, zs, rs = ref[s]
+            assert zb == zs, f"volume {v} step {s + 1}: sweeps {zb} vs {zs}"
+            assert rb == rs, f"volume {v} step {s + 1}: rounds {rb} vs {rs}"
+            assert np.array_equal(ub.view(np.uint64), us.view(np.uint64)), \
+                f"volume {v} step {s + 1}: u differs (max {np.abs(ub - us).max():.3e})"
+    print(f"  {len(DIMS)} volumes x {steps} steps bitwise equal; {gsteps} captured steps, "
+          f"plan {iters} iterations x {rounds} rounds", flush=True)
 
 
 def test_batch_fp64_vs_oracle(seglib, oracle, volumes):
