@@ -786,6 +786,15 @@ API int seg4dSetPipelineDepth(int depth) {
     return old;
 }
 
+API int seg4dSetFusedSweeps(int mode) {
+    if (!g_eng) return -2;
+    const int old = g_eng->fused_;
+    g_eng->fused_ = mode > 0 ? 1 : 0;
+    return old;
+}
+
+API int seg4dLastSolveFused(void) { return g_eng && g_eng->fused_last_ ? 1 : 0; }
+
 API int seg4dSetSweepTile(int runs, int planes) {
     if (!g_eng) return -1;
     if (runs < 1 || planes < 1) return -1;

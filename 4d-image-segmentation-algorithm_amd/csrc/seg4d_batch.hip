@@ -78,6 +78,10 @@ int Batch::init(int n, const int* dims) {
         const int* d = dims + 4 * v;
         E_.emplace_back(new Engine());
         Engine& E = *E_.back();
+        // no fused (cooperative) solves: the concurrent form runs several
+        // volumes' solves at once, and a cooperative grid assumes the GPU's
+        // every CU for itself
+        E.fused_ = 0;
         if (E.init(d[0] + 2, d[1] + 2, d[2] + 2, d[3] + 2, true, false) || E.adopt_stream(st_)) {
             err = "batch volume " + std::to_string(v) + ": " + E.err;
             set_error(err);

@@ -15,6 +15,7 @@ namespace s4d {
 
 struct Geo;
 struct SorCtl;
+struct GridBar;
 struct SweepMap;
 struct BallList;
 class Net;
@@ -191,6 +192,17 @@ public:
     int sweep_nband_ = 0;  // bands of the automatic linear order; 0 = as the L2 window needs
     int pipe_depth_ = 0;   // iterations enqueued ahead of the stopping test; 0 = automatic
     std::vector<hipEvent_t> evq_;  // one event per queued iteration
+    // Fused sweeps: a whole SOR solve in one cooperative launch (k_sor_fused,
+    // grid barriers between the passes).  0 = off (one launch per pass),
+    // 1 = whenever the grid fits on the GPU.  Identical results either way.
+    // Off by default: measured slower at every size (a grid barrier with its
+    // agent-scope release and acquire costs ~100 us, an iteration of the
+    // pass-per-launch solve of 64x64x32x8 36 us).
+    int fused_ = 0;
+    bool fused_last_ = false;      // the last SOR solve ran fused
+    GridBar* gbar_ = nullptr;      // the fused launch's barrier state
+    GridBar* hgbar_ = nullptr;     // pinned mirror (abort flag)
+    int fused_blocks_ = 0;         // co-resident blocks of k_sor_fused (0: not yet known)
     // 0 = FP64 (the reference's arithmetic); 1 = FP32-refined: FP32 SOR on
     // the correction equation inside FP64 defect correction.
     int precision_ = 0;

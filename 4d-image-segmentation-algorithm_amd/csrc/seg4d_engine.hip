@@ -192,6 +192,10 @@ void Engine::release() {
     balls_valid_ = false;
     if (ctl2_) hipFree(ctl2_);
     if (hctl2_) hipHostFree(hctl2_);
+    if (gbar_) hipFree(gbar_);
+    if (hgbar_) hipHostFree(hgbar_);
+    gbar_ = nullptr;
+    hgbar_ = nullptr;
     db_ = nullptr;
     ctl2_ = hctl2_ = nullptr;
     ctl_ = hctl_ = nullptr;
@@ -635,6 +639,78 @@ struct SorJob {
     int kind0;          // profiling kinds kind0 .. kind0+2
 };
 
+// The whole solve as one cooperative launch of k_sor_fused: as many blocks
+// as the GPU holds at once (a multiple of 8, at most the pass's blocks).
+// Returns 1 when the launch does not apply (the caller then runs the solve
+// pass by pass, with identical results), 0 on success, -1 on error.
+template <typename T>
+int sor_run_fused(Engine& E, const SorJob<T>& J, const Geo& G, const SweepMap& M, int nb,
+                  int rnc, int rch, int* zout) {
+    // the fused kernel is built for the default pass variant (two l-steps per
+    // load phase, plain operand loads)
+    if ((E.sweep_variant_ & 5) != 1 || E.prof.on) return 1;
+    auto kern = k_sor_fused<T, 2>;
+    if (E.fused_blocks_ == 0) {
+        int dev = 0, ncu = 0, per = 0;
+        if (E.check(hipGetDevice(&dev), "hipGetDevice") ||
+            E.check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev),
+                    "CU count") ||
+            E.check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)kern, kBlock,
+                                                                   0), "occupancy"))
+            return -1;
+        E.fused_blocks_ = per * ncu > 0 ? per * ncu : -1;
+    }
+    // a multiple of 8 blocks keeps every work item on its own launch's XCD;
+    // grids of fewer than 8 items run as they are
+    const int items = sweep_grid(M);
+    int nblk = std::min(E.fused_blocks_, items);
+    if (nblk >= 8) nblk = nblk / 8 * 8;
+    if (!E.gbar_) {
+        if (E.check(hipMalloc((void**)&E.gbar_, sizeof(GridBar)), "hipMalloc barrier") ||
+            E.check(hipHostMalloc((void**)&E.hgbar_, sizeof(GridBar), hipHostMallocDefault),
+                    "hipHostMalloc barrier"))
+            return -1;
+    }
+    k_bar_reset<<<1, 64, 0, E.st_>>>(E.gbar_);
+    if (E.launch_check("k_bar_reset")) return -1;
+    Geo g = G;
+    SweepMap m = M;
+    T c = J.c, om = J.om;
+    double h4 = J.h4;
+    T* red0 = J.red[0];
+    T* red1 = J.red[1];
+    T* black = J.black;
+    const T* op0 = J.opnd[0];
+    const T* op1 = J.opnd[1];
+    double* part_r = E.part_;
+    double* part_b = E.part_ + E.npart_;
+    double* red2 = E.red2_;
+    SorCtl* ctl = J.ctl;
+    int cap = J.cap;
+    GridBar* gb = E.gbar_;
+    void* args[] = {&g, &m, &c, &om, &h4, &red0, &red1, &black, &op0, &op1, &part_r, &part_b,
+                    &nb, &rnc, &rch, &red2, &ctl, &cap, &gb};
+    const hipError_t el = hipLaunchCooperativeKernel((const void*)kern, dim3((unsigned)nblk),
+                                                     dim3(kBlock), args, 0, E.st_);
+    if (el == hipErrorCooperativeLaunchTooLarge) {
+        (void)hipGetLastError();
+        E.fused_blocks_ = -1;  // never fits on this device: pass by pass from now on
+        return 1;
+    }
+    if (E.check(el, "k_sor_fused") ||
+        E.check(hipMemcpyAsync(J.hctl, J.ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, E.st_),
+                "ctl copy") ||
+        E.check(hipMemcpyAsync(E.hgbar_, E.gbar_, sizeof(GridBar), hipMemcpyDeviceToHost,
+                               E.st_), "barrier copy") ||
+        E.check(hipStreamSynchronize(E.st_), "fused solve"))
+        return -1;
+    if (E.hgbar_->abort)
+        return E.check(hipErrorLaunchTimeOut, "fused solve: a grid barrier timed out");
+    E.fused_last_ = true;
+    *zout = J.hctl->zfinal;
+    return 0;
+}
+
 template <typename T>
 int sor_run(Engine& E, const SorJob<T>& J, int* zout) {
     const Geo G = make_geo(E.I_, E.J_, E.K_, E.L_, E.H_);
@@ -646,6 +722,11 @@ int sor_run(Engine& E, const SorJob<T>& J, int* zout) {
     }
     int rnc = 0, rch = 0;
     resid_chunking(nb, &rnc, &rch);
+    E.fused_last_ = false;
+    if (E.fused_ == 1 && E.fused_blocks_ >= 0) {
+        const int r = sor_run_fused<T>(E, J, G, M, nb, rnc, rch, zout);
+        if (r <= 0) return r;
+    }
     const dim3 grid((unsigned)sweep_grid(M));
     double* part_r = E.part_;
     double* part_b = E.part_ + E.npart_;

@@ -853,6 +853,197 @@ static __global__ __launch_bounds__(1024) void k_reduce_resid(
     }
 }
 
+// ----------------------------------------------------------------------------
+// Fused sweeps: every iteration of one red-black SOR solve in a single
+// cooperative launch (all blocks co-resident), the passes separated by
+// grid-wide barriers.  Opt-in: each barrier makes every block release and
+// acquire its writes at agent scope (the L2 of each XCD written back and
+// invalidated), which on an MI355X costs more than the launch gaps between
+// single-pass launches it removes.
+// ----------------------------------------------------------------------------
+struct GridBar {
+    // each field on a 256-byte line of its own: arrivals, polling and the
+    // release do not contend for one line
+    unsigned xcnt[8][64];  // blocks of XCD group x (blockIdx % 8) arrived
+    unsigned count;        // groups whose last block arrived
+    unsigned pad0[63];
+    unsigned gen;          // barriers completed
+    unsigned pad1[63];
+    int abort;             // a block waited beyond kBarrierSpins: every block leaves
+    int pad2[63];
+};
+
+// about a second of polling: far beyond any barrier of a healthy solve
+constexpr long kBarrierSpins = 1L << 20;
+
+static __global__ void k_bar_reset(GridBar* gb) {
+    unsigned* w = reinterpret_cast<unsigned*>(gb);
+    for (unsigned q = threadIdx.x; q < sizeof(GridBar) / sizeof(unsigned); q += blockDim.x)
+        w[q] = 0u;
+}
+
+// Grid-wide barrier for a cooperative launch of gridDim.x blocks, in two
+// levels: thread 0 of each block releases the block's writes at agent scope
+// and arrives on its group's counter (group = blockIdx % 8, the XCD the block
+// runs on); the last block of a group arrives on the group counter, and the
+// last group advances the generation every block waits for.  The wait is
+// bounded: a timed-out wait raises the abort flag, which every waiting block
+// sees, so that all blocks leave the kernel instead of hanging.  Returns
+// false after an abort.  `gen` is the block's count of barriers passed.
+__device__ __forceinline__ bool grid_barrier(GridBar* gb, unsigned& gen) {
+    __shared__ int ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int good = 1;
+        const unsigned ng = gridDim.x < 8 ? gridDim.x : 8u;
+        const unsigned x = blockIdx.x % 8;
+        const unsigned gsize = (gridDim.x - x + 7) / 8;  // blocks of group x
+        __threadfence();
+        bool last = false;
+        const unsigned a =
+            __hip_atomic_fetch_add(&gb->xcnt[x][0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (a == gsize - 1) {
+            __hip_atomic_store(&gb->xcnt[x][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned b =
+                __hip_atomic_fetch_add(&gb->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            last = b == ng - 1;
+        }
+        if (last) {
+            __hip_atomic_store(&gb->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&gb->gen, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long spins = 0;
+            while (__hip_atomic_load(&gb->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (__hip_atomic_load(&gb->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                    ++spins > kBarrierSpins) {
+                    __hip_atomic_store(&gb->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    good = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(8);
+            }
+        }
+        __threadfence();
+        gen++;
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+// chunk_sum and resid_sum as a block of 1024 threads forms them, computed by
+// a block of kBlock = 256: thread t plays threads t, t + 256, t + 512 and
+// t + 768 (same elements, same order; lane 0 of wave w holds the wave sum of
+// waves w, w + 4, w + 8, w + 12), so the sums are bitwise equal.  The result
+// is valid in every thread.
+__device__ __forceinline__ double sum1024_finish(const double (&s)[4]) {
+    __shared__ double sm[16];
+    __syncthreads();  // a previous call's reads of sm are done
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int r = 0; r < 4; r++) sm[(threadIdx.x >> 6) + 4 * r] = s[r];
+    __syncthreads();
+    double S = sm[0];
+    for (int w = 1; w < 16; w++) S += sm[w];
+    return S;
+}
+
+__device__ __attribute__((noinline)) double chunk_sum_by256(const double* __restrict__ pa,
+                                                  const double* __restrict__ pb, int nb, int nc,
+                                                  int ch, int q) {
+    const double* src = q < nc ? pa : pb;
+    const int c = q < nc ? q : q - nc;
+    const int lo = c * ch, hi = lo + ch < nb ? lo + ch : nb;
+    double s[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        double a = 0.0;
+        for (int i = lo + (int)threadIdx.x + 256 * r; i < hi; i += 1024) a += src[i];
+        s[r] = wave_sum(a);
+    }
+    return sum1024_finish(s);
+}
+
+__device__ __attribute__((noinline)) double resid_sum_by256(const double* __restrict__ pa, int na,
+                                                  const double* __restrict__ pb, int nbp) {
+    double s[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int t = (int)threadIdx.x + 256 * r;
+        double a = 0.0;
+        for (int q = t; q < na; q += 1024) a += pa[q];
+        double a2 = 0.0;
+        for (int q = t; q < nbp; q += 1024) a2 += pb[q];
+        a = a + a2;
+        s[r] = wave_sum(a);
+    }
+    return sum1024_finish(s);
+}
+
+// One colour pass of the fused solve: this block's work items.
+template <typename T, int RESID, int U>
+__device__ __forceinline__ void fused_pass(const Geo& G, const SweepMap& M, T c, T om, double h4,
+                                           int colour, const T* xin, T* xout, const T* y,
+                                           const T* opnd, double* partials) {
+    const int nitems = sweep_grid(M);
+    for (int b = blockIdx.x; b < nitems; b += gridDim.x) {
+        int bx, kb;
+        if (map_block(M, b, nitems, bx, kb))
+            sor_block<T, RESID, U, false>(G, M, bx, kb, c, om, h4, colour, xin, xout, y, opnd,
+                                          partials, nullptr);
+    }
+}
+
+// The whole solve: red(1), black(1), then per iteration z red(z+1) (the red
+// residual terms of z), the chunk sums, the stopping test of z -- every
+// block forms the final sum itself, in the fixed order, so all take the same
+// decision -- and black(z+1).  red(1) runs the same residual-forming pass as
+// the later red passes (its terms are overwritten by red(2) before they are
+// summed; the update is the same), so that the kernel holds two pass bodies
+// only: with a third, the compiler's allocation grew to 155 VGPRs (3 waves
+// per SIMD) against 112-121.  Work items are the blocks of the single-pass
+// launch (same map, same partial index), block b of this grid taking items
+// b, b + gridDim.x, ... (gridDim.x a multiple of 8 unless the pass has fewer
+// than 8 blocks: an item runs on the XCD its own launch would give it).  Hence u, every residual and the sweep
+// count are bitwise those of the pass-per-launch solve.  ctl: tol set, stop
+// clear on entry; on exit R, z, stop and zfinal as k_reduce_resid leaves
+// them (gb->abort set instead if a barrier timed out).
+template <typename T, int U>
+static __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_sor_fused(
+    Geo G, SweepMap M, T c, T om, double h4, T* red0, T* red1, T* black,
+    const T* __restrict__ op0, const T* __restrict__ op1, double* part_r, double* part_b, int nb,
+    int nc, int ch, double* red2, SorCtl* ctl, int cap, GridBar* gb) {
+    unsigned gen = 0;
+    const double tol = ctl->tol;
+    for (int z = 0;; z++) {
+        // red(z + 1) reads red[z & 1] and writes red[(z + 1) & 1]
+        T* rin = (z & 1) ? red1 : red0;
+        T* rout = (z & 1) ? red0 : red1;
+        fused_pass<T, 1, U>(G, M, c, om, h4, 0, rin, rout, black, op0, part_r);
+        if (!grid_barrier(gb, gen)) return;
+        if (z > 0) {
+            for (int q = blockIdx.x; q < 2 * nc; q += gridDim.x) {
+                const double S = chunk_sum_by256(part_r, part_b, nb, nc, ch, q);
+                if (threadIdx.x == 0) red2[q] = S;
+            }
+            if (!grid_barrier(gb, gen)) return;
+            const double R = resid_sum_by256(red2, nc, red2 + nc, nc);
+            const bool cont = (R > tol) && (z < cap);
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                ctl->R = R;
+                ctl->z = z;
+                if (!cont) {
+                    ctl->stop = 1;
+                    ctl->zfinal = z;
+                }
+            }
+            if (!cont) return;
+        }
+        fused_pass<T, 2, U>(G, M, c, om, h4, 1, black, black, rout, op1, part_b);
+        if (!grid_barrier(gb, gen)) return;
+    }
+}
+
 // First level of a two-level fixed-order sum: block b of 1024 threads sums
 // in[b * chunk .. min(n, (b + 1) * chunk)) into out[b].
 static __global__ __launch_bounds__(1024) void k_sum_chunks(const double* __restrict__ in, long n,

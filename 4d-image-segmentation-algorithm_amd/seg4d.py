@@ -155,6 +155,9 @@ class SegLib(_Lib):
         L.seg4dSetSweepTile.argtypes = [c_int, c_int]
         L.seg4dSetInnerOmega.restype = c_double
         L.seg4dSetInnerOmega.argtypes = [c_double]
+        L.seg4dSetFusedSweeps.restype = c_int
+        L.seg4dSetFusedSweeps.argtypes = [c_int]
+        L.seg4dLastSolveFused.restype = c_int
         for fn in ("seg4dSetSweepChunk", "seg4dSetPipelineDepth", "seg4dSetSweepBands"):
             getattr(L, fn).restype = c_int
             getattr(L, fn).argtypes = [c_int]
@@ -475,6 +478,14 @@ class SegLib(_Lib):
 
     def set_sweep_variant(self, v):
         return self.lib.seg4dSetSweepVariant(int(v))
+
+    def set_fused_sweeps(self, mode):
+        """0 one launch per pass (default), 1 the whole solve in one
+        cooperative launch whenever it fits."""
+        return self.lib.seg4dSetFusedSweeps(int(mode))
+
+    def last_solve_fused(self):
+        return self.lib.seg4dLastSolveFused()
 
     def autotune(self, reps=3):
         """Tile of the fastest SOR launch order on the current volume:

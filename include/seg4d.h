@@ -217,6 +217,16 @@ int seg4dSetSweepBands(int nband);
  * Passes enqueued after the stop are no-ops; results are identical.
  * Returns the previous value. */
 int seg4dSetPipelineDepth(int depth);
+/* Fused sweeps: 1 runs every SOR solve (and FP32 correction solve) as one
+ * cooperative launch that holds all its red passes, black passes and
+ * stopping tests, with grid-wide barriers between them, whenever the grid
+ * fits on the GPU; 0 (the default) one launch per pass.  Results and sweep
+ * counts are identical; on an MI355X the fused solve is the slower one at
+ * every size measured (its grid barriers cost more than the launch gaps they
+ * remove).  Returns the previous value (-2 without a volume). */
+int seg4dSetFusedSweeps(int mode);
+/* 1 if the last SOR solve (or correction solve) ran fused. */
+int seg4dLastSolveFused(void);
 /* Arithmetic of the SOR solve.  0 (default): FP64, the reference's
  * arithmetic and bits.  1: FP32-refined -- FP64 defect correction in which
  * the defect b - A u of the FP64 system is recomputed in FP64 on the GPU and
