@@ -300,3 +300,26 @@ def test_banded_order_vs_oracle(seglib, oracle, variant, bands, depth):
         lib.lib.seg4dSetPipelineDepth(0)
         lib.lib.seg4dSetSweepBands(0)
         lib.deallocate()
+
+
+@pytest.mark.parametrize("dims", [(9, 8, 7, 6), (1, 1, 1, 1), (33, 5, 4, 3)])
+def test_reflection_entry_point(seglib, oracle, dims):
+    """_reflection4Ddata_ called on its own (mirror mode, host array in and
+    out): every ghost equals the oracle's sequential X, Y, Z, W copies, and
+    the interior is untouched, bitwise."""
+    nx, ny, nz, nt = dims
+    print(f"[gpu] _reflection4Ddata_ {dims}", flush=True)
+    shape = (nt + 2, nz + 2, ny + 2, nx + 2)
+    rng = np.random.default_rng(sum(dims) + 101)
+    a = rng.uniform(-1.0, 1.0, size=shape)
+    lib = seglib
+    lib.allocate(nx, ny, nz, nt)
+    try:
+        lib.arr("i0")[...] = a
+        lib.reflect("i0")
+        ref = a.copy()
+        oracle.reflection(params(shape), ref)
+        assert np.array_equal(lib.arr("i0").view(np.uint64), ref.view(np.uint64))
+        assert np.array_equal(lib.arr("i0")[1:-1, 1:-1, 1:-1, 1:-1], a[1:-1, 1:-1, 1:-1, 1:-1])
+    finally:
+        lib.deallocate()
