@@ -202,7 +202,8 @@ public:
     bool fused_last_ = false;      // the last SOR solve ran fused
     GridBar* gbar_ = nullptr;      // the fused launch's barrier state
     GridBar* hgbar_ = nullptr;     // pinned mirror (abort flag)
-    int fused_blocks_ = 0;         // co-resident blocks of k_sor_fused (0: not yet known)
+    int fused_blocks_[2] = {0, 0}; // co-resident blocks of k_sor_fused<double / float>
+                                   // (0: not yet known, -1: the launch never fits)
     // 0 = FP64 (the reference's arithmetic); 1 = FP32-refined: FP32 SOR on
     // the correction equation inside FP64 defect correction.
     int precision_ = 0;

@@ -650,7 +650,8 @@ int sor_run_fused(Engine& E, const SorJob<T>& J, const Geo& G, const SweepMap& M
     // load phase, plain operand loads)
     if ((E.sweep_variant_ & 5) != 1 || E.prof.on) return 1;
     auto kern = k_sor_fused<T, 2>;
-    if (E.fused_blocks_ == 0) {
+    int& fb = E.fused_blocks_[sizeof(T) == 4 ? 1 : 0];
+    if (fb == 0) {
         int dev = 0, ncu = 0, per = 0;
         if (E.check(hipGetDevice(&dev), "hipGetDevice") ||
             E.check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev),
@@ -658,12 +659,13 @@ int sor_run_fused(Engine& E, const SorJob<T>& J, const Geo& G, const SweepMap& M
             E.check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)kern, kBlock,
                                                                    0), "occupancy"))
             return -1;
-        E.fused_blocks_ = per * ncu > 0 ? per * ncu : -1;
+        fb = per * ncu > 0 ? per * ncu : -1;
     }
+    if (fb < 0) return 1;
     // a multiple of 8 blocks keeps every work item on its own launch's XCD;
     // grids of fewer than 8 items run as they are
     const int items = sweep_grid(M);
-    int nblk = std::min(E.fused_blocks_, items);
+    int nblk = std::min(fb, items);
     if (nblk >= 8) nblk = nblk / 8 * 8;
     if (!E.gbar_) {
         if (E.check(hipMalloc((void**)&E.gbar_, sizeof(GridBar)), "hipMalloc barrier") ||
@@ -694,7 +696,7 @@ int sor_run_fused(Engine& E, const SorJob<T>& J, const Geo& G, const SweepMap& M
                                                      dim3(kBlock), args, 0, E.st_);
     if (el == hipErrorCooperativeLaunchTooLarge) {
         (void)hipGetLastError();
-        E.fused_blocks_ = -1;  // never fits on this device: pass by pass from now on
+        fb = -1;  // never fits on this device: pass by pass from now on
         return 1;
     }
     if (E.check(el, "k_sor_fused") ||
@@ -723,7 +725,7 @@ int sor_run(Engine& E, const SorJob<T>& J, int* zout) {
     int rnc = 0, rch = 0;
     resid_chunking(nb, &rnc, &rch);
     E.fused_last_ = false;
-    if (E.fused_ == 1 && E.fused_blocks_ >= 0) {
+    if (E.fused_ == 1) {
         const int r = sor_run_fused<T>(E, J, G, M, nb, rnc, rch, zout);
         if (r <= 0) return r;
     }

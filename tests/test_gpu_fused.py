@@ -46,10 +46,10 @@ def _run(lib, dims, img, u0, fused, steps, precision=0, warm=0, variant=None, gs
         lib.set_fused_sweeps(fused)
         if variant is not None:
             lib.set_sweep_variant(variant)
-        lib.set_precision(precision)
         lib.set_warm_start(warm)
         out = []
         for st in range(steps):
+            lib.set_precision(precision[st] if isinstance(precision, (list, tuple)) else precision)
             lib.set(numberOfTimeStep=st + 1)
             z, _ = lib.seg_time_step()
             out.append((lib.arr("u").copy(), z, lib.last_rounds(), lib.last_solve_fused()))
@@ -133,3 +133,19 @@ def test_fused_matches_pass_per_launch(seglib, precision, warm, variant):
         assert a[2] == b[2], f"step {s + 1}: rounds {a[2]} vs {b[2]}"
         assert np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64)), f"step {s + 1}"
     print(f"  sweeps {[r[1] for r in on]}", flush=True)
+
+
+def test_fused_precision_switches(seglib):
+    """FP32, FP64 and FP32 again within one allocation (FP32 first, so the
+    FP64 solve sizes its own cooperative grid afterwards): every solve fused,
+    bitwise the pass-per-launch steps."""
+    print("[gpu] fused solves across precision switches", flush=True)
+    dims = (40, 33, 16, 7)
+    img, u0 = _inputs(dims, 23)
+    seq = [1, 0, 0, 1, 0]
+    on = _run(seglib, dims, img, u0, 1, len(seq), seq)
+    off = _run(seglib, dims, img, u0, 0, len(seq), seq)
+    for s, (a, b) in enumerate(zip(on, off)):
+        assert a[3] == 1, f"step {s + 1} did not run fused"
+        assert a[1] == b[1] and a[2] == b[2], f"step {s + 1}"
+        assert np.array_equal(a[0].view(np.uint64), b[0].view(np.uint64)), f"step {s + 1}"
