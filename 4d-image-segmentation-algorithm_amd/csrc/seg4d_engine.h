@@ -196,8 +196,8 @@ public:
     // grid barriers between the passes).  0 = off (one launch per pass),
     // 1 = whenever the grid fits on the GPU.  Identical results either way.
     // Off by default: measured slower at every size (a grid barrier with its
-    // agent-scope release and acquire costs ~100 us, an iteration of the
-    // pass-per-launch solve of 64x64x32x8 36 us).
+    // agent-scope release and acquire in every block costs ~40 us, a whole
+    // iteration of the pass-per-launch solve of 64x64x32x8 36 us).
     int fused_ = 0;
     bool fused_last_ = false;      // the last SOR solve ran fused
     GridBar* gbar_ = nullptr;      // the fused launch's barrier state

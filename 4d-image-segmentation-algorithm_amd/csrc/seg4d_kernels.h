@@ -883,13 +883,17 @@ static __global__ void k_bar_reset(GridBar* gb) {
 }
 
 // Grid-wide barrier for a cooperative launch of gridDim.x blocks, in two
-// levels: thread 0 of each block releases the block's writes at agent scope
-// and arrives on its group's counter (group = blockIdx % 8, the XCD the block
-// runs on); the last block of a group arrives on the group counter, and the
-// last group advances the generation every block waits for.  The wait is
-// bounded: a timed-out wait raises the abort flag, which every waiting block
-// sees, so that all blocks leave the kernel instead of hanging.  Returns
-// false after an abort.  `gen` is the block's count of barriers passed.
+// levels: thread 0 of each block releases the block's writes (one agent-scope
+// fence) and arrives on its group's counter (group = blockIdx % 8, the XCD
+// the block runs on); the last block of a group arrives on the group counter,
+// and the last group advances the generation every block waits for.  The
+// counters and the generation are accessed with relaxed agent-scope atomics,
+// the memory order coming from the fence before the arrival and one fence
+// after the wait (an acquire on every poll would invalidate the XCD's L2 at
+// each of them, under the blocks still working).  The wait is bounded: a
+// timed-out wait raises the abort flag, which every waiting block sees, so
+// that all blocks leave the kernel instead of hanging.  Returns false after
+// an abort.  `gen` is the block's count of barriers passed.
 __device__ __forceinline__ bool grid_barrier(GridBar* gb, unsigned& gen) {
     __shared__ int ok;
     __syncthreads();
@@ -898,22 +902,25 @@ __device__ __forceinline__ bool grid_barrier(GridBar* gb, unsigned& gen) {
         const unsigned ng = gridDim.x < 8 ? gridDim.x : 8u;
         const unsigned x = blockIdx.x % 8;
         const unsigned gsize = (gridDim.x - x + 7) / 8;  // blocks of group x
-        __threadfence();
+        __threadfence();  // release: this block's writes before its arrival
         bool last = false;
         const unsigned a =
-            __hip_atomic_fetch_add(&gb->xcnt[x][0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&gb->xcnt[x][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (a == gsize - 1) {
+            // every block of the group has arrived (and released its writes)
             __hip_atomic_store(&gb->xcnt[x][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
             const unsigned b =
-                __hip_atomic_fetch_add(&gb->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&gb->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             last = b == ng - 1;
         }
         if (last) {
             __hip_atomic_store(&gb->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&gb->gen, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            __hip_atomic_store(&gb->gen, gen + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             long spins = 0;
-            while (__hip_atomic_load(&gb->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+            while (__hip_atomic_load(&gb->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 if (__hip_atomic_load(&gb->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
                     ++spins > kBarrierSpins) {
                     __hip_atomic_store(&gb->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -923,7 +930,7 @@ __device__ __forceinline__ bool grid_barrier(GridBar* gb, unsigned& gen) {
                 __builtin_amdgcn_s_sleep(8);
             }
         }
-        __threadfence();
+        __threadfence();  // acquire: the other blocks' writes after the wait
         gen++;
         ok = good;
     }

@@ -178,6 +178,9 @@ def main():
                          "cubic start) that attribute the refined forms' speed-up")
     ap.add_argument("--mirror-steps", type=int, default=2,
                     help="time steps timed in mirror mode (host arrays, PCIe) after the runs")
+    ap.add_argument("--fused", action="store_true",
+                    help="after the timed run, time the same FP64 steps again with fused sweeps "
+                         "(each solve one cooperative launch) and check them bitwise")
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed region, one device copy of u (padded volume, "
                          "FP64) as the known-size reference for rocprofv3 --pmc byte counts")
@@ -293,6 +296,37 @@ def main():
     mask64 = inner >= thr
     mask_frac = float(mask64.mean())
     u64 = inner.copy()
+
+    fused = None
+    if args.fused:
+        # the same FP64 steps with every solve in one cooperative launch (not
+        # profiled: per-pass events would split the launch)
+        lib.set_fused_sweeps(1)
+        lib.arr("u")[...] = u0
+        lib.upload("u")
+        lib.set(numberOfTimeStep=1)
+        lib.seg_time_step()
+        lib.arr("u")[...] = u0
+        lib.upload("u")
+        lib.synchronize()
+        fsweeps, fflags = [], []
+        tf0 = time.perf_counter()
+        for s in range(args.steps):
+            lib.set(numberOfTimeStep=s + 1)
+            z, _ = lib.seg_time_step()
+            fsweeps.append(z)
+            fflags.append(lib.last_solve_fused())
+        lib.synchronize()
+        tf = time.perf_counter() - tf0
+        lib.set_fused_sweeps(0)
+        lib.download("u")
+        same = bool(np.array_equal(lib.arr("u")[1:-1, 1:-1, 1:-1, 1:-1].view(np.uint64),
+                                   u64.view(np.uint64))) and fsweeps == sweeps
+        fused = {"time_s": tf, "ms_per_step": tf / args.steps * 1e3,
+                 "speed_vs_pass_per_launch": total / tf, "all_solves_fused": all(fflags),
+                 "bitwise_equal_to_pass_per_launch": same}
+        log(f"[bench] fused sweeps: {tf:.3f} s ({total / tf:.2f}x the pass-per-launch solve), "
+            f"bitwise equal {same}")
 
     def run_form(precision, warm, omega=1.8, label=None):
         """The same time steps from the same u0 in another form: precision 1 =
@@ -485,6 +519,7 @@ def main():
         "fp32_refined_cubic_start": mixed_cubic,
         "fp32_refined_learned_start": learned,
         "controls_non_parity": controls,
+        "fused_sweeps": fused,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
