@@ -440,16 +440,24 @@ def main():
     if args.mirror_steps > 0:
         lib.set_resident(False)
         lib.arr("u")[...] = u0
-        ms = []
+        ms, l2ms = [], []
         for s in range(args.mirror_steps):
             lib.set(numberOfTimeStep=s + 1)
             t = time.perf_counter()
             lib.seg_time_step()
             ms.append((time.perf_counter() - t) * 1e3)
+            # the reference driver's stopping test after every step
+            # (segmentation4D.c:1408): both host arrays go up again
+            t = time.perf_counter()
+            lib.l2norm("prev_u", "u")
+            l2ms.append((time.perf_counter() - t) * 1e3)
         mirror = {"ms_per_step": ms, "resident_ms_per_step_same_steps":
                   total / args.steps * 1e3,
-                  "pcie_bytes_per_step": 3 * 8 * int(np.prod(shape))}  # u up, Mmask up, u down
-        log(f"[bench] mirror mode: {[round(x, 1) for x in ms]} ms per step")
+                  "pcie_bytes_per_step": 3 * 8 * int(np.prod(shape)),  # u up, Mmask up, u down
+                  "l2norm_ms_after_step": l2ms,
+                  "driver_loop_ms_per_step": [a + b for a, b in zip(ms, l2ms)]}
+        log(f"[bench] mirror mode: {[round(x, 1) for x in ms]} ms per step, "
+            f"_l2norm_(prev_u, u) after it {[round(x, 1) for x in l2ms]} ms")
         lib.set_resident(True)
 
     red1_avg_s = prof["red1_ms"] / max(prof["red1_launches"], 1) / 1e3
