@@ -694,13 +694,14 @@ int sor_run_fused(Engine& E, const SorJob<T>& J, const Geo& G, const SweepMap& M
                     &nb, &rnc, &rch, &red2, &ctl, &cap, &gb};
     const hipError_t el = hipLaunchCooperativeKernel((const void*)kern, dim3((unsigned)nblk),
                                                      dim3(kBlock), args, 0, E.st_);
-    if (el == hipErrorCooperativeLaunchTooLarge) {
+    if (el != hipSuccess) {
+        // refused (too large for the device, or no cooperative launch): pass
+        // by pass from now on, with the same results; nothing was enqueued
         (void)hipGetLastError();
-        fb = -1;  // never fits on this device: pass by pass from now on
+        fb = -1;
         return 1;
     }
-    if (E.check(el, "k_sor_fused") ||
-        E.check(hipMemcpyAsync(J.hctl, J.ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, E.st_),
+    if (E.check(hipMemcpyAsync(J.hctl, J.ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, E.st_),
                 "ctl copy") ||
         E.check(hipMemcpyAsync(E.hgbar_, E.gbar_, sizeof(GridBar), hipMemcpyDeviceToHost,
                                E.st_), "barrier copy") ||
