@@ -3,11 +3,15 @@
   cfg 0  64x64x32x8, FP64: edge indicator and 20 time steps bitwise equal to
          the oracle after every step (u, prev_u, sweep count), identical final
          mask; the C driver bin/seg4d_run on the same workload writes the same
-         mask.
+         mask; the FP32-refined form from the cubic start, 20 steps, against
+         the oracle's u after every step (max|du| <= 1e-4 max|u|, mask IoU
+         >= 0.999, FP64 residual bound met).
   cfg 1  256x256x128x16, FP64: edge indicator and the first two time steps
          bitwise equal to the oracle (threaded over t with serial residual
          sums, par_reduce = 2: bitwise the serial oracle, pinned in
-         test_oracle_omp.py), equal sweep counts, identical mask.
+         test_oracle_omp.py), equal sweep counts, identical mask; the
+         FP32-refined form (linear start) against the oracle's u after both
+         steps, with the same bounds.
   cfg 2  the same volume, FP32-refined, all 30 steps: every step meets the
          FP64 stopping test in at most 8 correction rounds; mask IoU >= 0.999
          against the FP64 path after every step (whose first steps cfg 1 pins
@@ -73,8 +77,12 @@ def iou(a, b):
     return float(np.logical_and(a, b).sum() / un) if un else 1.0
 
 
-def seg_parity(lib, oracle, preset, steps, check_driver=False):
-    """Edge indicator + `steps` FP64 time steps, GPU vs oracle, bitwise."""
+def seg_parity(lib, oracle, preset, steps, refined_warm=None):
+    """Edge indicator + `steps` FP64 time steps, GPU vs oracle, bitwise.
+    refined_warm: afterwards the same steps from the same u0 in the
+    FP32-refined form with that warm start, each step's u against the
+    oracle's u of that step (max |du| <= 1e-4 max|u|, mask IoU >= 0.999) and
+    its FP64 residual within the bound."""
     dims, shape, img, u0 = volume(lib, preset)
     nthr = threads()
     base = params(shape, **KW)
@@ -100,10 +108,13 @@ def seg_parity(lib, oracle, preset, steps, check_driver=False):
         coef = zeros(shape, 8)
         mask = np.ones(shape)
         sweeps = []
+        u_oracle = []
         for st in range(steps):
             tol = REL2 * oracle.sumsq_interior(base, u)
             p = params(shape, gsTol=tol, nthreads=nthr, par_reduce=2, **KW)
             z_o, _ = oracle.seg_time_step(p, u, prev, g, coef, mask)
+            if refined_warm is not None:
+                u_oracle.append(interior(u).copy())
             lib.set(gsTol=tol, numberOfTimeStep=st + 1)
             z_g, _ = lib.seg_time_step()
             assert z_g == z_o, f"step {st + 1}: {z_g} sweeps against the oracle's {z_o}"
@@ -114,14 +125,44 @@ def seg_parity(lib, oracle, preset, steps, check_driver=False):
         m_o = mask_of(u)
         assert np.array_equal(mask_of(lib.arr("u")), m_o)
         print(f"  final mask identical ({int(m_o.sum())} voxels); sweeps {sweeps}", flush=True)
+        if refined_warm is not None:
+            # the device keeps the edge indicator; the relative tolerance is
+            # the workload's (sum of the step's own prev^2), as in the bench
+            lib.set(gsTol=0.0)
+            lib.set_relative_tolerance(1e-6)
+            lib.set_resident(True)
+            lib.upload("Mmask")
+            lib.set_precision(1)
+            lib.set_warm_start(refined_warm)
+            lib.arr("u")[...] = u0
+            lib.upload("u")
+            worst_du, worst_iou, fp32 = 0.0, 1.0, []
+            for st in range(steps):
+                lib.set(numberOfTimeStep=st + 1)
+                z, R = lib.seg_time_step()
+                assert R <= lib.lib.seg4dLastTolerance(), f"refined step {st + 1}: R {R}"
+                lib.download("u")
+                ur, uo = interior(lib.arr("u")), u_oracle[st]
+                du = float(np.abs(ur - uo).max())
+                io = iou(ur >= 0.5 * (ur.max() + ur.min()), uo >= 0.5 * (uo.max() + uo.min()))
+                assert du <= 1e-4 * np.abs(uo).max(), f"refined step {st + 1}: max|du| {du}"
+                assert io >= 0.999, f"refined step {st + 1}: IoU {io} against the oracle"
+                worst_du, worst_iou = max(worst_du, du), min(worst_iou, io)
+                fp32.append(z)
+            print(f"  FP32-refined (warm {refined_warm}) against the oracle: FP32 sweeps {fp32}, "
+                  f"max|du| {worst_du:.2e}, min IoU {worst_iou:.6f}", flush=True)
     finally:
+        lib.set_precision(0)
+        lib.set_warm_start(0)
+        lib.set_resident(False)
+        lib.set_relative_tolerance(0.0)
         lib.deallocate()
     return m_o
 
 
 def test_cfg0_twenty_steps_and_driver(seglib, oracle, tmp_path):
     print("[gpu] cfg 0: 64x64x32x8, edge indicator + 20 steps vs the oracle", flush=True)
-    m_o = seg_parity(seglib, oracle, 0, 20)
+    m_o = seg_parity(seglib, oracle, 0, 20, refined_warm=4)
     # the C driver (segmentation4D.c:1326-1424's call sequence) on the same
     # workload: relative tolerance 1e-6, 20 steps, mask written as bytes
     out = tmp_path / "mask.raw"
@@ -136,7 +177,7 @@ def test_cfg0_twenty_steps_and_driver(seglib, oracle, tmp_path):
 
 def test_cfg1_edge_indicator_and_two_steps(seglib, oracle):
     print("[gpu] cfg 1: 256x256x128x16, edge indicator + 2 steps vs the oracle", flush=True)
-    seg_parity(seglib, oracle, 1, 2)
+    seg_parity(seglib, oracle, 1, 2, refined_warm=1)
 
 
 def test_cfg2_refined_thirty_steps(seglib):
