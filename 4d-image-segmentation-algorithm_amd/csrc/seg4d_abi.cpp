@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "host_mem.h"
+#include "host_track.h"
 #include "seg4d_batch.h"
 #include "seg4d_engine.h"
 #include "seg4d_net.h"
@@ -66,12 +67,57 @@ bool fail(const char* what) {
     return false;
 }
 
-bool need_engine(const char* fn) {
+// Host-write tracking of u, prev_u and Mmask in mirror mode (host_track.h):
+// off unless switched on by seg4dSetWriteTracking(1) or SEG4D_TRACK_WRITES=1
+// in the environment at the first _allocateMem_.
+enum { TR_U = 0, TR_PREV = 1, TR_MASK = 2 };
+int g_track = -1;  // -1: not yet read from the environment
+
+int track_slot(int id) {
+    if (id == s4d::A_U) return TR_U;
+    if (id == s4d::A_PREV) return TR_PREV;
+    if (id == s4d::A_MASK) return TR_MASK;
+    return -1;
+}
+
+// Tracked arrays whose host copy is known to equal the device copy skip
+// their upload.  Only _segTimeStep_ and _l2norm_ keep that knowledge; every
+// other entry point (need_engine) first makes all three arrays dirty, since
+// it may change the device copies.
+void track_register() {
+    s4d::track::forget_all();
+    if (g_track != 1 || !u) return;
+    if (!s4d::track::install()) return;
+    const size_t bytes = (size_t)nelem() * sizeof(double);
+    s4d::track::set(TR_U, u, bytes);
+    s4d::track::set(TR_PREV, prev_u, bytes);
+    s4d::track::set(TR_MASK, Mmask, bytes);
+}
+
+bool tracked_clean(int id) {
+    const int s = track_slot(id);
+    return s >= 0 && !g_resident && s4d::track::clean(s);
+}
+
+void mark_clean(int id) {
+    const int s = track_slot(id);
+    if (s >= 0 && !g_resident) s4d::track::mark_clean(s);
+}
+
+// For the tracked entry points: the engine is usable; tracking knowledge kept
+// (dropped in resident mode, where the device copies change unseen).
+bool need_engine_tracked(const char* fn) {
+    if (g_resident) s4d::track::touch_all();
     if (g_eng && g_eng->ready()) return true;
     std::string m = std::string(fn) + ": library not allocated (call _allocateMem_ first)";
     s4d::set_error(m);
     std::fprintf(stderr, "seg4d: %s\n", m.c_str());
     return false;
+}
+
+bool need_engine(const char* fn) {
+    s4d::track::touch_all();
+    return need_engine_tracked(fn);
 }
 
 // Library array id of a host pointer, -1 if foreign, -(10+f) for coef f.
@@ -127,7 +173,10 @@ s4d::Balls collect_balls() {
 double* device_of(double* hp, s4d::ArrayId spare, bool upload) {
     const int id = resolve(hp);
     if (id >= 0) {
-        if (upload && !g_resident && g_eng->upload((s4d::ArrayId)id, hp)) return nullptr;
+        if (upload && !g_resident && !tracked_clean(id)) {
+            if (g_eng->upload((s4d::ArrayId)id, hp)) return nullptr;
+            mark_clean(id);
+        }
         return g_eng->full((s4d::ArrayId)id);
     }
     if (g_resident || id <= -10) {
@@ -166,6 +215,7 @@ bool upload_g_if_dirty() {
 s4d::HostArrays g_host;
 
 void free_host() {
+    s4d::track::forget_all();
     double** all[] = {&u, &prev_u, &i0, &Mmask, &e, &w, &n, &s, &t, &b, &fw, &bw,
                       &g_e, &g_w, &g_n, &g_s, &g_t, &g_b, &g_fw, &g_bw};
     for (double** p : all) {
@@ -241,6 +291,11 @@ API bool _allocateMem_(void) {
         return fail("_allocateMem_: device allocation failed");
     g_eng->net_ = g_net;
     g_g_dirty = true;
+    if (g_track < 0) {
+        const char* ev = std::getenv("SEG4D_TRACK_WRITES");
+        g_track = (ev && ev[0] == '1') ? 1 : 0;
+    }
+    track_register();
     return true;
 }
 
@@ -257,7 +312,7 @@ API bool _deallocateMem_(void) {
 // ---- segmentation4D.c:194-213 ------------------------------------------------
 API double _l2norm_(double* a, double* bb) {
     if (!a || !bb) return false;
-    if (!need_engine("_l2norm_")) return 0.0;
+    if (!need_engine_tracked("_l2norm_")) return 0.0;
     double* da = device_of(a, s4d::A_TMP, true);
     double* db = device_of(bb, s4d::A_TMP2, true);
     if (!da || !db) {
@@ -286,18 +341,23 @@ API bool _gs_coefficients_(void) {
 
 // ---- segmentation4D.c:215-315 ------------------------------------------------
 API bool _segTimeStep_(void) {
-    if (!need_engine("_segTimeStep_")) return false;
+    if (!need_engine_tracked("_segTimeStep_")) return false;
     if (!upload_g_if_dirty()) return fail("_segTimeStep_: upload g");
     // Mirror mode: u goes up before the step; Mmask, read only by the step's
     // epilogue, goes up beside it on a second stream; the reference's
     // prev_u <- u (segmentation4D.c:224) is done on the host while the device
-    // computes; after the step only u comes down.
+    // computes; after the step only u comes down.  With write tracking, u and
+    // Mmask go up only when the host wrote them since they last matched the
+    // device copies.
     std::thread prev_copy;
     if (!g_resident) {
-        if (g_eng->upload(s4d::A_U, u) || g_eng->upload_side(s4d::A_MASK, Mmask)) {
+        if ((!tracked_clean(s4d::A_U) && g_eng->upload(s4d::A_U, u)) ||
+            (!tracked_clean(s4d::A_MASK) && g_eng->upload_side(s4d::A_MASK, Mmask))) {
             g_eng->finish_side();
+            s4d::track::touch_all();
             return fail("_segTimeStep_: upload");
         }
+        s4d::track::touch(TR_PREV);  // written by the copy below
         prev_copy = std::thread([] { s4d::host_copy(prev_u, u, (size_t)nelem()); });
     }
     const s4d::Balls balls = collect_balls();
@@ -307,10 +367,20 @@ API bool _segTimeStep_(void) {
     const int rs = g_eng->finish_side();
     // a failed step leaves u as it was; in mirror mode prev_u then holds u^n,
     // as after the reference's first statement
-    if (rc || rs) return fail("_segTimeStep_");
+    if (rc || rs) {
+        s4d::track::touch_all();
+        return fail("_segTimeStep_");
+    }
     g_last = st;
     if (!g_resident) {
-        if (g_eng->download(s4d::A_U, u)) return fail("_segTimeStep_: download");
+        if (g_eng->download(s4d::A_U, u)) {
+            s4d::track::touch_all();
+            return fail("_segTimeStep_: download");
+        }
+        // host and device now hold u^(n+1), u^n and the mask alike
+        mark_clean(s4d::A_U);
+        mark_clean(s4d::A_PREV);
+        mark_clean(s4d::A_MASK);
     }
     if (g_verbose) {
         std::printf("Step is %d\n", numberOfTimeStep);
@@ -494,6 +564,7 @@ API bool _generate4DsegmFunct_(double* a) {
 
 // ---- extensions ------------------------------------------------------------------
 API int seg4dSetResident(int on) {
+    s4d::track::touch_all();
     const int old = g_resident;
     g_resident = on ? 1 : 0;
     return old;
@@ -523,7 +594,22 @@ API bool seg4dDownload(double* hp) {
 
 API void seg4dHostWrote(double* hp) {
     const int id = resolve(hp);
+    const int slot = track_slot(id);
+    if (slot >= 0) s4d::track::touch(slot);
     if (id >= s4d::A_G0 && id <= s4d::A_G7) g_g_dirty = true;
+}
+
+API int seg4dSetWriteTracking(int on) {
+    const int old = g_track == 1 ? 1 : 0;
+    g_track = on ? 1 : 0;
+    track_register();
+    return old;
+}
+
+API int seg4dWriteTrackingState(void) {
+    if (g_track != 1 || !s4d::track::current()) return 0;
+    return 1 | (s4d::track::clean(TR_U) ? 2 : 0) | (s4d::track::clean(TR_PREV) ? 4 : 0) |
+           (s4d::track::clean(TR_MASK) ? 8 : 0);
 }
 
 API void seg4dSetVerbose(int on) { g_verbose = on; }

@@ -141,6 +141,10 @@ class SegLib(_Lib):
         L.seg4dSetResident.argtypes = [c_int]
         L.seg4dSetRelativeTolerance.argtypes = [c_double]
         L.seg4dHostWrote.argtypes = [POINTER(c_double)]
+        L.seg4dSetWriteTracking.restype = c_int
+        L.seg4dSetWriteTracking.argtypes = [c_int]
+        L.seg4dWriteTrackingState.restype = c_int
+        L.seg4dWriteTrackingState.argtypes = []
         L.seg4dSetVerbose.argtypes = [c_int]
         L.seg4dLastError.restype = c_char_p
         L.seg4dLastSweeps.restype = c_int
@@ -340,6 +344,15 @@ class SegLib(_Lib):
 
     def set_resident(self, on):
         return self.lib.seg4dSetResident(1 if on else 0)
+
+    def set_write_tracking(self, on):
+        """Mirror-mode host-write tracking of u, prev_u and Mmask; returns the
+        previous setting."""
+        return self.lib.seg4dSetWriteTracking(1 if on else 0)
+
+    def write_tracking_state(self):
+        """Bit 0: tracking active; bits 1-3: u, prev_u, Mmask clean."""
+        return self.lib.seg4dWriteTrackingState()
 
     def set_relative_tolerance(self, rel):
         self.lib.seg4dSetRelativeTolerance(float(rel))

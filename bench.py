@@ -458,6 +458,29 @@ def main():
                   "driver_loop_ms_per_step": [a + b for a, b in zip(ms, l2ms)]}
         log(f"[bench] mirror mode: {[round(x, 1) for x in ms]} ms per step, "
             f"_l2norm_(prev_u, u) after it {[round(x, 1) for x in l2ms]} ms")
+        # the same loop with host-write tracking: after the first step only u
+        # comes down (u, prev_u and Mmask stay equal to their device copies)
+        u_plain = lib.arr("u").copy()
+        lib.set_write_tracking(True)
+        lib.arr("u")[...] = u0
+        tms, tl2 = [], []
+        for s in range(args.mirror_steps):
+            lib.set(numberOfTimeStep=s + 1)
+            t = time.perf_counter()
+            lib.seg_time_step()
+            tms.append((time.perf_counter() - t) * 1e3)
+            t = time.perf_counter()
+            lib.l2norm("prev_u", "u")
+            tl2.append((time.perf_counter() - t) * 1e3)
+        mirror["write_tracking"] = {
+            "active": bool(lib.write_tracking_state() & 1),
+            "ms_per_step": tms, "l2norm_ms_after_step": tl2,
+            "driver_loop_ms_per_step": [a + b for a, b in zip(tms, tl2)],
+            "bitwise_equal_to_untracked": bool(np.array_equal(lib.arr("u"), u_plain))}
+        lib.set_write_tracking(False)
+        log(f"[bench] mirror mode with write tracking: {[round(x, 1) for x in tms]} ms per "
+            f"step, _l2norm_ after it {[round(x, 1) for x in tl2]} ms, bitwise "
+            f"{mirror['write_tracking']['bitwise_equal_to_untracked']}")
         lib.set_resident(True)
 
     red1_avg_s = prof["red1_ms"] / max(prof["red1_launches"], 1) / 1e3

@@ -167,6 +167,25 @@ bool seg4dUpload(double *hostArray);
 bool seg4dDownload(double *hostArray);
 /* Tells the library that the caller wrote a library array directly. */
 void seg4dHostWrote(double *hostArray);
+/* Mirror-mode host-write tracking of u, prev_u and Mmask (default off; also
+ * switched on by SEG4D_TRACK_WRITES=1 in the environment at the first
+ * _allocateMem_).  After a mirror _segTimeStep_ the three arrays equal their
+ * device copies; the library keeps them read-only (mprotect) and catches the
+ * caller's first write to each with a SIGSEGV handler that chains to the one
+ * installed before it.  _segTimeStep_ and _l2norm_ then upload only arrays
+ * written since, which removes three of the five volume transfers of the
+ * reference's loop (segmentation4D.c:1398-1411: the step, then
+ * _l2norm_(prev_u, u)).  Caveats: a system call that writes into a clean
+ * array (read(2), fread into it) fails with EFAULT -- call seg4dHostWrote on
+ * it first; a SIGSEGV handler installed after the library's turns the
+ * protection off for later steps but must not meet a write to an array
+ * still protected -- call seg4dSetWriteTracking(0) before installing one.
+ * Only pinned, page-aligned arrays of at least 64 pages are tracked.
+ * Returns the previous setting. */
+int seg4dSetWriteTracking(int on);
+/* Bit 0: tracking on and its handler the installed one; bits 1, 2, 3: u,
+ * prev_u, Mmask known equal to their device copies (no upload needed). */
+int seg4dWriteTrackingState(void);
 /* 0 silences the reference's per-step printf lines (default 1). */
 void seg4dSetVerbose(int on);
 const char *seg4dLastError(void);

@@ -1,0 +1,164 @@
+"""Mirror-mode host-write tracking (seg4dSetWriteTracking) on the GPU, through
+the C ABI.
+
+The reference's driver loop (segmentation4D.c:1398-1411: _segTimeStep_, then
+_l2norm_(prev_u, u)) with tracking on must give what it gives without it:
+bitwise u and prev_u and equal sweep counts against the oracle, while host
+writes to u, Mmask and prev_u between calls are seen and uploaded.  A failed
+step drops what tracking knew.  In the refined form with the cubic start,
+mirror mode with tracking keeps the device history like resident mode and so
+equals resident mode bitwise.
+"""
+import numpy as np
+import pytest
+
+from harness import params, zeros
+
+FACES = ("e", "w", "n", "s", "t", "b", "fw", "bw")
+KW = dict(h=1.0, tau=0.5, eps2=1e-6, K=1000.0, omega_c=1.8, omega_sigma=1.3, sigma=0.5,
+          theta_img=1.0, rho_thr=0.0, maxNoGSIteration_sigma=1000.0)
+DIMS = (40, 30, 20, 6)  # each padded array 1.8 MiB (tracking needs >= 64 pages)
+ALL_CLEAN = 1 | 2 | 4 | 8
+pytestmark = pytest.mark.gpu
+
+
+def _generated(lib, dims, seed):
+    spec = lib.gen_spec(0, seed)
+    spec.nx, spec.ny, spec.nz, spec.nt = dims
+    spec.axes[0][0], spec.axes[0][1], spec.axes[0][2] = dims[0] / 4, dims[1] / 5, dims[2] / 4
+    spec.axes[1][0], spec.axes[1][1], spec.axes[1][2] = dims[0] / 8, dims[1] / 8, dims[2] / 6
+    shape = (dims[3] + 2, dims[2] + 2, dims[1] + 2, dims[0] + 2)
+    img, u0 = np.zeros(shape), np.zeros(shape)
+    lib.generate(spec, img, u0)
+    return shape, img, u0
+
+
+def _prepare(lib, oracle, seed):
+    shape, img, u0 = _generated(lib, DIMS, seed)
+    lib.allocate(*DIMS)
+    lib.set(gsTol=0.0, maxNoGSIteration=100000, **KW)
+    lib.arr("i0")[...] = img
+    lib.arr("u")[...] = 0.0
+    lib.arr("Mmask")[...] = 1.0
+    lib.gf("i0", "u")
+    g = [lib.arr("g_" + f).copy() for f in FACES]
+    tol = 1e-12 * oracle.sumsq_interior(params(shape, **KW), u0)
+    lib.set(gsTol=tol)
+    return shape, u0, g, params(shape, gsTol=tol, maxNoGSIteration=100000, **KW)
+
+
+def test_tracked_driver_loop_vs_oracle(seglib, oracle):
+    print("[gpu] tracking: driver loop with host writes vs oracle", flush=True)
+    lib = seglib
+    old = lib.set_write_tracking(True)
+    try:
+        shape, u0, g, p = _prepare(lib, oracle, 23)
+        assert lib.write_tracking_state() & 1, "tracking handler not installed"
+        mask = np.ones(shape)
+        u, prev, coef = u0.copy(), np.zeros(shape), zeros(shape, 8)
+        lib.arr("u")[...] = u0
+        lib.arr("Mmask")[...] = 1.0
+        rng = np.random.default_rng(5)
+        for st in range(1, 7):
+            if st == 3:  # host writes Mmask: seen, uploaded by the step
+                m = np.ones(shape)
+                m[:, :, :, : shape[3] // 3] = 0.0
+                lib.arr("Mmask")[...] = m
+                mask = m
+                assert not lib.write_tracking_state() & 8
+            if st == 4:  # host writes u: seen, uploaded by the step
+                d = np.zeros(shape)
+                d[1:-1, 1:-1, 1:-1, 1:-1] = 0.01 * rng.uniform(size=tuple(s - 2 for s in shape))
+                lib.arr("u")[...] += d
+                u += d
+                assert not lib.write_tracking_state() & 2
+            lib.set(numberOfTimeStep=st)
+            z_g, _ = lib.seg_time_step()
+            z_o, _ = oracle.seg_time_step(p, u, prev, g, coef, mask)
+            assert z_g == z_o, f"sweeps differ at step {st}"
+            assert np.array_equal(lib.arr("u"), u), f"u differs at step {st}"
+            assert np.array_equal(lib.arr("prev_u"), prev), f"prev_u differs at step {st}"
+            assert lib.write_tracking_state() == ALL_CLEAN, st
+            if st == 5:  # host writes prev_u after the step: _l2norm_ must see it
+                lib.arr("prev_u")[2, 3, 4, 5:9] += 1e-3
+                prev[2, 3, 4, 5:9] += 1e-3
+                assert not lib.write_tracking_state() & 4
+            r_g = lib.l2norm("prev_u", "u")
+            r_o = oracle.l2norm(p, prev, u)
+            assert r_g == pytest.approx(r_o, rel=1e-12, abs=1e-300), f"l2norm at step {st}"
+            assert lib.write_tracking_state() == ALL_CLEAN, st
+    finally:
+        lib.set_write_tracking(old)
+        lib.deallocate()
+
+
+def test_tracked_failed_step(seglib, oracle):
+    """A step made to fail at its final wait (after k_unpack ran) leaves the
+    host u as it was and nothing known clean; the next step uploads again
+    and equals the oracle's step from u^n."""
+    print("[gpu] tracking: injected failure", flush=True)
+    lib = seglib
+    old = lib.set_write_tracking(True)
+    try:
+        shape, u0, g, p = _prepare(lib, oracle, 29)
+        lib.arr("u")[...] = u0
+        lib.seg_time_step()
+        u1 = lib.arr("u").copy()
+        assert lib.write_tracking_state() == ALL_CLEAN
+        lib.inject_step_failure(2)
+        assert lib.lib._segTimeStep_() is False
+        assert lib.write_tracking_state() == 1
+        assert np.array_equal(lib.arr("u"), u1)
+        z2, _ = lib.seg_time_step()
+        u, prev = u1.copy(), np.zeros(shape)
+        zo, _ = oracle.seg_time_step(p, u, prev, g, zeros(shape, 8), np.ones(shape))
+        assert z2 == zo
+        assert np.array_equal(lib.arr("u"), u)
+        assert np.array_equal(lib.arr("prev_u"), u1)
+    finally:
+        lib.set_write_tracking(old)
+        lib.deallocate()
+
+
+def test_tracked_mirror_equals_resident_refined(seglib, oracle):
+    """FP32-refined form, cubic start: with tracking the mirror loop keeps
+    the device history between steps, as resident mode does, and equals it
+    bitwise (u, sweeps, rounds)."""
+    print("[gpu] tracking: refined cubic mirror vs resident", flush=True)
+    lib = seglib
+    old = lib.set_write_tracking(True)
+    runs = {}
+    try:
+        for mode in ("resident", "mirror_tracked"):
+            shape, u0, g, p = _prepare(lib, oracle, 31)
+            lib.set_precision(1)
+            lib.set_warm_start(4)
+            lib.set_relative_tolerance(1e-6)
+            lib.arr("u")[...] = u0
+            lib.arr("Mmask")[...] = 1.0
+            if mode == "resident":
+                lib.upload("u")
+                lib.upload("Mmask")
+                lib.set_resident(True)
+            out = []
+            for st in range(1, 6):
+                lib.set(numberOfTimeStep=st)
+                z, _ = lib.seg_time_step()
+                r = lib.l2norm("prev_u", "u")
+                out.append((z, lib.lib.seg4dLastRounds(), r))
+            if mode == "resident":
+                lib.set_resident(False)
+                lib.download("u")
+            else:
+                assert lib.write_tracking_state() == ALL_CLEAN
+            runs[mode] = (out, lib.arr("u").copy())
+            lib.set_relative_tolerance(0.0)
+            lib.set_precision(0)
+            lib.set_warm_start(0)
+            lib.deallocate()
+        assert runs["resident"][0] == runs["mirror_tracked"][0]
+        assert np.array_equal(runs["resident"][1], runs["mirror_tracked"][1])
+    finally:
+        lib.set_resident(False)
+        lib.set_write_tracking(old)
+        lib.deallocate()
