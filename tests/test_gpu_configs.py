@@ -173,6 +173,20 @@ def test_cfg0_twenty_steps_and_driver(seglib, oracle, tmp_path):
     assert "20 steps" in r.stdout
     got = np.fromfile(out, dtype=np.uint8).reshape(m_o.shape)
     assert np.array_equal(got.astype(bool), m_o)
+    # the same driver in mirror mode with host-write tracking switched on from
+    # the environment (no code change): same mask, same l2 norms per step
+    out2 = tmp_path / "mask_mirror.raw"
+    env = dict(os.environ, SEG4D_TRACK_WRITES="1")
+    r2 = subprocess.run([os.path.join(BIN, "seg4d_run"), "0", "20", "1e-6", "--mask", str(out2)],
+                        capture_output=True, text=True, timeout=300, env=env)
+    assert r2.returncode == 0, r2.stderr[-2000:]
+
+    def norms(text):
+        return [ln for ln in text.splitlines() if ln.startswith("l2 norm of difference")]
+
+    assert norms(r2.stdout) == norms(r.stdout)
+    got2 = np.fromfile(out2, dtype=np.uint8).reshape(m_o.shape)
+    assert np.array_equal(got2.astype(bool), m_o)
 
 
 def test_cfg1_edge_indicator_and_two_steps(seglib, oracle):
