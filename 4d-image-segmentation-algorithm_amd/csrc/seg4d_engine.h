@@ -268,6 +268,7 @@ public:
     double* hscal_ = nullptr;    // pinned host scalar
     SorCtl* ctl_ = nullptr;      // device control block
     SorCtl* hctl_ = nullptr;     // pinned host mirror
+    SorCtl* hctl_dev_ = nullptr; // its device alias (the reduce writes it directly)
     hipEvent_t ev_ = nullptr;
     hipStream_t side_ = nullptr;   // upload_side's stream
     hipEvent_t side_ev_ = nullptr;
@@ -282,6 +283,7 @@ public:
     float* db_ = nullptr;        // correction, black
     SorCtl* ctl2_ = nullptr;     // control block of the correction solves
     SorCtl* hctl2_ = nullptr;
+    SorCtl* hctl2_dev_ = nullptr;
     // device ball list (count, offset, cx, cy, cz), re-sent when it changes
     int put_balls(const Balls& b, BallList* out);
     char* balls_buf_ = nullptr;

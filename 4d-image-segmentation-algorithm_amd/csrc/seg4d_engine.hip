@@ -79,6 +79,17 @@ int Engine::check(hipError_t e, const char* what) {
 
 int Engine::launch_check(const char* what) { return check(hipGetLastError(), what); }
 
+// Device alias of a page-locked host control block (null if the runtime
+// gives none: the block is then copied in the stream).
+static SorCtl* host_alias(SorCtl* h) {
+    void* d = nullptr;
+    if (!h || hipHostGetDevicePointer(&d, h, 0) != hipSuccess || !d) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return static_cast<SorCtl*>(d);
+}
+
 int Engine::init(int I, int J, int K, int L, bool seg, bool mcf) {
     release();
     if (I < 3 || J < 3 || K < 3 || L < 3) {
@@ -149,6 +160,7 @@ int Engine::init(int I, int J, int K, int L, bool seg, bool mcf) {
               "hipHostMalloc"))
         return -1;
     std::memset(hctl_, 0, sizeof(SorCtl));
+    hctl_dev_ = host_alias(hctl_);
     if (check(hipStreamSynchronize(st_), "init sync")) return -1;
     ready_ = true;
     return 0;
@@ -197,8 +209,8 @@ void Engine::release() {
     gbar_ = nullptr;
     hgbar_ = nullptr;
     db_ = nullptr;
-    ctl2_ = hctl2_ = nullptr;
-    ctl_ = hctl_ = nullptr;
+    ctl2_ = hctl2_ = hctl2_dev_ = nullptr;
+    ctl_ = hctl_ = hctl_dev_ = nullptr;
     for (auto e : pev_) hipEventDestroy(e);
     for (auto e : evq_) hipEventDestroy(e);
     evq_.clear();
@@ -635,6 +647,7 @@ struct SorJob {
     T* black;
     SorCtl* ctl;        // device control block (tol already set)
     SorCtl* hctl;       // pinned host mirror
+    SorCtl* hctl_dev = nullptr;  // its device alias (null: copied in the stream instead)
     int cap;
     int kind0;          // profiling kinds kind0 .. kind0+2
 };
@@ -803,9 +816,10 @@ int sor_run(Engine& E, const SorJob<T>& J, int* zout) {
             // two fixed-order levels: chunk sums (many blocks), then the test
             k_resid_chunks<<<2 * rnc, 1024, 0, st>>>(part_r, part_b, nb, rnc, rch, E.red2_, J.ctl);
             k_reduce_resid<<<1, 1024, 0, st>>>(E.red2_, rnc, E.red2_ + rnc, rnc, J.ctl, nullptr, z,
-                                               J.cap, 1);
+                                               J.cap, 1, J.hctl_dev);
             if (E.launch_check("k_reduce_resid")) return -1;
-            if (E.check(hipMemcpyAsync(J.hctl, J.ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, st),
+            if (!J.hctl_dev &&
+                E.check(hipMemcpyAsync(J.hctl, J.ctl, sizeof(SorCtl), hipMemcpyDeviceToHost, st),
                         "ctl copy"))
                 return -1;
             if (E.check(hipEventRecord(E.evq_[z % depth], st), "event")) return -1;
@@ -859,6 +873,7 @@ int Engine::solve(const StepParams& P, SolveStats* out) {
     J.black = ub_;
     J.ctl = ctl_;
     J.hctl = hctl_;
+    J.hctl_dev = hctl_dev_;
     J.cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
     J.kind0 = 0;
     int z = 0;
@@ -891,6 +906,7 @@ int Engine::ensure_mixed() {
     if (check(hipHostMalloc((void**)&hctl2_, sizeof(SorCtl), hipHostMallocDefault), "hipHostMalloc"))
         return -1;
     std::memset(hctl2_, 0, sizeof(SorCtl));
+    hctl2_dev_ = host_alias(hctl2_);
     return check(hipStreamSynchronize(st_), "mixed alloc");
 }
 
@@ -980,6 +996,7 @@ int Engine::solve_mixed(const StepParams& P, SolveStats* out) {
     J.black = db_;
     J.ctl = ctl2_;
     J.hctl = hctl2_;
+    J.hctl_dev = hctl2_dev_;
     J.cap = P.cap > 0x7fffffffL ? 0x7fffffff : (int)P.cap;
     J.kind0 = 3;
     long sweeps = 0;

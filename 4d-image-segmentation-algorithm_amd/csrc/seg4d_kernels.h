@@ -834,9 +834,14 @@ static __global__ __launch_bounds__(1024) void k_resid_chunks(const double* __re
 
 // Final residual reduction of one SOR iteration and the reference's stopping
 // test `while (R > gsTol && z < maxNoGSIteration)` (segmentation4D.c:309).
+// `host` (device alias of page-locked host memory, or null): the control
+// block is also written there and made visible to the host, so the host
+// reads the decision without a copy in the stream.  A reduce that returns
+// at once after a stop leaves it as the stopping reduce wrote it.
 static __global__ __launch_bounds__(1024) void k_reduce_resid(
     const double* __restrict__ pa, int na, const double* __restrict__ pb, int nbp,
-    SorCtl* ctl, double* __restrict__ hist, int z, int cap, int queued) {
+    SorCtl* ctl, double* __restrict__ hist, int z, int cap, int queued,
+    SorCtl* host = nullptr) {
     // a reduce enqueued ahead of the host's decision (queued != 0) keeps a
     // stopped state: the passes after the stop left stale partials
     if (queued && ctl->stop) return;
@@ -849,6 +854,10 @@ static __global__ __launch_bounds__(1024) void k_reduce_resid(
         if (!cont) {
             ctl->stop = 1;
             ctl->zfinal = z;
+        }
+        if (host) {
+            *host = *ctl;
+            __threadfence_system();
         }
     }
 }
