@@ -10,6 +10,7 @@
 #include <thread>
 
 #include "host_mem.h"
+#include "host_track.h"
 #include "seg4d_engine.h"
 
 #define API extern "C" __attribute__((visibility("default")))
@@ -45,12 +46,53 @@ bool fail(const char* what) {
     return false;
 }
 
-bool need_engine(const char* fn) {
+// Host-write tracking of u, prev_u and exact_u in mirror mode (host_track.h):
+// off unless switched on by mcf4dSetWriteTracking(1) or MCF4D_TRACK_WRITES=1
+// in the environment at the first _allocateMem_mcf.  _mcfTimeStep,
+// _exactSolution_mcf and _l2norm_mcf keep what it knows; every other entry
+// point (need_engine) first makes all three arrays dirty.
+enum { TR_U = 0, TR_PREV = 1, TR_EXACT = 2 };
+int g_track = -1;  // -1: not yet read from the environment
+
+int track_slot(int id) {
+    if (id == s4d::A_U) return TR_U;
+    if (id == s4d::A_PREV) return TR_PREV;
+    if (id == s4d::A_EXACT) return TR_EXACT;
+    return -1;
+}
+
+void track_register() {
+    s4d::track::forget_all();
+    if (g_track != 1 || !u) return;
+    if (!s4d::track::install()) return;
+    const size_t bytes = (size_t)nelem() * sizeof(double);
+    s4d::track::set(TR_U, u, bytes);
+    s4d::track::set(TR_PREV, prev_u, bytes);
+    s4d::track::set(TR_EXACT, exact_u, bytes);
+}
+
+bool tracked_clean(int id) {
+    const int s = track_slot(id);
+    return s >= 0 && !g_resident && s4d::track::clean(s);
+}
+
+void mark_clean(int id) {
+    const int s = track_slot(id);
+    if (s >= 0 && !g_resident) s4d::track::mark_clean(s);
+}
+
+bool need_engine_tracked(const char* fn) {
+    if (g_resident) s4d::track::touch_all();
     if (g_eng && g_eng->ready()) return true;
     std::string m = std::string(fn) + ": library not allocated (call _allocateMem_mcf first)";
     s4d::set_error(m);
     std::fprintf(stderr, "mcf4d: %s\n", m.c_str());
     return false;
+}
+
+bool need_engine(const char* fn) {
+    s4d::track::touch_all();
+    return need_engine_tracked(fn);
 }
 
 int resolve(const double* p) {
@@ -66,7 +108,10 @@ int resolve(const double* p) {
 double* device_of(double* hp, s4d::ArrayId spare, bool upload) {
     const int id = resolve(hp);
     if (id >= 0) {
-        if (upload && !g_resident && g_eng->upload((s4d::ArrayId)id, hp)) return nullptr;
+        if (upload && !g_resident && !tracked_clean(id)) {
+            if (g_eng->upload((s4d::ArrayId)id, hp)) return nullptr;
+            mark_clean(id);
+        }
         return g_eng->full((s4d::ArrayId)id);
     }
     if (g_resident || id <= -10) {
@@ -96,6 +141,7 @@ s4d::StepParams step_params() {
 s4d::HostArrays g_host;
 
 void free_host() {
+    s4d::track::forget_all();
     double** all[] = {&u, &prev_u, &exact_u, &e, &w, &n, &s, &t, &b, &fw, &bw};
     for (double** p : all) {
         g_host.free(*p);
@@ -138,6 +184,11 @@ API bool _allocateMem_mcf(void) {
     if (!g_eng) g_eng = new s4d::Engine();
     if (g_eng->init(iMax, jMax, kMax, lMax, false, true))
         return fail("_allocateMem_mcf: device allocation failed");
+    if (g_track < 0) {
+        const char* ev = std::getenv("MCF4D_TRACK_WRITES");
+        g_track = (ev && ev[0] == '1') ? 1 : 0;
+    }
+    track_register();
     return true;
 }
 
@@ -154,7 +205,7 @@ API bool _deallocateMem_mcf(void) {
 // ---- mcf_4D.c:81-100 ---------------------------------------------------------
 API double _l2norm_mcf(double* a, double* bb) {
     if (!a || !bb) return false;
-    if (!need_engine("_l2norm_mcf")) return 0.0;
+    if (!need_engine_tracked("_l2norm_mcf")) return 0.0;
     double* da = device_of(a, s4d::A_TMP, true);
     double* db = device_of(bb, s4d::A_TMP2, true);
     if (!da || !db) {
@@ -171,22 +222,38 @@ API double _l2norm_mcf(double* a, double* bb) {
 
 // ---- mcf_4D.c:102-205 --------------------------------------------------------
 API bool _mcfTimeStep(void) {
-    if (!need_engine("_mcfTimeStep")) return false;
+    if (!need_engine_tracked("_mcfTimeStep")) return false;
     const double time = tau * numberOfTimeStep;
-    // mirror mode: u up; the reference's prev_u <- u (mcf_4D.c:113) on host
-    // threads while the device computes; u down
-    if (!g_resident && g_eng->upload(s4d::A_U, u)) return fail("_mcfTimeStep: upload");
+    // mirror mode: u up (with write tracking only when the host wrote it);
+    // the reference's prev_u <- u (mcf_4D.c:113) on host threads while the
+    // device computes; u down
+    if (!g_resident && !tracked_clean(s4d::A_U) && g_eng->upload(s4d::A_U, u)) {
+        s4d::track::touch_all();
+        return fail("_mcfTimeStep: upload");
+    }
     std::thread prev_copy;
-    if (!g_resident) prev_copy = std::thread([] { s4d::host_copy(prev_u, u, (size_t)nelem()); });
+    if (!g_resident) {
+        s4d::track::touch(TR_PREV);  // written by the copy
+        prev_copy = std::thread([] { s4d::host_copy(prev_u, u, (size_t)nelem()); });
+    }
     s4d::SolveStats st;
     const int rc = g_eng->mcf_step(step_params(), time, &st);
     if (prev_copy.joinable()) prev_copy.join();
     // a failed step leaves u as it was; in mirror mode prev_u then holds u^n,
     // as after the reference's first statement
-    if (rc) return fail("_mcfTimeStep");
+    if (rc) {
+        s4d::track::touch_all();
+        return fail("_mcfTimeStep");
+    }
     g_last = st;
     if (!g_resident) {
-        if (g_eng->download(s4d::A_U, u)) return fail("_mcfTimeStep: download");
+        if (g_eng->download(s4d::A_U, u)) {
+            s4d::track::touch_all();
+            return fail("_mcfTimeStep: download");
+        }
+        // host and device hold u^(n+1) and u^n alike
+        mark_clean(s4d::A_U);
+        mark_clean(s4d::A_PREV);
     }
     if (g_verbose) {
         std::printf("Residuum is %e\n", st.R);
@@ -219,12 +286,19 @@ API bool _set4DBoundaryToExactValues_mcf(double tt) {
 
 // ---- mcf_4D.c:444-462 --------------------------------------------------------
 API void _exactSolution_mcf(double tt) {
-    if (!need_engine("_exactSolution_mcf")) return;
+    if (!need_engine_tracked("_exactSolution_mcf")) return;
+    s4d::track::touch(TR_EXACT);  // its device copy changes here
     if (g_eng->mcf_exact(g_eng->full(s4d::A_EXACT), h, tt, false)) {
         fail("_exactSolution_mcf");
         return;
     }
-    if (!g_resident && g_eng->download(s4d::A_EXACT, exact_u)) fail("_exactSolution_mcf: download");
+    if (!g_resident) {
+        if (g_eng->download(s4d::A_EXACT, exact_u)) {
+            fail("_exactSolution_mcf: download");
+            return;
+        }
+        mark_clean(s4d::A_EXACT);
+    }
 }
 
 // ---- mcf_4D.c:463-471 --------------------------------------------------------
@@ -242,6 +316,7 @@ API void _copy4DdataFromSrcToDest_mcf(double* src, double* dst) {
 
 // ---- extensions ------------------------------------------------------------------
 API int mcf4dSetResident(int on) {
+    s4d::track::touch_all();
     const int old = g_resident;
     g_resident = on ? 1 : 0;
     return old;
@@ -259,6 +334,17 @@ API bool mcf4dDownload(double* hp) {
     if (id <= -10) return g_eng->download_coef(-(id + 10), hp) == 0 ? true : fail("mcf4dDownload");
     if (id < 0) return fail("mcf4dDownload: not a library array");
     return g_eng->download((s4d::ArrayId)id, hp) == 0 ? true : fail("mcf4dDownload");
+}
+API int mcf4dSetWriteTracking(int on) {
+    const int old = g_track == 1 ? 1 : 0;
+    g_track = on ? 1 : 0;
+    track_register();
+    return old;
+}
+API int mcf4dWriteTrackingState(void) {
+    if (g_track != 1 || !s4d::track::current()) return 0;
+    return 1 | (s4d::track::clean(TR_U) ? 2 : 0) | (s4d::track::clean(TR_PREV) ? 4 : 0) |
+           (s4d::track::clean(TR_EXACT) ? 8 : 0);
 }
 API void mcf4dSetVerbose(int on) { g_verbose = on; }
 API const char* mcf4dLastError(void) { return s4d::last_error(); }

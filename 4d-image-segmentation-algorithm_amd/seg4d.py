@@ -652,6 +652,10 @@ class McfLib(_Lib):
         for fn in ("mcf4dUpload", "mcf4dDownload"):
             getattr(L, fn).restype = c_bool
             getattr(L, fn).argtypes = [POINTER(c_double)]
+        L.mcf4dSetWriteTracking.restype = c_int
+        L.mcf4dSetWriteTracking.argtypes = [c_int]
+        L.mcf4dWriteTrackingState.restype = c_int
+        L.mcf4dWriteTrackingState.argtypes = []
         L.mcf4dSetResident.restype = c_int
         L.mcf4dSetResident.argtypes = [c_int]
         L.mcf4dSetRelativeTolerance.argtypes = [c_double]
@@ -701,6 +705,13 @@ class McfLib(_Lib):
 
     def set_resident(self, on):
         return self.lib.mcf4dSetResident(1 if on else 0)
+
+    def set_write_tracking(self, on):
+        return self.lib.mcf4dSetWriteTracking(1 if on else 0)
+
+    def write_tracking_state(self):
+        """Bit 0: tracking active; bits 1-3: u, prev_u, exact_u clean."""
+        return self.lib.mcf4dWriteTrackingState()
 
     def upload(self, name):
         self.check(self.lib.mcf4dUpload(self.ptr(name)), "mcf4dUpload")

@@ -55,6 +55,18 @@ int mcf4dSetResident(int on);
 void mcf4dSetRelativeTolerance(double rel);
 bool mcf4dUpload(double *hostArray);
 bool mcf4dDownload(double *hostArray);
+/* Mirror-mode host-write tracking of u, prev_u and exact_u, as
+ * seg4dSetWriteTracking (include/seg4d.h; environment MCF4D_TRACK_WRITES=1
+ * at the first _allocateMem_mcf): _mcfTimeStep, _exactSolution_mcf and
+ * _l2norm_mcf upload only arrays the host wrote since they last matched
+ * their device copies, so the reference's loop (mcf_4D.c:500-516: step,
+ * exact solution, _l2norm_mcf(exact_u, u)) moves u and exact_u down and
+ * nothing up.  Same caveats: a system call writing into a clean array fails
+ * with EFAULT; a SIGSEGV handler installed later must come after
+ * mcf4dSetWriteTracking(0).  Returns the previous setting. */
+int mcf4dSetWriteTracking(int on);
+/* Bit 0: tracking active; bits 1, 2, 3: u, prev_u, exact_u clean. */
+int mcf4dWriteTrackingState(void);
 void mcf4dSetVerbose(int on);
 const char *mcf4dLastError(void);
 int mcf4dLastSweeps(void);

@@ -85,6 +85,13 @@ def test_mcf_driver_program(oracle):
     assert len(got) == 3
     # printed with 7 significant digits (%e)
     assert np.allclose(got, want, rtol=1e-6, atol=0)
+    # the same run with host-write tracking switched on from the environment
+    print("[gpu] mcf4d_run 12 3, MCF4D_TRACK_WRITES=1", flush=True)
+    r2 = subprocess.run([exe, "12", "3"], capture_output=True, text=True, timeout=300,
+                        env=dict(os.environ, MCF4D_TRACK_WRITES="1"))
+    assert r2.returncode == 0, r2.stderr[-2000:]
+    errs = [x for x in r.stdout.splitlines() if x.startswith("l2 norm of error at time")]
+    assert [x for x in r2.stdout.splitlines() if x.startswith("l2 norm of error at time")] == errs
     print("[gpu] mcf4d_run eoc 10 2", flush=True)
     r = subprocess.run([exe, "eoc", "10", "2"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr[-2000:]

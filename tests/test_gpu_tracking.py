@@ -7,7 +7,9 @@ bitwise u and prev_u and equal sweep counts against the oracle, while host
 writes to u, Mmask and prev_u between calls are seen and uploaded.  A failed
 step drops what tracking knew.  In the refined form with the cubic start,
 mirror mode with tracking keeps the device history like resident mode and so
-equals resident mode bitwise.
+equals resident mode bitwise.  The MCF library's loop (mcf_4D.c:500-516:
+step, exact solution, _l2norm_mcf(exact_u, u)) with tracking against the
+oracle's MCF steps.
 """
 import numpy as np
 import pytest
@@ -160,5 +162,46 @@ def test_tracked_mirror_equals_resident_refined(seglib, oracle):
         assert np.array_equal(runs["resident"][1], runs["mirror_tracked"][1])
     finally:
         lib.set_resident(False)
+        lib.set_write_tracking(old)
+        lib.deallocate()
+
+
+def test_tracked_mcf_loop_vs_oracle(mcflib, oracle):
+    print("[gpu] tracking: MCF loop with a host write vs oracle", flush=True)
+    lib = mcflib
+    n, steps = 20, 5
+    hh = 2.5 / n
+    old = lib.set_write_tracking(True)
+    try:
+        lib.set(maxNoGSIteration=1000)
+        lib.allocate(n)
+        lib.set(h=hh, tau=hh * hh, eps2=hh * hh, gsTol=1e-12, omega_c=1.8)
+        assert lib.write_tracking_state() & 1, "tracking handler not installed"
+        shape = (n + 2,) * 4
+        p = params(shape, h=hh, tau=hh * hh, eps2=hh * hh, gsTol=1e-12, omega_c=1.8,
+                   maxNoGSIteration=1000)
+        u, prev, coef, ex = np.zeros(shape), np.zeros(shape), zeros(shape, 8), np.zeros(shape)
+        lib.exact(0.0)
+        lib.copy("exact_u", "u")
+        oracle.mcf_exact(p, u, 0.0)
+        assert np.array_equal(lib.arr("u"), u)
+        for st in range(1, steps + 1):
+            if st == 3:  # host writes u between calls: seen, uploaded by the step
+                lib.arr("u")[5, 6, 7, 3:9] += 1e-3
+                u[5, 6, 7, 3:9] += 1e-3
+                assert not lib.write_tracking_state() & 2
+            lib.set(numberOfTimeStep=st)
+            lib.time_step()
+            oracle.mcf_time_step(p, u, prev, coef, st)
+            assert np.array_equal(lib.arr("u"), u), f"u differs at step {st}"
+            assert np.array_equal(lib.arr("prev_u"), prev), f"prev_u differs at step {st}"
+            lib.exact(st * hh * hh)
+            oracle.mcf_exact(p, ex, st * hh * hh)
+            assert np.array_equal(lib.arr("exact_u"), ex)
+            assert lib.write_tracking_state() == ALL_CLEAN, st
+            e_g = lib.l2norm("exact_u", "u")
+            assert e_g == pytest.approx(oracle.l2norm(p, ex, u), rel=1e-12, abs=1e-300)
+            assert lib.write_tracking_state() == ALL_CLEAN, st
+    finally:
         lib.set_write_tracking(old)
         lib.deallocate()
