@@ -53,9 +53,20 @@ __host__ __device__ __forceinline__ int cidx(const Geo& G, int j, int k, int l, 
 // number of concurrent streams).
 constexpr int kOpnd = 9;
 
+// Order of the interior (k,l) planes in the operand blocks: frame pairs
+// (1,2), (3,4), ... one after the other; within a pair k-major with the two
+// frames of one k adjacent (a last unpaired frame: k-major alone).  The
+// passes' automatic launch order works through chunks of 2 frames k plane by
+// k plane, so each XCD then reads one contiguous operand stream instead of
+// two (red pass at 256x256x128x16: 1.256 -> 1.238 ms in a timing copy).
+__host__ __device__ __forceinline__ long opnd_plane(const Geo& G, int k, int l) {
+    const int l0 = (l - 1) & ~1;                            // first frame of the pair - 1
+    const int w = (G.L - 2 - l0) < 2 ? (G.L - 2 - l0) : 2;  // frames in the pair
+    return (long)l0 * (G.K - 2) + (long)(k - 1) * w + ((l - 1) & 1);
+}
+
 __device__ __forceinline__ long opnd_index(const Geo& G, int k, int l, int t) {
-    const long plane = (long)(l - 1) * (G.K - 2) + (k - 1);
-    return plane * G.RP * kOpnd + (long)(t >> 6) * (64 * kOpnd) + (t & 63);
+    return opnd_plane(G, k, l) * G.RP * kOpnd + (long)(t >> 6) * (64 * kOpnd) + (t & 63);
 }
 
 // Split operands of the FP32-refined form (written by its set-up kernel): the
@@ -71,8 +82,7 @@ __device__ __forceinline__ long opnd_index(const Geo& G, int k, int l, int t) {
 constexpr int kSplit = 10;
 
 __device__ __forceinline__ long split_index(const Geo& G, int k, int l, int t) {
-    const long plane = (long)(l - 1) * (G.K - 2) + (k - 1);
-    return plane * G.RP * kSplit + (long)(t >> 6) * (64 * kSplit) + (t & 63);
+    return opnd_plane(G, k, l) * G.RP * kSplit + (long)(t >> 6) * (64 * kSplit) + (t & 63);
 }
 
 // FP32 coefficient of a split high word (exact: 20 mantissa bits fit in 23).
@@ -696,17 +706,16 @@ __device__ __forceinline__ void sor_block(
     const bool rowok = kok && j <= G.J - 2;
     double acc = 0.0;
     const int base = cidx(G, j, k, 0, m);  // l = 0
-    // operand block of (k, l = 1, t); the next l plane is ostride further
-    const long ob_off = opnd_index(G, k, 1, t);
-    const T* ob0 = opnd + ob_off;
-    const long ostride = (long)(G.K - 2) * G.RP * kOpnd;
-    float* def0 = (RESID >= 3) ? defect_ob + ob_off + 8 * 64 : nullptr;
-    // split operands: high words in the FP32 block (same index), low words and
-    // prev in the FP64 buffer's [10][64] word groups, sstride words per plane
-    const float* hi0 = SPLIT ? defect_ob + ob_off : nullptr;
-    const unsigned* ow0 =
-        SPLIT ? reinterpret_cast<const unsigned*>(opnd) + split_index(G, k, 1, t) : nullptr;
-    const long sstride = (long)(G.K - 2) * G.RP * kSplit;
+    // operand block of (k, l, t): the plane's offset (opnd_plane) plus the
+    // point's place in its 64-point group; split operands: high words in the
+    // FP32 block (same index), low words and prev in the FP64 buffer's
+    // [10][64] word groups
+    const long og = (long)(t >> 6) * (64 * kOpnd) + (t & 63);
+    const long sg = (long)(t >> 6) * (64 * kSplit) + (t & 63);
+    const long opl = (long)G.RP * kOpnd, spl = (long)G.RP * kSplit;
+    auto ooff = [&](int lq) { return opnd_plane(G, k, lq) * opl + og; };
+    auto soff = [&](int lq) { return opnd_plane(G, k, lq) * spl + sg; };
+    const unsigned* ow = SPLIT ? reinterpret_cast<const unsigned*>(opnd) : nullptr;
     // window w[q] = other colour at column m, row l-1+q
     T w[U + 2];
     w[0] = rowok ? other<T, CORR>(y, dy, base + (l0 - 1) * G.pw) : T(0);
@@ -722,15 +731,15 @@ __device__ __forceinline__ void sor_block(
             w[q + 2] = rowok ? other<T, CORR>(y, dy, pp + G.pw) : T(0);
             const int sft = colour ^ ((j + k + lq) & 1);
             const int i = 2 * m + sft;
-            load_point<T, NTO, CORR, SPLIT>(G, ob0 + (lq - 1) * ostride, y, xin, pp, sft,
+            load_point<T, NTO, CORR, SPLIT>(G, opnd + ooff(lq), y, xin, pp, sft,
                                             rowok && i >= 1 && i <= G.I - 2, P[q], dx, dy,
-                                            SPLIT ? hi0 + (lq - 1) * ostride : nullptr,
-                                            SPLIT ? ow0 + (lq - 1) * sstride : nullptr);
+                                            SPLIT ? defect_ob + ooff(lq) : nullptr,
+                                            SPLIT ? ow + soff(lq) : nullptr);
         }
 #pragma unroll
         for (int q = 0; q < U; q++)
             update_point<T, RESID>(P[q], w[q], w[q + 1], w[q + 2], c, om, h4, xout,
-                                   def0 ? def0 + (l + q - 1) * ostride : nullptr, acc);
+                                   RESID >= 3 ? defect_ob + ooff(l + q) + 8 * 64 : nullptr, acc);
         w[0] = w[U];
         w[1] = w[U + 1];
     }
@@ -740,12 +749,12 @@ __device__ __forceinline__ void sor_block(
         w[2] = rowok ? other<T, CORR>(y, dy, pp + G.pw) : T(0);
         const int sft = colour ^ ((j + k + l) & 1);
         const int i = 2 * m + sft;
-        load_point<T, NTO, CORR, SPLIT>(G, ob0 + (l - 1) * ostride, y, xin, pp, sft,
+        load_point<T, NTO, CORR, SPLIT>(G, opnd + ooff(l), y, xin, pp, sft,
                                         rowok && i >= 1 && i <= G.I - 2, P, dx, dy,
-                                        SPLIT ? hi0 + (l - 1) * ostride : nullptr,
-                                        SPLIT ? ow0 + (l - 1) * sstride : nullptr);
+                                        SPLIT ? defect_ob + ooff(l) : nullptr,
+                                        SPLIT ? ow + soff(l) : nullptr);
         update_point<T, RESID>(P, w[0], w[1], w[2], c, om, h4, xout,
-                               def0 ? def0 + (l - 1) * ostride : nullptr, acc);
+                               RESID >= 3 ? defect_ob + ooff(l) + 8 * 64 : nullptr, acc);
         w[0] = w[1];
         w[1] = w[2];
     }
@@ -1204,7 +1213,7 @@ static __global__ __launch_bounds__(kBlock) void k_unpack_gated(
 // ----------------------------------------------------------------------------
 __device__ __forceinline__ long heat_prev_index(const Geo& G, int c, int k, int l, int t) {
     const long nplane = (long)(G.K - 2) * (G.L - 2);
-    return ((long)c * nplane + (long)(l - 1) * (G.K - 2) + (k - 1)) * G.RP + t;
+    return ((long)c * nplane + opnd_plane(G, k, l)) * G.RP + t;
 }
 
 static __global__ __launch_bounds__(kBlock) void k_heat_pack(Geo G, const double* __restrict__ d,
