@@ -535,6 +535,10 @@ def main():
             "frac_of_copy": achieved / copy_gbps if copy_gbps else None,
         },
         "hbm_gbps_all_sweeps": sweep_bytes / (sweep_ms / 1e3) / 1e9 if sweep_ms > 0 else None,
+        # average HIP-event time per launch of each FP64 pass kind (launches
+        # queued after a solve's stop are not counted)
+        "pass_avg_ms": {k: prof[k + "_ms"] / prof[k + "_launches"] if prof[k + "_launches"] else None
+                        for k in ("red0", "red1", "black")},
         "time_to_tolerance_s": total,
         "sweeps_per_step": sweeps,
         "setup_ms_per_step": prof["setup_ms"] / max(prof["setup_launches"], 1),
