@@ -167,55 +167,55 @@ int Engine::init(int I, int J, int K, int L, bool seg, bool mcf) {
 }
 
 void Engine::release() {
-    if (st_) hipStreamSynchronize(st_);
+    if (st_) (void)hipStreamSynchronize(st_);
     for (auto& p : full_) {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
     }
     for (int c = 0; c < 2; c++) {
-        if (op_[c]) hipFree(op_[c]);
-        if (ur_[c]) hipFree(ur_[c]);
+        if (op_[c]) (void)hipFree(op_[c]);
+        if (ur_[c]) (void)hipFree(ur_[c]);
         op_[c] = ur_[c] = nullptr;
     }
-    if (ub_) hipFree(ub_);
-    if (part_) hipFree(part_);
-    if (dscal_) hipFree(dscal_);
-    if (red2_) hipFree(red2_);
+    if (ub_) (void)hipFree(ub_);
+    if (part_) (void)hipFree(part_);
+    if (dscal_) (void)hipFree(dscal_);
+    if (red2_) (void)hipFree(red2_);
     red2_ = nullptr;
-    if (ctl_) hipFree(ctl_);
-    if (hctl_) hipHostFree(hctl_);
-    if (hscal_) hipHostFree(hscal_);
+    if (ctl_) (void)hipFree(ctl_);
+    if (hctl_) (void)hipHostFree(hctl_);
+    if (hscal_) (void)hipHostFree(hscal_);
     ub_ = part_ = dscal_ = hscal_ = nullptr;
     for (int c = 0; c < 2; c++) {
-        if (op32_[c]) hipFree(op32_[c]);
-        if (dr_[c]) hipFree(dr_[c]);
+        if (op32_[c]) (void)hipFree(op32_[c]);
+        if (dr_[c]) (void)hipFree(dr_[c]);
         op32_[c] = dr_[c] = nullptr;
     }
-    if (db_) hipFree(db_);
-    if (hp_) hipFree(hp_);
+    if (db_) (void)hipFree(db_);
+    if (hp_) (void)hipFree(hp_);
     hp_ = nullptr;
-    if (prev2_) hipFree(prev2_);
-    if (prev3_) hipFree(prev3_);
+    if (prev2_) (void)hipFree(prev2_);
+    if (prev3_) (void)hipFree(prev3_);
     prev2_ = prev3_ = nullptr;
     hist_ok_ = hist2_ok_ = hist3_ok_ = false;
-    if (balls_buf_) hipFree(balls_buf_);
+    if (balls_buf_) (void)hipFree(balls_buf_);
     balls_buf_ = nullptr;
     balls_cap_ = 0;
     balls_valid_ = false;
-    if (ctl2_) hipFree(ctl2_);
-    if (hctl2_) hipHostFree(hctl2_);
-    if (gbar_) hipFree(gbar_);
-    if (hgbar_) hipHostFree(hgbar_);
+    if (ctl2_) (void)hipFree(ctl2_);
+    if (hctl2_) (void)hipHostFree(hctl2_);
+    if (gbar_) (void)hipFree(gbar_);
+    if (hgbar_) (void)hipHostFree(hgbar_);
     gbar_ = nullptr;
     hgbar_ = nullptr;
     db_ = nullptr;
     ctl2_ = hctl2_ = hctl2_dev_ = nullptr;
     ctl_ = hctl_ = hctl_dev_ = nullptr;
-    for (auto e : pev_) hipEventDestroy(e);
-    for (auto e : evq_) hipEventDestroy(e);
+    for (auto e : pev_) (void)hipEventDestroy(e);
+    for (auto e : evq_) (void)hipEventDestroy(e);
     evq_.clear();
     pev_.clear();
-    if (ev_) hipEventDestroy(ev_);
+    if (ev_) (void)hipEventDestroy(ev_);
     ev_ = nullptr;
     if (side_) {
         (void)hipStreamSynchronize(side_);
@@ -225,7 +225,7 @@ void Engine::release() {
     side_ = nullptr;
     side_ev_ = nullptr;
     side_pending_ = false;
-    if (st_ && own_st_) hipStreamDestroy(st_);
+    if (st_ && own_st_) (void)hipStreamDestroy(st_);
     st_ = nullptr;
     own_st_ = true;
     ready_ = false;
@@ -493,14 +493,14 @@ int Engine::setup(const StepParams& P, bool with_g, bool write_prev, SorCtl* ctl
         full_[A_U] = freed;  // written whole by k_unpack at the end of the step
     }
     if (prof.on) {
-        hipEventRecord(e1, st_);
-        hipEventSynchronize(e1);
+        (void)hipEventRecord(e1, st_);
+        (void)hipEventSynchronize(e1);
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
         prof.setup_ms += ms;
         prof.setup_launches++;
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     }
     // one partial per (run, k, l), summed in two fixed-order levels
     const long np = (long)PM.gx * PM.gk * PM.gl;
@@ -1107,8 +1107,8 @@ int Engine::put_balls(const Balls& b, BallList* out) {
                 return check(hipErrorInvalidValue, "ball list offsets");
         if (bytes > balls_cap_) {
             if (balls_buf_) {
-                hipStreamSynchronize(st_);
-                hipFree(balls_buf_);
+                (void)hipStreamSynchronize(st_);
+                (void)hipFree(balls_buf_);
                 balls_buf_ = nullptr;
                 balls_cap_ = 0;
             }
@@ -1246,9 +1246,9 @@ int Engine::mass(const double* d, double h, double* out) {
 int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof.on) {
-        hipEventCreate(&e0);
-        hipEventCreate(&e1);
-        hipEventRecord(e0, st_);
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, st_);
     }
     if (setup(P, true, true, nullptr, precision_ == 1)) {
         undo_rotation();  // set-up may fail after it rotated (its reduction)
@@ -1271,14 +1271,14 @@ int Engine::seg_step(const StepParams& P, const Balls* balls, SolveStats* st) {
         return -1;
     }
     if (prof.on) {
-        hipEventRecord(e1, st_);
-        hipEventSynchronize(e1);
+        (void)hipEventRecord(e1, st_);
+        (void)hipEventSynchronize(e1);
         float ms = 0.f;
-        hipEventElapsedTime(&ms, e0, e1);
+        (void)hipEventElapsedTime(&ms, e0, e1);
         prof.step_ms += ms;
         prof.steps++;
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     }
     if (check(hipStreamSynchronize(st_), "seg_step")) {
         undo_rotation();
