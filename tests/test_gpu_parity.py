@@ -110,6 +110,39 @@ def test_mcf_golden_gpu(mcflib):
         lib.deallocate()
 
 
+@pytest.mark.parametrize("n", [9, 11])
+def test_mcf_odd_sizes_vs_oracle(mcflib, oracle, n):
+    """MCF steps on odd n (an odd number of frames: the operand blocks' last
+    frame pair has one frame) and 5 steps: u after every step bitwise the
+    oracle's, the error norm to 1e-13 of it (mcf_4D.c:102-205, 444-462)."""
+    print(f"[gpu] mcf odd n = {n}", flush=True)
+    lib = mcflib
+    hh = 2.5 / n
+    lib.set(maxNoGSIteration=1000)
+    lib.allocate(n)
+    lib.set(h=hh, tau=hh * hh, eps2=hh * hh, gsTol=1e-12, omega_c=1.8)
+    try:
+        shape = (n + 2,) * 4
+        p = params(shape, h=hh, tau=hh * hh, eps2=hh * hh, gsTol=1e-12, omega_c=1.8,
+                   maxNoGSIteration=1000)
+        u, prev, coef, ex = np.zeros(shape), np.zeros(shape), zeros(shape, 8), np.zeros(shape)
+        lib.exact(0.0)
+        lib.copy("exact_u", "u")
+        oracle.mcf_exact(p, u, 0.0)
+        assert np.array_equal(lib.arr("u"), u)
+        for st in range(1, 6):
+            lib.set(numberOfTimeStep=st)
+            z_g, _ = lib.time_step()
+            z_o, _ = oracle.mcf_time_step(p, u, prev, coef, st)
+            assert z_g == z_o, f"sweeps differ at step {st}"
+            assert np.array_equal(lib.arr("u"), u), f"u differs at step {st}"
+            lib.exact(st * hh * hh)
+            oracle.mcf_exact(p, ex, st * hh * hh)
+            assert lib.l2norm("exact_u", "u") == pytest.approx(oracle.l2norm(p, ex, u), rel=1e-13)
+    finally:
+        lib.deallocate()
+
+
 def _random_volume(rng, shape):
     a = rng.uniform(0.0, 1.0, size=shape)
     a[0] = a[-1] = 0
