@@ -180,7 +180,8 @@ void seg4dHostWrote(double *hostArray);
  * it first; a SIGSEGV handler installed after the library's turns the
  * protection off for later steps but must not meet a write to an array
  * still protected -- call seg4dSetWriteTracking(0) before installing one.
- * Only pinned, page-aligned arrays of at least 64 pages are tracked.
+ * Only page-aligned arrays of at least 64 pages are tracked (the library's
+ * page-locked host arrays are page-aligned).
  * Returns the previous setting. */
 int seg4dSetWriteTracking(int on);
 /* Bit 0: tracking on and its handler the installed one; bits 1, 2, 3: u,
