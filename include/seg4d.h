@@ -173,7 +173,7 @@ void seg4dHostWrote(double *hostArray);
  * device copies; the library keeps them read-only (mprotect) and catches the
  * caller's first write to each with a SIGSEGV handler that chains to the one
  * installed before it.  _segTimeStep_ and _l2norm_ then upload only arrays
- * written since, which removes three of the five volume transfers of the
+ * written since, which removes four of the five volume transfers of the
  * reference's loop (segmentation4D.c:1398-1411: the step, then
  * _l2norm_(prev_u, u)).  Caveats: a system call that writes into a clean
  * array (read(2), fread into it) fails with EFAULT -- call seg4dHostWrote on
